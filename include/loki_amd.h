@@ -1,0 +1,206 @@
+/*
+ * loki_amd.h -- C-ABI of the MI355X (gfx950) fast-folding (FFA) hot path.
+ *
+ * Drop-in boundary for pravirkr/loki's CUDA backend (libculoki).  Every entry
+ * point below names the reference interface it replaces (file:line in
+ * pravirkr/loki v0.0.3).  Plain pointers and sizes only; no C++ or torch
+ * types.  All functions return LOKI_OK (0) or an error code; the message of
+ * the last failure on the calling thread is returned by loki_last_error().
+ *
+ * Device entry points (*_device) take device pointers and a hipStream_t
+ * passed as void*; they are asynchronous on that stream (the reference
+ * synchronises per level: kernels_cuda.cu:1081-1082 -- this library does not).
+ * Host entry points (*_host) take host pointers and are synchronous, like the
+ * reference's host-span overloads.
+ */
+#ifndef LOKI_AMD_H
+#define LOKI_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LOKI_OK 0
+#define LOKI_ERR_INVALID 1  /* bad argument / config (reference: DetailedException) */
+#define LOKI_ERR_HIP 2      /* HIP runtime failure (reference: CudaException)     */
+#define LOKI_ERR_INTERNAL 3
+
+#define LOKI_FOLD_TIME 0
+#define LOKI_FOLD_FOURIER 1
+
+typedef struct loki_config_s* loki_config_t;
+typedef struct loki_plan_s* loki_plan_t;
+typedef struct loki_ffa_s* loki_ffa_t;
+
+/* PulsarSearchConfig constructor arguments (configs.hpp:25-47).
+ * param_limits: nparams (min, max) pairs, order (..., jerk, accel, freq).
+ * bseg_brute / bseg_ffa: 0 selects the reference default (configs.cpp:295-302). */
+typedef struct {
+    size_t nsamps;
+    double tsamp;
+    size_t nbins;
+    double eta;
+    const double* param_limits;
+    size_t nparams;
+    double ducy_max;
+    double wtsp;
+    int use_fourier;
+    int nthreads;
+    double max_process_memory_gb;
+    double octave_scale;
+    size_t nbins_max;
+    size_t nbins_min_lossy_bf;
+    size_t bseg_brute;
+    size_t bseg_ffa;
+    double snr_min;
+    size_t max_passing_candidates;
+    size_t prune_poly_order;
+    double p_orb_min;
+    double m_c_max;
+    double m_p_min;
+    double minimum_snap_cells;
+    int use_conservative_tile;
+} loki_config_params;
+
+/* Derived config scalars (configs.cpp:77-83, getters configs.hpp:57-128). */
+typedef struct {
+    size_t nbins_f;
+    size_t nparams;
+    size_t bseg_brute;
+    size_t bseg_ffa;
+    size_t niters_ffa;
+    size_t n_scoring_widths;
+    double tobs;
+    double tseg_brute;
+    double tseg_ffa;
+    double f_min;
+    double f_max;
+} loki_config_info;
+
+/* FFAPlan sizes (plans.hpp:36-190). */
+typedef struct {
+    size_t n_params;
+    size_t n_levels;
+    size_t brute_fold_size;
+    size_t fold_size;
+    size_t fold_size_time;
+    size_t buffer_size;
+    size_t buffer_size_time;
+    size_t coord_size;
+    float buffer_memory_usage;
+    float coord_memory_usage;
+    float gflops;
+    float gflops_return_in_time;
+} loki_plan_info;
+
+/* One FFAPlan level: segment_lens / nsegments / tsegments / ncoords /
+ * ncoords_offsets / param_counts / dparams / dparams_actual / fold_shapes
+ * entries of that level (plans.cpp:280-320, 491-518). */
+typedef struct {
+    size_t segment_len;
+    size_t nsegments;
+    double tsegment;
+    size_t ncoords;
+    size_t ncoords_offset;
+    float ncoords_lb;
+    size_t param_counts[5];
+    size_t param_cart_strides[5];
+    double dparams[5];
+    double dparams_actual[5];
+    size_t fold_shape[8];
+    size_t fold_shape_time[8];
+    size_t fold_ndim;
+} loki_plan_level;
+
+/* Message of the last failure on this thread ("" if none). */
+const char* loki_last_error(void);
+/* Number of visible HIP devices. */
+int loki_device_count(int* count);
+
+/* --- PulsarSearchConfig (configs.hpp:22-164) --------------------------- */
+int loki_config_create(const loki_config_params* params, loki_config_t* out);
+void loki_config_destroy(loki_config_t cfg);
+int loki_config_info_get(loki_config_t cfg, loki_config_info* out);
+/* Copies the constructor arguments back (param_limits points into cfg). */
+int loki_config_params_get(loki_config_t cfg, loki_config_params* out);
+/* score_widths (configs.cpp:82-83); *n receives the count. */
+int loki_config_scoring_widths(loki_config_t cfg, size_t* out, size_t cap, size_t* n);
+/* kind 0: get_dparams, 1: get_dparams_f, 2: get_dparams_actual
+ * (configs.cpp:152-174); out has nparams entries. */
+int loki_config_dparams(loki_config_t cfg, double tseg_cur, int kind, double* out);
+/* get_param_grid_count (configs.cpp:175-184). */
+int loki_config_param_grid_count(loki_config_t cfg, double tseg_cur, size_t* out);
+/* generate_box_width_trials (score.cpp:390-405). */
+int loki_generate_box_width_trials(size_t nbins, double ducy_max, double wtsp, size_t* out,
+                                   size_t cap, size_t* n);
+
+/* --- FFAPlan (plans.hpp:21-201, FFAPlan<float> / FFAPlan<ComplexType>) -- */
+int loki_plan_create(loki_config_t cfg, int fold_type, loki_plan_t* out);
+void loki_plan_destroy(loki_plan_t plan);
+int loki_plan_info_get(loki_plan_t plan, loki_plan_info* out);
+int loki_plan_level_get(loki_plan_t plan, size_t level, loki_plan_level* out);
+/* compute_param_grid(level)[param] (plans.cpp:152-162); *n receives count. */
+int loki_plan_param_grid(loki_plan_t plan, size_t level, size_t param, double* out, size_t cap,
+                         size_t* n);
+/* resolve_coordinates[_freq] (plans.cpp:64-150), evaluated by the FP64 HIP
+ * resolver on device_id: coord_size records of 16 B {u32 i_tail, f32
+ * shift_tail, u32 i_head, f32 shift_head} (nparams >= 2) or 8 B {u32 idx,
+ * f32 shift} (nparams == 1), levels concatenated at ncoords_offsets, level 0
+ * zero-filled.  Host output. */
+int loki_plan_resolve_coordinates_host(loki_plan_t plan, void* coords, size_t coords_bytes,
+                                       int device_id);
+
+/* --- FFA executor (ffa.hpp:100-188 FFACUDA<T>) ------------------------- */
+int loki_ffa_create(loki_config_t cfg, int fold_type, int device_id, loki_ffa_t* out);
+void loki_ffa_destroy(loki_ffa_t ffa);
+int loki_ffa_plan(loki_ffa_t ffa, loki_plan_t* out_plan_view); /* borrowed; do not destroy */
+/* FFACUDA::execute(device spans, stream) (ffa.hpp:130-136): fold holds
+ * buffer_size elements; the final level is its first fold_size elements. */
+int loki_ffa_execute_device(loki_ffa_t ffa, const float* ts_e, const float* ts_v, void* fold,
+                            size_t fold_len, void* stream);
+/* FFACUDA::execute(host spans) (ffa.hpp:125-129): synchronous. */
+int loki_ffa_execute_host(loki_ffa_t ffa, const float* ts_e, const float* ts_v, size_t nsamps,
+                          void* fold, size_t fold_len);
+/* compute_ffa_scores_cuda (ffa_cuda.cu:670-692) on resident buffers, with
+ * explicit widths (widths[i] in [1, nbins)).  scores: [ncoords_final, nw]
+ * or [ncoords_final] if find_max. */
+int loki_ffa_scores_device(loki_ffa_t ffa, const float* ts_e, const float* ts_v,
+                           const size_t* widths, size_t nwidths, float* scores,
+                           size_t scores_len, int find_max, void* stream);
+int loki_ffa_scores_host(loki_ffa_t ffa, const float* ts_e, const float* ts_v, size_t nsamps,
+                         const size_t* widths, size_t nwidths, float* scores, size_t scores_len,
+                         int find_max);
+
+/* --- Brute fold (fold.hpp:78-133 BruteFoldCUDA) ------------------------ */
+/* compute_brute_fold_cuda<float> (fold_cuda.cu:260-277): fold has
+ * (nsamps/segment_len) * nfreqs * 2 * nbins floats. */
+int loki_brute_fold_time_host(const float* ts_e, const float* ts_v, size_t nsamps,
+                              const double* freqs, size_t nfreqs, size_t segment_len,
+                              size_t nbins, double tsamp, double t_ref, float* fold,
+                              int device_id);
+int loki_brute_fold_time_device(const float* ts_e, const float* ts_v, size_t nsamps,
+                                const double* freqs_host, size_t nfreqs, size_t segment_len,
+                                size_t nbins, double tsamp, double t_ref, float* fold,
+                                void* stream);
+
+/* --- Scores (score.hpp:123-165 snr_boxcar_*_cuda) ---------------------- */
+/* folds [nprof, 2, nbins] -> [nprof, nw] (find_max 0) or [nprof] (1). */
+int loki_snr_boxcar_3d_device(const float* folds, size_t nprof, size_t nbins,
+                              const size_t* widths, size_t nwidths, float* scores, int find_max,
+                              void* stream);
+int loki_snr_boxcar_3d_host(const float* folds, size_t nprof, size_t nbins,
+                            const size_t* widths, size_t nwidths, float* scores, int find_max,
+                            int device_id);
+/* folds [nprof, nbins] -> [nprof] max over widths of snr / stdnoise. */
+int loki_snr_boxcar_2d_max_host(const float* folds, size_t nprof, size_t nbins,
+                                const size_t* widths, size_t nwidths, float stdnoise,
+                                float* scores, int device_id);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LOKI_AMD_H */

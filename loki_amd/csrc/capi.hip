@@ -1,0 +1,467 @@
+// extern "C" surface declared in include/loki_amd.h.  Exceptions from the
+// C++ layer become status codes plus a thread-local message.
+#include "../../include/loki_amd.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "ffa.hpp"
+#include "kernels.hpp"
+
+struct loki_config_s {
+    loki_amd::SearchConfig cfg;
+};
+struct loki_plan_s {
+    loki_amd::FFAPlan plan;
+};
+struct loki_ffa_s {
+    loki_amd::FFAExecutor exec;
+    loki_plan_s plan_view;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <typename F> int guarded(F&& fn) {
+    try {
+        fn();
+        g_last_error.clear();
+        return LOKI_OK;
+    } catch (const loki_amd::InvalidArgument& e) {
+        g_last_error = e.what();
+        return LOKI_ERR_INVALID;
+    } catch (const loki_amd::HipError& e) {
+        g_last_error = e.what();
+        return LOKI_ERR_HIP;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "out of host memory";
+        return LOKI_ERR_INTERNAL;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return LOKI_ERR_INTERNAL;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return LOKI_ERR_INTERNAL;
+    }
+}
+
+#define REQ(cond, msg) LOKI_REQUIRE(cond, msg)
+
+loki_amd::FoldKind fold_kind(int t) {
+    REQ(t == LOKI_FOLD_TIME || t == LOKI_FOLD_FOURIER, "fold_type must be 0 (time) or 1 (fourier)");
+    return t == LOKI_FOLD_TIME ? loki_amd::FoldKind::kTime : loki_amd::FoldKind::kFourier;
+}
+
+hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
+
+}  // namespace
+
+extern "C" {
+
+const char* loki_last_error(void) { return g_last_error.c_str(); }
+
+int loki_device_count(int* count) {
+    return guarded([&] {
+        REQ(count != nullptr, "count is null");
+        loki_amd::hip_check(hipGetDeviceCount(count), "hipGetDeviceCount");
+    });
+}
+
+int loki_config_create(const loki_config_params* p, loki_config_t* out) {
+    return guarded([&] {
+        REQ(p != nullptr && out != nullptr, "null argument");
+        REQ(p->nparams >= 1 && p->param_limits != nullptr, "param_limits must be non-empty");
+        loki_amd::SearchConfigArgs a;
+        a.nsamps = p->nsamps;
+        a.tsamp  = p->tsamp;
+        a.nbins  = p->nbins;
+        a.eta    = p->eta;
+        for (size_t i = 0; i < p->nparams; ++i) {
+            a.param_limits.push_back({p->param_limits[2 * i], p->param_limits[2 * i + 1]});
+        }
+        a.ducy_max               = p->ducy_max;
+        a.wtsp                   = p->wtsp;
+        a.use_fourier            = p->use_fourier != 0;
+        a.nthreads               = p->nthreads;
+        a.max_process_memory_gb  = p->max_process_memory_gb;
+        a.octave_scale           = p->octave_scale;
+        a.nbins_max              = p->nbins_max;
+        a.nbins_min_lossy_bf     = p->nbins_min_lossy_bf;
+        if (p->bseg_brute != 0) {
+            a.bseg_brute = p->bseg_brute;
+        }
+        if (p->bseg_ffa != 0) {
+            a.bseg_ffa = p->bseg_ffa;
+        }
+        a.snr_min                = p->snr_min;
+        a.max_passing_candidates = p->max_passing_candidates;
+        a.prune_poly_order       = p->prune_poly_order;
+        a.p_orb_min              = p->p_orb_min;
+        a.m_c_max                = p->m_c_max;
+        a.m_p_min                = p->m_p_min;
+        a.minimum_snap_cells     = p->minimum_snap_cells;
+        a.use_conservative_tile  = p->use_conservative_tile != 0;
+        *out = new loki_config_s{loki_amd::SearchConfig(std::move(a))};
+    });
+}
+
+void loki_config_destroy(loki_config_t cfg) { delete cfg; }
+
+int loki_config_info_get(loki_config_t c, loki_config_info* out) {
+    return guarded([&] {
+        REQ(c != nullptr && out != nullptr, "null argument");
+        const auto& g          = c->cfg;
+        out->nbins_f           = g.get_nbins_f();
+        out->nparams           = g.get_nparams();
+        out->bseg_brute        = g.get_bseg_brute();
+        out->bseg_ffa          = g.get_bseg_ffa();
+        out->niters_ffa        = g.get_niters_ffa();
+        out->n_scoring_widths  = g.get_scoring_widths().size();
+        out->tobs              = g.get_tobs();
+        out->tseg_brute        = g.get_tseg_brute();
+        out->tseg_ffa          = g.get_tseg_ffa();
+        out->f_min             = g.get_f_min();
+        out->f_max             = g.get_f_max();
+    });
+}
+
+int loki_config_params_get(loki_config_t c, loki_config_params* out) {
+    return guarded([&] {
+        REQ(c != nullptr && out != nullptr, "null argument");
+        const auto& a               = c->cfg.args();
+        out->nsamps                 = a.nsamps;
+        out->tsamp                  = a.tsamp;
+        out->nbins                  = a.nbins;
+        out->eta                    = a.eta;
+        out->param_limits           = reinterpret_cast<const double*>(a.param_limits.data());
+        out->nparams                = a.param_limits.size();
+        out->ducy_max               = a.ducy_max;
+        out->wtsp                   = a.wtsp;
+        out->use_fourier            = a.use_fourier ? 1 : 0;
+        out->nthreads               = a.nthreads;
+        out->max_process_memory_gb  = a.max_process_memory_gb;
+        out->octave_scale           = a.octave_scale;
+        out->nbins_max              = a.nbins_max;
+        out->nbins_min_lossy_bf     = a.nbins_min_lossy_bf;
+        out->bseg_brute             = c->cfg.get_bseg_brute();
+        out->bseg_ffa               = c->cfg.get_bseg_ffa();
+        out->snr_min                = a.snr_min;
+        out->max_passing_candidates = a.max_passing_candidates;
+        out->prune_poly_order       = a.prune_poly_order;
+        out->p_orb_min              = a.p_orb_min;
+        out->m_c_max                = a.m_c_max;
+        out->m_p_min                = a.m_p_min;
+        out->minimum_snap_cells     = a.minimum_snap_cells;
+        out->use_conservative_tile  = a.use_conservative_tile ? 1 : 0;
+    });
+}
+
+int loki_config_scoring_widths(loki_config_t c, size_t* out, size_t cap, size_t* n) {
+    return guarded([&] {
+        REQ(c != nullptr && n != nullptr, "null argument");
+        const auto& w = c->cfg.get_scoring_widths();
+        *n            = w.size();
+        REQ(out == nullptr || cap >= w.size(), "output buffer too small");
+        if (out != nullptr) {
+            std::memcpy(out, w.data(), w.size() * sizeof(size_t));
+        }
+    });
+}
+
+int loki_config_dparams(loki_config_t c, double tseg_cur, int kind, double* out) {
+    return guarded([&] {
+        REQ(c != nullptr && out != nullptr, "null argument");
+        std::vector<double> v;
+        if (kind == 0) {
+            v = c->cfg.get_dparams(tseg_cur);
+        } else if (kind == 1) {
+            v = c->cfg.get_dparams_f(tseg_cur);
+        } else if (kind == 2) {
+            v = c->cfg.get_dparams_actual(tseg_cur);
+        } else {
+            REQ(false, "kind must be 0, 1 or 2");
+        }
+        std::memcpy(out, v.data(), v.size() * sizeof(double));
+    });
+}
+
+int loki_config_param_grid_count(loki_config_t c, double tseg_cur, size_t* out) {
+    return guarded([&] {
+        REQ(c != nullptr && out != nullptr, "null argument");
+        const auto v = c->cfg.get_param_grid_count(tseg_cur);
+        std::memcpy(out, v.data(), v.size() * sizeof(size_t));
+    });
+}
+
+int loki_generate_box_width_trials(size_t nbins, double ducy_max, double wtsp, size_t* out,
+                                   size_t cap, size_t* n) {
+    return guarded([&] {
+        REQ(n != nullptr, "null argument");
+        const auto w = loki_amd::generate_box_width_trials(nbins, ducy_max, wtsp);
+        *n           = w.size();
+        REQ(out == nullptr || cap >= w.size(), "output buffer too small");
+        if (out != nullptr) {
+            std::memcpy(out, w.data(), w.size() * sizeof(size_t));
+        }
+    });
+}
+
+int loki_plan_create(loki_config_t cfg, int fold_type, loki_plan_t* out) {
+    return guarded([&] {
+        REQ(cfg != nullptr && out != nullptr, "null argument");
+        *out = new loki_plan_s{loki_amd::FFAPlan(cfg->cfg, fold_kind(fold_type))};
+    });
+}
+
+void loki_plan_destroy(loki_plan_t plan) { delete plan; }
+
+int loki_plan_info_get(loki_plan_t p, loki_plan_info* out) {
+    return guarded([&] {
+        REQ(p != nullptr && out != nullptr, "null argument");
+        const auto& pl              = p->plan;
+        out->n_params               = pl.n_params();
+        out->n_levels               = pl.n_levels();
+        out->brute_fold_size        = pl.brute_fold_size();
+        out->fold_size              = pl.fold_size();
+        out->fold_size_time         = pl.fold_size_time();
+        out->buffer_size            = pl.buffer_size();
+        out->buffer_size_time       = pl.buffer_size_time();
+        out->coord_size             = pl.coord_size();
+        out->buffer_memory_usage    = pl.buffer_memory_usage();
+        out->coord_memory_usage     = pl.coord_memory_usage();
+        out->gflops                 = pl.gflops(false);
+        out->gflops_return_in_time  = pl.gflops(true);
+    });
+}
+
+int loki_plan_level_get(loki_plan_t p, size_t l, loki_plan_level* out) {
+    return guarded([&] {
+        REQ(p != nullptr && out != nullptr, "null argument");
+        const auto& pl = p->plan;
+        REQ(l < pl.n_levels(), "level out of range");
+        std::memset(out, 0, sizeof(*out));
+        out->segment_len    = pl.segment_lens()[l];
+        out->nsegments      = pl.nsegments()[l];
+        out->tsegment       = pl.tsegments()[l];
+        out->ncoords        = pl.ncoords()[l];
+        out->ncoords_offset = pl.ncoords_offsets64()[l];
+        out->ncoords_lb     = pl.ncoords_lb()[l];
+        for (size_t i = 0; i < pl.n_params(); ++i) {
+            out->param_counts[i]       = pl.param_counts()[l][i];
+            out->param_cart_strides[i] = pl.param_cart_strides()[l][i];
+            out->dparams[i]            = pl.dparams()[l][i];
+            out->dparams_actual[i]     = pl.dparams_actual()[l][i];
+        }
+        out->fold_ndim = pl.fold_shapes()[l].size();
+        for (size_t i = 0; i < out->fold_ndim; ++i) {
+            out->fold_shape[i]      = pl.fold_shapes()[l][i];
+            out->fold_shape_time[i] = pl.fold_shapes_time()[l][i];
+        }
+    });
+}
+
+int loki_plan_param_grid(loki_plan_t p, size_t level, size_t param, double* out, size_t cap,
+                         size_t* n) {
+    return guarded([&] {
+        REQ(p != nullptr && n != nullptr, "null argument");
+        const auto grid = p->plan.compute_param_grid(level);
+        REQ(param < grid.size(), "param out of range");
+        *n = grid[param].size();
+        REQ(out == nullptr || cap >= grid[param].size(), "output buffer too small");
+        if (out != nullptr) {
+            std::memcpy(out, grid[param].data(), grid[param].size() * sizeof(double));
+        }
+    });
+}
+
+int loki_plan_resolve_coordinates_host(loki_plan_t p, void* coords, size_t coords_bytes,
+                                       int device_id) {
+    return guarded([&] {
+        REQ(p != nullptr && coords != nullptr, "null argument");
+        const auto& pl       = p->plan;
+        const size_t rec     = pl.n_params() == 1 ? sizeof(loki_amd::FFACoordFreq)
+                                                  : sizeof(loki_amd::FFACoord);
+        const size_t need    = pl.coord_size() * rec;
+        REQ(coords_bytes >= need, "coords must have size >= coord_size records");
+        loki_amd::DeviceGuard g(device_id);
+        loki_amd::FFAExecutor exec(pl.config(), loki_amd::FoldKind::kTime, device_id);
+        loki_amd::DeviceBuffer d(need);
+        exec.resolve_coordinates(d.get(), nullptr);
+        loki_amd::hip_check(hipDeviceSynchronize(), "resolve");
+        const size_t l0 = pl.ncoords()[0] * rec;
+        std::memset(coords, 0, l0);
+        loki_amd::hip_check(hipMemcpy(static_cast<char*>(coords) + l0,
+                                      static_cast<char*>(d.get()) + l0, need - l0,
+                                      hipMemcpyDeviceToHost),
+                            "copy coords");
+    });
+}
+
+int loki_ffa_create(loki_config_t cfg, int fold_type, int device_id, loki_ffa_t* out) {
+    return guarded([&] {
+        REQ(cfg != nullptr && out != nullptr, "null argument");
+        auto kind = fold_kind(fold_type);
+        loki_amd::FFAExecutor exec(cfg->cfg, kind, device_id);
+        loki_amd::FFAPlan plan(cfg->cfg, kind);
+        *out = new loki_ffa_s{std::move(exec), loki_plan_s{std::move(plan)}};
+    });
+}
+
+void loki_ffa_destroy(loki_ffa_t ffa) { delete ffa; }
+
+int loki_ffa_plan(loki_ffa_t ffa, loki_plan_t* out) {
+    return guarded([&] {
+        REQ(ffa != nullptr && out != nullptr, "null argument");
+        *out = &ffa->plan_view;
+    });
+}
+
+int loki_ffa_execute_device(loki_ffa_t ffa, const float* ts_e, const float* ts_v, void* fold,
+                            size_t fold_len, void* stream) {
+    return guarded([&] {
+        REQ(ffa != nullptr, "null ffa");
+        ffa->exec.execute(ts_e, ts_v, fold, fold_len, as_stream(stream));
+    });
+}
+
+int loki_ffa_execute_host(loki_ffa_t ffa, const float* ts_e, const float* ts_v, size_t nsamps,
+                          void* fold, size_t fold_len) {
+    return guarded([&] {
+        REQ(ffa != nullptr && ts_e != nullptr && ts_v != nullptr && fold != nullptr,
+            "null argument");
+        const auto& pl = ffa->exec.plan();
+        REQ(nsamps == pl.config().get_nsamps(), "FFA::execute: ts_e must have size nsamps");
+        REQ(fold_len == pl.buffer_size(), "FFA::execute: fold must have size buffer_size");
+        loki_amd::DeviceGuard g(ffa->exec.device_id());
+        loki_amd::DeviceBuffer de(nsamps * sizeof(float)), dv(nsamps * sizeof(float));
+        loki_amd::DeviceBuffer df(fold_len * sizeof(float));
+        loki_amd::hip_check(hipMemcpy(de.get(), ts_e, nsamps * 4, hipMemcpyHostToDevice), "H2D");
+        loki_amd::hip_check(hipMemcpy(dv.get(), ts_v, nsamps * 4, hipMemcpyHostToDevice), "H2D");
+        ffa->exec.execute(de.as<float>(), dv.as<float>(), df.get(), fold_len, nullptr);
+        loki_amd::hip_check(hipMemcpy(fold, df.get(), pl.fold_size() * sizeof(float),
+                                      hipMemcpyDeviceToHost),
+                            "D2H");
+    });
+}
+
+int loki_ffa_scores_device(loki_ffa_t ffa, const float* ts_e, const float* ts_v,
+                           const size_t* widths, size_t nwidths, float* scores,
+                           size_t scores_len, int find_max, void* stream) {
+    return guarded([&] {
+        REQ(ffa != nullptr && widths != nullptr, "null argument");
+        ffa->exec.execute_scores(ts_e, ts_v, std::span<const size_t>(widths, nwidths), scores,
+                                 scores_len, find_max != 0, as_stream(stream));
+    });
+}
+
+int loki_ffa_scores_host(loki_ffa_t ffa, const float* ts_e, const float* ts_v, size_t nsamps,
+                         const size_t* widths, size_t nwidths, float* scores, size_t scores_len,
+                         int find_max) {
+    return guarded([&] {
+        REQ(ffa != nullptr && ts_e != nullptr && ts_v != nullptr && scores != nullptr &&
+                widths != nullptr,
+            "null argument");
+        const auto& pl = ffa->exec.plan();
+        REQ(nsamps == pl.config().get_nsamps(), "FFA::execute: ts_e must have size nsamps");
+        loki_amd::DeviceGuard g(ffa->exec.device_id());
+        loki_amd::DeviceBuffer de(nsamps * 4), dv(nsamps * 4), ds(scores_len * 4);
+        loki_amd::hip_check(hipMemcpy(de.get(), ts_e, nsamps * 4, hipMemcpyHostToDevice), "H2D");
+        loki_amd::hip_check(hipMemcpy(dv.get(), ts_v, nsamps * 4, hipMemcpyHostToDevice), "H2D");
+        ffa->exec.execute_scores(de.as<float>(), dv.as<float>(),
+                                 std::span<const size_t>(widths, nwidths), ds.as<float>(),
+                                 scores_len, find_max != 0, nullptr);
+        loki_amd::hip_check(hipMemcpy(scores, ds.get(), scores_len * 4, hipMemcpyDeviceToHost),
+                            "D2H");
+    });
+}
+
+int loki_brute_fold_time_host(const float* ts_e, const float* ts_v, size_t nsamps,
+                              const double* freqs, size_t nfreqs, size_t segment_len,
+                              size_t nbins, double tsamp, double t_ref, float* fold,
+                              int device_id) {
+    return guarded([&] {
+        REQ(ts_e != nullptr && ts_v != nullptr && freqs != nullptr && fold != nullptr,
+            "null argument");
+        REQ(segment_len > 0 && nsamps % segment_len == 0,
+            "BruteFold: Number of samples is not a multiple of segment length");
+        loki_amd::DeviceGuard g(device_id);
+        const size_t out_len = (nsamps / segment_len) * nfreqs * 2 * nbins;
+        loki_amd::DeviceBuffer de(nsamps * 4), dv(nsamps * 4), df(out_len * 4);
+        loki_amd::hip_check(hipMemcpy(de.get(), ts_e, nsamps * 4, hipMemcpyHostToDevice), "H2D");
+        loki_amd::hip_check(hipMemcpy(dv.get(), ts_v, nsamps * 4, hipMemcpyHostToDevice), "H2D");
+        loki_amd::brute_fold_time_device(de.as<float>(), dv.as<float>(), nsamps,
+                                         std::span<const double>(freqs, nfreqs), segment_len,
+                                         nbins, tsamp, t_ref, df.as<float>(), nullptr);
+        loki_amd::hip_check(hipMemcpy(fold, df.get(), out_len * 4, hipMemcpyDeviceToHost), "D2H");
+    });
+}
+
+int loki_brute_fold_time_device(const float* ts_e, const float* ts_v, size_t nsamps,
+                                const double* freqs_host, size_t nfreqs, size_t segment_len,
+                                size_t nbins, double tsamp, double t_ref, float* fold,
+                                void* stream) {
+    return guarded([&] {
+        REQ(freqs_host != nullptr && fold != nullptr, "null argument");
+        loki_amd::brute_fold_time_device(ts_e, ts_v, nsamps,
+                                         std::span<const double>(freqs_host, nfreqs),
+                                         segment_len, nbins, tsamp, t_ref, fold,
+                                         as_stream(stream));
+    });
+}
+
+int loki_snr_boxcar_3d_device(const float* folds, size_t nprof, size_t nbins,
+                              const size_t* widths, size_t nwidths, float* scores, int find_max,
+                              void* stream) {
+    return guarded([&] {
+        REQ(folds != nullptr && scores != nullptr && widths != nullptr, "null argument");
+        const auto wt = loki_amd::kern::make_width_table(widths, nwidths, nbins);
+        loki_amd::kern::snr_boxcar_3d(folds, nprof, static_cast<uint32_t>(nbins), wt, scores,
+                                      find_max != 0, as_stream(stream));
+        loki_amd::hip_check(hipGetLastError(), "score launch");
+    });
+}
+
+int loki_snr_boxcar_3d_host(const float* folds, size_t nprof, size_t nbins,
+                            const size_t* widths, size_t nwidths, float* scores, int find_max,
+                            int device_id) {
+    return guarded([&] {
+        REQ(folds != nullptr && scores != nullptr && widths != nullptr, "null argument");
+        const auto wt = loki_amd::kern::make_width_table(widths, nwidths, nbins);
+        loki_amd::DeviceGuard g(device_id);
+        const size_t in_len  = nprof * 2 * nbins;
+        const size_t out_len = find_max ? nprof : nprof * nwidths;
+        loki_amd::DeviceBuffer di(in_len * 4), dout(out_len * 4);
+        loki_amd::hip_check(hipMemcpy(di.get(), folds, in_len * 4, hipMemcpyHostToDevice), "H2D");
+        loki_amd::kern::snr_boxcar_3d(di.as<float>(), nprof, static_cast<uint32_t>(nbins), wt,
+                                      dout.as<float>(), find_max != 0, nullptr);
+        loki_amd::hip_check(hipGetLastError(), "score launch");
+        loki_amd::hip_check(hipMemcpy(scores, dout.get(), out_len * 4, hipMemcpyDeviceToHost),
+                            "D2H");
+    });
+}
+
+int loki_snr_boxcar_2d_max_host(const float* folds, size_t nprof, size_t nbins,
+                                const size_t* widths, size_t nwidths, float stdnoise,
+                                float* scores, int device_id) {
+    return guarded([&] {
+        REQ(folds != nullptr && scores != nullptr && widths != nullptr, "null argument");
+        const auto wt = loki_amd::kern::make_width_table(widths, nwidths, nbins);
+        loki_amd::DeviceGuard g(device_id);
+        loki_amd::DeviceBuffer di(nprof * nbins * 4), dout(nprof * 4);
+        loki_amd::hip_check(hipMemcpy(di.get(), folds, nprof * nbins * 4, hipMemcpyHostToDevice),
+                            "H2D");
+        loki_amd::kern::snr_boxcar_2d_max(di.as<float>(), nprof, static_cast<uint32_t>(nbins),
+                                          wt, stdnoise, dout.as<float>(), nullptr);
+        loki_amd::hip_check(hipGetLastError(), "score launch");
+        loki_amd::hip_check(hipMemcpy(scores, dout.get(), nprof * 4, hipMemcpyDeviceToHost),
+                            "D2H");
+    });
+}
+
+}  // extern "C"

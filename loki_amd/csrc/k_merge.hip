@@ -1,0 +1,217 @@
+// Time-domain FFA merge: one launch per level, no host synchronisation.
+//
+//   out[s, c, q, j] = in[2s, i_tail(c), q, (j - S_t) mod n]
+//                   + in[2s+1, i_head(c), q, (j - S_h) mod n]
+// with S = u32(shift + 0.5f), n -> 0 (kernels.cpp:71-146, 722-940;
+// kernels_cuda.cu:280-383).  Freq-only levels use S_t = 0 and
+// i_tail = i_head = idx.  One FP32 add per output element: bit-identical to
+// the reference CPU order.
+//
+// Layout: a row (2 * nbins floats = e then v) is handled by nbins/2 lanes,
+// each producing one float4 of output.  A rotated float4 straddles at most
+// two aligned float4 chunks of the parent row, so each lane loads two
+// dwordx4 per side and funnels them; the parent rows of a block's tile of
+// consecutive coordinates overlap heavily and are served by L1/L2.  Blocks are
+// remapped so each XCD works on a contiguous range of tiles (parent reuse
+// stays inside one L2).  All offsets are 64-bit.
+#include "kernels.hpp"
+
+namespace loki_amd::kern {
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t nblocks) {
+    // Workgroups are dispatched round-robin over the 8 XCDs; give XCD x the
+    // contiguous logical range [x*per(+rem), ...).
+    constexpr uint32_t kXcd = 8;
+    const uint32_t per = nblocks / kXcd;
+    const uint32_t rem = nblocks % kXcd;
+    const uint32_t xcd = bid % kXcd;
+    const uint32_t loc = bid / kXcd;
+    return xcd < rem ? xcd * (per + 1) + loc : rem * (per + 1) + (xcd - rem) * per + loc;
+}
+
+__device__ __forceinline__ float4 funnel4(const float4 a, const float4 b, uint32_t r) {
+    float4 o;
+    o.x = r == 0 ? a.x : (r == 1 ? a.y : (r == 2 ? a.z : a.w));
+    o.y = r == 0 ? a.y : (r == 1 ? a.z : (r == 2 ? a.w : b.x));
+    o.z = r == 0 ? a.z : (r == 1 ? a.w : (r == 2 ? b.x : b.y));
+    o.w = r == 0 ? a.w : (r == 1 ? b.x : (r == 2 ? b.y : b.z));
+    return o;
+}
+
+__device__ __forceinline__ uint32_t shift_int(float s, uint32_t nbins) {
+    uint32_t k = static_cast<uint32_t>(s + 0.5F);
+    return k == nbins ? 0U : k;
+}
+
+// Rotated float4 (output bins 4*i4 .. 4*i4+3) of one component row.
+template <int NB>
+__device__ __forceinline__ float4 rotated4(const float4* __restrict__ row, uint32_t i4,
+                                           uint32_t S) {
+    constexpr uint32_t C4 = NB / 4;
+    const uint32_t e0 = (4U * i4 + NB - S) & (NB - 1);
+    const uint32_t c0 = e0 >> 2;
+    const uint32_t c1 = (c0 + 1) & (C4 - 1);
+    const float4 a    = row[c0];
+    const float4 b    = row[c1];
+    return funnel4(a, b, e0 & 3U);
+}
+
+template <int NB, bool POLY, int ITERS>
+__global__ void __launch_bounds__(kBlock)
+k_merge_v4(const float* __restrict__ in, float* __restrict__ out,
+           const FFACoord* __restrict__ coords, const FFACoordFreq* __restrict__ cf,
+           uint64_t ncur, uint64_t nprev, uint32_t tiles_per_seg, uint32_t nblocks) {
+    constexpr uint32_t C4   = NB / 4;        // float4 chunks per component
+    constexpr uint32_t LPR  = 2 * C4;        // chunks per row
+    constexpr uint32_t LANES = LPR < kBlock ? LPR : kBlock;
+    constexpr uint32_t ROWS = kBlock / LANES;
+    constexpr uint32_t TILE = ROWS * ITERS;
+    const uint32_t bid  = xcd_remap(blockIdx.x, nblocks);
+    const uint64_t seg  = bid / tiles_per_seg;
+    const uint64_t tile = bid % tiles_per_seg;
+    const uint32_t rsub = threadIdx.x / LANES;
+    const uint32_t lch  = threadIdx.x % LANES;
+    const uint64_t in_seg_t = (2 * seg) * nprev;
+    const uint64_t in_seg_h = (2 * seg + 1) * nprev;
+#pragma unroll
+    for (int it = 0; it < ITERS; ++it) {
+        const uint64_t c = tile * TILE + static_cast<uint64_t>(it) * ROWS + rsub;
+        if (c >= ncur) {
+            break;
+        }
+        uint64_t it_row, ih_row;
+        uint32_t St, Sh;
+        if constexpr (POLY) {
+            const FFACoord cd = coords[c];
+            it_row = in_seg_t + cd.i_tail;
+            ih_row = in_seg_h + cd.i_head;
+            St     = shift_int(cd.shift_tail, NB);
+            Sh     = shift_int(cd.shift_head, NB);
+        } else {
+            const FFACoordFreq cd = cf[c];
+            it_row = in_seg_t + cd.idx;
+            ih_row = in_seg_h + cd.idx;
+            St     = 0;
+            Sh     = shift_int(cd.shift, NB);
+        }
+        const float4* __restrict__ trow = reinterpret_cast<const float4*>(in + it_row * (2 * NB));
+        const float4* __restrict__ hrow = reinterpret_cast<const float4*>(in + ih_row * (2 * NB));
+        float4* __restrict__ orow =
+            reinterpret_cast<float4*>(out + (seg * ncur + c) * (2 * NB));
+        for (uint32_t ch = lch; ch < LPR; ch += LANES) {
+            const uint32_t q  = ch / C4;
+            const uint32_t i4 = ch % C4;
+            float4 tv;
+            if constexpr (POLY) {
+                tv = rotated4<NB>(trow + q * C4, i4, St);
+            } else {
+                tv = trow[ch];
+            }
+            const float4 hv = rotated4<NB>(hrow + q * C4, i4, Sh);
+            float4 o;
+            o.x = tv.x + hv.x;
+            o.y = tv.y + hv.y;
+            o.z = tv.z + hv.z;
+            o.w = tv.w + hv.w;
+            orow[ch] = o;
+        }
+    }
+}
+
+// Any nbins: one lane per output element pair (e and v of bin j).
+template <bool POLY>
+__global__ void __launch_bounds__(kBlock)
+k_merge_scalar(const float* __restrict__ in, float* __restrict__ out,
+               const FFACoord* __restrict__ coords, const FFACoordFreq* __restrict__ cf,
+               uint64_t ncur, uint64_t nprev, uint64_t nseg, uint32_t nbins) {
+    const uint64_t total = nseg * ncur * nbins;
+    for (uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
+         t += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const uint32_t j   = static_cast<uint32_t>(t % nbins);
+        const uint64_t r   = t / nbins;
+        const uint64_t c   = r % ncur;
+        const uint64_t seg = r / ncur;
+        uint64_t it_row, ih_row;
+        uint32_t St, Sh;
+        if constexpr (POLY) {
+            const FFACoord cd = coords[c];
+            it_row = (2 * seg) * nprev + cd.i_tail;
+            ih_row = (2 * seg + 1) * nprev + cd.i_head;
+            St     = shift_int(cd.shift_tail, nbins);
+            Sh     = shift_int(cd.shift_head, nbins);
+        } else {
+            const FFACoordFreq cd = cf[c];
+            it_row = (2 * seg) * nprev + cd.idx;
+            ih_row = (2 * seg + 1) * nprev + cd.idx;
+            St     = 0;
+            Sh     = shift_int(cd.shift, nbins);
+        }
+        const uint32_t jt = j < St ? j + nbins - St : j - St;
+        const uint32_t jh = j < Sh ? j + nbins - Sh : j - Sh;
+        const float* __restrict__ tr = in + it_row * 2 * nbins;
+        const float* __restrict__ hr = in + ih_row * 2 * nbins;
+        float* __restrict__ o        = out + (seg * ncur + c) * 2 * nbins;
+        o[j]         = tr[jt] + hr[jh];
+        o[nbins + j] = tr[nbins + jt] + hr[nbins + jh];
+    }
+}
+
+template <int NB, bool POLY>
+void launch_v4(const float* in, float* out, const FFACoord* coords, const FFACoordFreq* cf,
+               uint64_t ncur, uint64_t nprev, uint64_t nseg, hipStream_t s) {
+    constexpr uint32_t LPR   = NB / 2;
+    constexpr uint32_t LANES = LPR < kBlock ? LPR : kBlock;
+    constexpr uint32_t ROWS  = kBlock / LANES;
+    constexpr int ITERS      = ROWS >= 64 ? 1 : static_cast<int>(64 / ROWS);
+    constexpr uint32_t TILE  = ROWS * ITERS;
+    const uint64_t tiles     = (ncur + TILE - 1) / TILE;
+    const uint64_t blocks    = tiles * nseg;
+    LOKI_REQUIRE(tiles < (1ULL << 31) && blocks < (1ULL << 31), "merge grid too large");
+    k_merge_v4<NB, POLY, ITERS><<<static_cast<unsigned>(blocks), kBlock, 0, s>>>(
+        in, out, coords, cf, ncur, nprev, static_cast<uint32_t>(tiles),
+        static_cast<uint32_t>(blocks));
+}
+
+template <bool POLY>
+void dispatch(const float* in, float* out, const FFACoord* coords, const FFACoordFreq* cf,
+              uint64_t ncur, uint64_t nprev, uint64_t nseg, uint32_t nbins, hipStream_t s) {
+    switch (nbins) {
+    case 4: return launch_v4<4, POLY>(in, out, coords, cf, ncur, nprev, nseg, s);
+    case 8: return launch_v4<8, POLY>(in, out, coords, cf, ncur, nprev, nseg, s);
+    case 16: return launch_v4<16, POLY>(in, out, coords, cf, ncur, nprev, nseg, s);
+    case 32: return launch_v4<32, POLY>(in, out, coords, cf, ncur, nprev, nseg, s);
+    case 64: return launch_v4<64, POLY>(in, out, coords, cf, ncur, nprev, nseg, s);
+    case 128: return launch_v4<128, POLY>(in, out, coords, cf, ncur, nprev, nseg, s);
+    case 256: return launch_v4<256, POLY>(in, out, coords, cf, ncur, nprev, nseg, s);
+    case 512: return launch_v4<512, POLY>(in, out, coords, cf, ncur, nprev, nseg, s);
+    case 1024: return launch_v4<1024, POLY>(in, out, coords, cf, ncur, nprev, nseg, s);
+    default: break;
+    }
+    const uint64_t total = nseg * ncur * nbins;
+    const uint64_t want  = (total + kBlock - 1) / kBlock;
+    const unsigned grid  = static_cast<unsigned>(want < 65536 * 8 ? want : 65536 * 8);
+    k_merge_scalar<POLY><<<grid, kBlock, 0, s>>>(in, out, coords, cf, ncur, nprev, nseg, nbins);
+}
+
+}  // namespace
+
+void merge_time(const float* in, float* out, const FFACoord* coords,
+                const FFACoordFreq* coords_freq, uint64_t ncur, uint64_t nprev, uint64_t nseg,
+                uint32_t nbins, hipStream_t s) {
+    LOKI_REQUIRE(nbins >= 1, "merge: nbins must be positive");
+    LOKI_REQUIRE((coords == nullptr) != (coords_freq == nullptr),
+                 "merge: exactly one coordinate table");
+    if (ncur == 0 || nseg == 0) {
+        return;
+    }
+    if (coords != nullptr) {
+        dispatch<true>(in, out, coords, nullptr, ncur, nprev, nseg, nbins, s);
+    } else {
+        dispatch<false>(in, out, nullptr, coords_freq, ncur, nprev, nseg, nbins, s);
+    }
+}
+
+}  // namespace loki_amd::kern

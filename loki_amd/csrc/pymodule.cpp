@@ -1,0 +1,582 @@
+// Python bindings over the C-ABI (include/loki_amd.h), mirroring the names,
+// argument names/defaults and return types of loki's pybind modules:
+//   libloki:   configs.PulsarSearchConfig, plans.FFAPlan{Base,Time,Fourier},
+//              scores.generate_box_width_trials   (src/loki_python.cpp:71-520)
+//   libculoki: ffa.FFACUDA / compute_ffa_time_cuda / compute_ffa_scores_cuda,
+//              fold.compute_brute_fold_time_cuda, scores.snr_boxcar_*
+//              (src/loki_python_cuda.cpp:18-203, loki_templates_cuda.cuh:21-51)
+// C-ABI failures raise RuntimeError with the library's message, the same
+// Python exception type pybind11 gives for the reference's C++ exceptions.
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <optional>
+#include <string>
+#include <vector>
+
+#include "../../include/loki_amd.h"
+
+namespace py = pybind11;
+
+template <typename T>
+using PyArrayT = py::array_t<T, py::array::c_style | py::array::forcecast>;
+
+namespace {
+
+void check(int rc) {
+    if (rc != LOKI_OK) {
+        throw std::runtime_error(loki_last_error());
+    }
+}
+
+template <typename T> py::array_t<T> to_array(const std::vector<T>& v) {
+    py::array_t<T> a(static_cast<py::ssize_t>(v.size()));
+    std::copy(v.begin(), v.end(), a.mutable_data());
+    return a;
+}
+
+const std::vector<std::string> kNames = {"crackle", "snap", "jerk", "accel", "freq"};
+
+class Config {
+public:
+    Config(size_t nsamps, double tsamp, size_t nbins, double eta,
+           const PyArrayT<double>& param_limits, double ducy_max, double wtsp, bool use_fourier,
+           int nthreads, double max_process_memory_gb, double octave_scale, size_t nbins_max,
+           size_t nbins_min_lossy_bf, std::optional<size_t> bseg_brute,
+           std::optional<size_t> bseg_ffa, double snr_min, size_t max_passing_candidates,
+           size_t prune_poly_order, double p_orb_min, double m_c_max, double m_p_min,
+           double minimum_snap_cells, bool use_conservative_tile) {
+        if (param_limits.ndim() != 2 || param_limits.shape(1) != 2) {
+            throw std::invalid_argument(
+                "param_limits must be a 2D NumPy array with shape (n_params, 2)");
+        }
+        loki_config_params p{};
+        p.nsamps                 = nsamps;
+        p.tsamp                  = tsamp;
+        p.nbins                  = nbins;
+        p.eta                    = eta;
+        p.param_limits           = param_limits.data();
+        p.nparams                = static_cast<size_t>(param_limits.shape(0));
+        p.ducy_max               = ducy_max;
+        p.wtsp                   = wtsp;
+        p.use_fourier            = use_fourier ? 1 : 0;
+        p.nthreads               = nthreads;
+        p.max_process_memory_gb  = max_process_memory_gb;
+        p.octave_scale           = octave_scale;
+        p.nbins_max              = nbins_max;
+        p.nbins_min_lossy_bf     = nbins_min_lossy_bf;
+        p.bseg_brute             = bseg_brute.value_or(0);
+        p.bseg_ffa               = bseg_ffa.value_or(0);
+        p.snr_min                = snr_min;
+        p.max_passing_candidates = max_passing_candidates;
+        p.prune_poly_order       = prune_poly_order;
+        p.p_orb_min              = p_orb_min;
+        p.m_c_max                = m_c_max;
+        p.m_p_min                = m_p_min;
+        p.minimum_snap_cells     = minimum_snap_cells;
+        p.use_conservative_tile  = use_conservative_tile ? 1 : 0;
+        loki_config_t h          = nullptr;
+        check(loki_config_create(&p, &h));
+        m_h.reset(h, loki_config_destroy);
+        check(loki_config_info_get(h, &m_info));
+        check(loki_config_params_get(h, &m_p));
+    }
+    loki_config_t handle() const { return m_h.get(); }
+    const loki_config_info& info() const { return m_info; }
+    const loki_config_params& params() const { return m_p; }
+    py::array_t<double> param_limits() const {
+        py::array_t<double> a({static_cast<py::ssize_t>(m_p.nparams), py::ssize_t{2}});
+        std::copy(m_p.param_limits, m_p.param_limits + 2 * m_p.nparams, a.mutable_data());
+        return a;
+    }
+    std::vector<std::string> param_names() const {
+        return {kNames.end() - static_cast<long>(m_p.nparams), kNames.end()};
+    }
+    py::array_t<size_t> score_widths() const {
+        size_t n = 0;
+        check(loki_config_scoring_widths(handle(), nullptr, 0, &n));
+        std::vector<size_t> w(n);
+        check(loki_config_scoring_widths(handle(), w.data(), n, &n));
+        return to_array(w);
+    }
+    std::vector<double> dparams(double tseg, int kind) const {
+        std::vector<double> v(m_p.nparams);
+        check(loki_config_dparams(handle(), tseg, kind, v.data()));
+        return v;
+    }
+
+private:
+    std::shared_ptr<loki_config_s> m_h;
+    loki_config_info m_info{};
+    loki_config_params m_p{};
+};
+
+class Plan {
+public:
+    Plan(const Config& cfg, int fold_type) : m_cfg(cfg), m_fold_type(fold_type) {
+        loki_plan_t h = nullptr;
+        check(loki_plan_create(cfg.handle(), fold_type, &h));
+        m_h.reset(h, loki_plan_destroy);
+        check(loki_plan_info_get(h, &m_info));
+        m_levels.resize(m_info.n_levels);
+        for (size_t l = 0; l < m_info.n_levels; ++l) {
+            check(loki_plan_level_get(h, l, &m_levels[l]));
+        }
+    }
+    const loki_plan_info& info() const { return m_info; }
+    const Config& config() const { return m_cfg; }
+    int fold_type() const { return m_fold_type; }
+    template <typename F> py::array level_array(F get) const {
+        using T = decltype(get(m_levels[0]));
+        std::vector<T> v;
+        for (const auto& lv : m_levels) {
+            v.push_back(get(lv));
+        }
+        return to_array(v);
+    }
+    py::list per_level(bool counts, bool strides, bool dp, bool dpa) const {
+        py::list out;
+        for (const auto& lv : m_levels) {
+            if (counts || strides) {
+                std::vector<size_t> v(lv.param_counts, lv.param_counts + m_info.n_params);
+                if (strides) {
+                    v.assign(lv.param_cart_strides, lv.param_cart_strides + m_info.n_params);
+                }
+                out.append(to_array(v));
+            } else {
+                const double* src = dp ? lv.dparams : lv.dparams_actual;
+                (void)dpa;
+                out.append(to_array(std::vector<double>(src, src + m_info.n_params)));
+            }
+        }
+        return out;
+    }
+    py::list fold_shapes(bool time) const {
+        py::list out;
+        for (const auto& lv : m_levels) {
+            const size_t* s = time ? lv.fold_shape_time : lv.fold_shape;
+            out.append(to_array(std::vector<size_t>(s, s + lv.fold_ndim)));
+        }
+        return out;
+    }
+    std::vector<double> param_grid(size_t level, size_t param) const {
+        size_t n = 0;
+        check(loki_plan_param_grid(m_h.get(), level, param, nullptr, 0, &n));
+        std::vector<double> v(n);
+        check(loki_plan_param_grid(m_h.get(), level, param, v.data(), n, &n));
+        return v;
+    }
+    py::list compute_param_grid(size_t level) const {
+        py::list out;
+        for (size_t p = 0; p < m_info.n_params; ++p) {
+            out.append(to_array(param_grid(level, p)));
+        }
+        return out;
+    }
+    py::dict params_dict() const {
+        py::dict d;
+        const auto names = m_cfg.param_names();
+        for (size_t p = 0; p < m_info.n_params; ++p) {
+            d[py::str(names[p])] = to_array(param_grid(m_info.n_levels - 1, p));
+        }
+        return d;
+    }
+    py::list resolve_coordinates(bool freq_only, int device_id) const {
+        if (freq_only != (m_info.n_params == 1)) {
+            throw std::runtime_error(freq_only
+                                         ? "resolve_coordinates_freq() only supports nparams=1"
+                                         : "resolve_coordinates only supports nparams>=2. For "
+                                           "frequency coordinates, use "
+                                           "resolve_coordinates_freq() instead.");
+        }
+        const size_t rec = freq_only ? 8 : 16;
+        std::vector<uint8_t> buf(m_info.coord_size * rec);
+        check(loki_plan_resolve_coordinates_host(m_h.get(), buf.data(), buf.size(), device_id));
+        py::list out;
+        py::list fields;
+        if (freq_only) {
+            fields.append(py::make_tuple("idx", "<u4"));
+            fields.append(py::make_tuple("shift", "<f4"));
+        } else {
+            fields.append(py::make_tuple("i_tail", "<u4"));
+            fields.append(py::make_tuple("shift_tail", "<f4"));
+            fields.append(py::make_tuple("i_head", "<u4"));
+            fields.append(py::make_tuple("shift_head", "<f4"));
+        }
+        py::dtype dt = py::module_::import("numpy").attr("dtype")(fields).cast<py::dtype>();
+        for (const auto& lv : m_levels) {
+            py::array a(dt, std::vector<py::ssize_t>{static_cast<py::ssize_t>(lv.ncoords)});
+            std::memcpy(a.mutable_data(), buf.data() + lv.ncoords_offset * rec, lv.ncoords * rec);
+            out.append(a);
+        }
+        return out;
+    }
+
+private:
+    Config m_cfg;
+    int m_fold_type;
+    std::shared_ptr<loki_plan_s> m_h;
+    loki_plan_info m_info{};
+    std::vector<loki_plan_level> m_levels;
+};
+
+struct PlanTime : Plan {
+    explicit PlanTime(const Config& c) : Plan(c, LOKI_FOLD_TIME) {}
+};
+struct PlanFourier : Plan {
+    explicit PlanFourier(const Config& c) : Plan(c, LOKI_FOLD_FOURIER) {}
+};
+
+class FFA {
+public:
+    FFA(const Config& cfg, int device_id, int fold_type)
+        : m_cfg(cfg), m_device(device_id), m_plan(cfg, fold_type) {
+        loki_ffa_t h = nullptr;
+        check(loki_ffa_create(cfg.handle(), fold_type, device_id, &h));
+        m_h.reset(h, loki_ffa_destroy);
+    }
+    const Plan& plan() const { return m_plan; }
+    void execute(const PyArrayT<float>& ts_e, const PyArrayT<float>& ts_v,
+                 PyArrayT<float>& fold) {
+        if (ts_v.size() != ts_e.size()) {
+            throw std::runtime_error("FFA::execute: ts_v must have size nsamps");
+        }
+        check(loki_ffa_execute_host(m_h.get(), ts_e.data(), ts_v.data(),
+                                    static_cast<size_t>(ts_e.size()), fold.mutable_data(),
+                                    static_cast<size_t>(fold.size())));
+    }
+    void execute_device(uintptr_t ts_e, uintptr_t ts_v, uintptr_t fold, size_t fold_len,
+                        uintptr_t stream) {
+        check(loki_ffa_execute_device(m_h.get(), reinterpret_cast<const float*>(ts_e),
+                                      reinterpret_cast<const float*>(ts_v),
+                                      reinterpret_cast<void*>(fold), fold_len,
+                                      reinterpret_cast<void*>(stream)));
+    }
+    void scores_device(uintptr_t ts_e, uintptr_t ts_v, const PyArrayT<size_t>& widths,
+                       uintptr_t scores, size_t scores_len, bool find_max, uintptr_t stream) {
+        check(loki_ffa_scores_device(m_h.get(), reinterpret_cast<const float*>(ts_e),
+                                     reinterpret_cast<const float*>(ts_v), widths.data(),
+                                     static_cast<size_t>(widths.size()),
+                                     reinterpret_cast<float*>(scores), scores_len,
+                                     find_max ? 1 : 0, reinterpret_cast<void*>(stream)));
+    }
+    py::array_t<float> scores_host(const PyArrayT<float>& ts_e, const PyArrayT<float>& ts_v,
+                                   const PyArrayT<size_t>& widths, bool find_max) {
+        const size_t nc = m_plan.info().fold_size / (2 * m_cfg.params().nbins);
+        const size_t n  = find_max ? nc : nc * static_cast<size_t>(widths.size());
+        py::array_t<float> out(static_cast<py::ssize_t>(n));
+        check(loki_ffa_scores_host(m_h.get(), ts_e.data(), ts_v.data(),
+                                   static_cast<size_t>(ts_e.size()), widths.data(),
+                                   static_cast<size_t>(widths.size()), out.mutable_data(), n,
+                                   find_max ? 1 : 0));
+        return out;
+    }
+
+private:
+    Config m_cfg;
+    int m_device;
+    Plan m_plan;
+    std::shared_ptr<loki_ffa_s> m_h;
+};
+
+void check_nsamps(const PyArrayT<float>& ts_e, const PyArrayT<float>& ts_v, const Config& cfg) {
+    if (static_cast<size_t>(ts_e.size()) != cfg.params().nsamps) {
+        throw std::runtime_error("FFA::execute: ts_e must have size nsamps");
+    }
+    if (ts_v.size() != ts_e.size()) {
+        throw std::runtime_error("FFA::execute: ts_v must have size nsamps");
+    }
+}
+
+}  // namespace
+
+PYBIND11_MODULE(_loki_amd, m) {
+    m.doc() = "MI355X (gfx950) FFA hot path: loki-compatible bindings over the loki_amd C-ABI";
+
+    // ---------------------------------------------------------------- configs
+    auto m_configs = m.def_submodule("configs", "Configs submodule");
+    py::class_<Config>(m_configs, "PulsarSearchConfig")
+        .def(py::init<size_t, double, size_t, double, const PyArrayT<double>&, double, double,
+                      bool, int, double, double, size_t, size_t, std::optional<size_t>,
+                      std::optional<size_t>, double, size_t, size_t, double, double, double,
+                      double, bool>(),
+             py::arg("nsamps"), py::arg("tsamp"), py::arg("nbins"), py::arg("eta"),
+             py::arg("param_limits"), py::arg("ducy_max") = 0.2, py::arg("wtsp") = 1.5,
+             py::arg("use_fourier") = true, py::arg("nthreads") = 1,
+             py::arg("max_process_memory_gb") = 8.0, py::arg("octave_scale") = 2.0,
+             py::arg("nbins_max") = 1024, py::arg("nbins_min_lossy_bf") = 64,
+             py::arg("bseg_brute") = std::nullopt, py::arg("bseg_ffa") = std::nullopt,
+             py::arg("snr_min") = 5.0, py::arg("max_passing_candidates") = 1U << 22U,
+             py::arg("prune_poly_order") = 3, py::arg("p_orb_min") = 1e-5,
+             py::arg("m_c_max") = 10.0, py::arg("m_p_min") = 1.4,
+             py::arg("minimum_snap_cells") = 5.0, py::arg("use_conservative_tile") = false)
+        .def_property_readonly("nsamps", [](const Config& c) { return c.params().nsamps; })
+        .def_property_readonly("tsamp", [](const Config& c) { return c.params().tsamp; })
+        .def_property_readonly("tobs", [](const Config& c) { return c.info().tobs; })
+        .def_property_readonly("nbins", [](const Config& c) { return c.params().nbins; })
+        .def_property_readonly("nbins_f", [](const Config& c) { return c.info().nbins_f; })
+        .def_property_readonly("eta", [](const Config& c) { return c.params().eta; })
+        .def_property_readonly("param_limits", &Config::param_limits)
+        .def_property_readonly("ducy_max", [](const Config& c) { return c.params().ducy_max; })
+        .def_property_readonly("wtsp", [](const Config& c) { return c.params().wtsp; })
+        .def_property_readonly("prune_poly_order",
+                               [](const Config& c) { return c.params().prune_poly_order; })
+        .def_property_readonly("bseg_brute", [](const Config& c) { return c.info().bseg_brute; })
+        .def_property_readonly("bseg_ffa", [](const Config& c) { return c.info().bseg_ffa; })
+        .def_property_readonly("p_orb_min", [](const Config& c) { return c.params().p_orb_min; })
+        .def_property_readonly("minimum_snap_cells",
+                               [](const Config& c) { return c.params().minimum_snap_cells; })
+        .def_property_readonly("use_fourier",
+                               [](const Config& c) { return c.params().use_fourier != 0; })
+        .def_property_readonly("use_conservative_tile",
+                               [](const Config& c) { return c.params().use_conservative_tile != 0; })
+        .def_property_readonly("nthreads", [](const Config& c) { return c.params().nthreads; })
+        .def_property_readonly("tseg_brute", [](const Config& c) { return c.info().tseg_brute; })
+        .def_property_readonly("tseg_ffa", [](const Config& c) { return c.info().tseg_ffa; })
+        .def_property_readonly("niters_ffa", [](const Config& c) { return c.info().niters_ffa; })
+        .def_property_readonly("nparams", [](const Config& c) { return c.info().nparams; })
+        .def_property_readonly("param_names", &Config::param_names)
+        .def_property_readonly("f_min", [](const Config& c) { return c.info().f_min; })
+        .def_property_readonly("f_max", [](const Config& c) { return c.info().f_max; })
+        .def_property_readonly("score_widths", &Config::score_widths)
+        .def_property_readonly("n_scoring_widths",
+                               [](const Config& c) { return c.info().n_scoring_widths; })
+        .def("dparams_f", [](const Config& c, double t) { return c.dparams(t, 1); },
+             py::arg("tseg_cur"))
+        .def("dparams", [](const Config& c, double t) { return c.dparams(t, 0); },
+             py::arg("tseg_cur"))
+        .def("dparams_actual", [](const Config& c, double t) { return c.dparams(t, 2); },
+             py::arg("tseg_cur"));
+
+    // ------------------------------------------------------------------ plans
+    auto m_plans = m.def_submodule("plans", "Plans submodule");
+    py::class_<Plan>(m_plans, "FFAPlanBase")
+        .def(py::init<const Config&, int>(), py::arg("cfg"), py::arg("fold_type") = 0)
+        .def_property_readonly("n_params", [](const Plan& p) { return p.info().n_params; })
+        .def_property_readonly("n_levels", [](const Plan& p) { return p.info().n_levels; })
+        .def_property_readonly("segment_lens", [](const Plan& p) {
+            return p.level_array([](const loki_plan_level& l) { return l.segment_len; });
+        })
+        .def_property_readonly("nsegments", [](const Plan& p) {
+            return p.level_array([](const loki_plan_level& l) { return l.nsegments; });
+        })
+        .def_property_readonly("tsegments", [](const Plan& p) {
+            return p.level_array([](const loki_plan_level& l) { return l.tsegment; });
+        })
+        .def_property_readonly("ncoords", [](const Plan& p) {
+            return p.level_array([](const loki_plan_level& l) { return l.ncoords; });
+        })
+        .def_property_readonly("ncoords_lb", [](const Plan& p) {
+            return p.level_array([](const loki_plan_level& l) { return l.ncoords_lb; });
+        })
+        .def_property_readonly("ncoords_offsets", [](const Plan& p) {
+            // u32 offsets with the final sentinel (plans.cpp:311-319).
+            auto off = p.level_array([](const loki_plan_level& l) {
+                             return static_cast<uint32_t>(l.ncoords_offset);
+                         }).cast<std::vector<uint32_t>>();
+            off.push_back(static_cast<uint32_t>(p.info().coord_size));
+            return to_array(off);
+        })
+        .def_property_readonly("param_counts",
+                               [](const Plan& p) { return p.per_level(true, false, false, false); })
+        .def_property_readonly("param_cart_strides",
+                               [](const Plan& p) { return p.per_level(false, true, false, false); })
+        .def_property_readonly("dparams",
+                               [](const Plan& p) { return p.per_level(false, false, true, false); })
+        .def_property_readonly("dparams_actual",
+                               [](const Plan& p) { return p.per_level(false, false, false, true); })
+        .def_property_readonly("config", &Plan::config)
+        .def_property_readonly("coord_size", [](const Plan& p) { return p.info().coord_size; })
+        .def_property_readonly("coord_memory_usage",
+                               [](const Plan& p) { return p.info().coord_memory_usage; })
+        .def("resolve_coordinates",
+             [](const Plan& p, int device_id) { return p.resolve_coordinates(false, device_id); },
+             py::arg("device_id") = 0)
+        .def("resolve_coordinates_freq",
+             [](const Plan& p, int device_id) { return p.resolve_coordinates(true, device_id); },
+             py::arg("device_id") = 0)
+        .def_property_readonly("param_grid",
+                               [](const Plan& p) {
+                                   py::list out;
+                                   for (size_t l = 0; l < p.info().n_levels; ++l) {
+                                       out.append(p.compute_param_grid(l));
+                                   }
+                                   return out;
+                               })
+        .def_property_readonly("params_dict", &Plan::params_dict)
+        .def("compute_param_grid", &Plan::compute_param_grid, py::arg("ffa_level"))
+        .def_property_readonly("fold_shapes", [](const Plan& p) { return p.fold_shapes(false); })
+        .def_property_readonly("fold_shapes_time",
+                               [](const Plan& p) { return p.fold_shapes(true); })
+        .def_property_readonly("brute_fold_size",
+                               [](const Plan& p) { return p.info().brute_fold_size; })
+        .def_property_readonly("fold_size", [](const Plan& p) { return p.info().fold_size; })
+        .def_property_readonly("fold_size_time",
+                               [](const Plan& p) { return p.info().fold_size_time; })
+        .def_property_readonly("buffer_size", [](const Plan& p) { return p.info().buffer_size; })
+        .def_property_readonly("buffer_size_time",
+                               [](const Plan& p) { return p.info().buffer_size_time; })
+        .def_property_readonly("buffer_memory_usage",
+                               [](const Plan& p) { return p.info().buffer_memory_usage; })
+        .def("get_gflops", [](const Plan& p, bool t) {
+            return t ? p.info().gflops_return_in_time : p.info().gflops;
+        }, py::arg("return_in_time") = false);
+    py::class_<PlanTime, Plan>(m_plans, "FFAPlanTime").def(py::init<const Config&>(), py::arg("cfg"));
+    py::class_<PlanFourier, Plan>(m_plans, "FFAPlanFourier")
+        .def(py::init<const Config&>(), py::arg("cfg"));
+
+    // ----------------------------------------------------------------- scores
+    auto m_scores = m.def_submodule("scores", "Scores submodule");
+    m_scores.def(
+        "generate_box_width_trials",
+        [](size_t nbins, double ducy_max, double wtsp) {
+            size_t n = 0;
+            check(loki_generate_box_width_trials(nbins, ducy_max, wtsp, nullptr, 0, &n));
+            std::vector<size_t> v(n);
+            check(loki_generate_box_width_trials(nbins, ducy_max, wtsp, v.data(), n, &n));
+            return to_array(v);
+        },
+        py::arg("nbins"), py::arg("ducy_max") = 0.3, py::arg("wtsp") = 1.5);
+    m_scores.def(
+        "snr_boxcar_3d",
+        [](const PyArrayT<float>& arr, const PyArrayT<size_t>& widths, int device_id) {
+            if (arr.ndim() != 3 || widths.ndim() != 1) {
+                throw std::runtime_error(
+                    "Input array must be 3-dimensional, widths must be 1-dimensional");
+            }
+            if (arr.shape(0) == 0 || arr.shape(1) == 0 || widths.size() == 0) {
+                throw std::runtime_error("Input arrays cannot be empty");
+            }
+            const auto nprof = static_cast<size_t>(arr.shape(0));
+            const auto nbins = static_cast<size_t>(arr.shape(2));
+            py::array_t<float> out({static_cast<py::ssize_t>(nprof), widths.size()});
+            check(loki_snr_boxcar_3d_host(arr.data(), nprof, nbins, widths.data(),
+                                          static_cast<size_t>(widths.size()), out.mutable_data(),
+                                          0, device_id));
+            return out;
+        },
+        py::arg("arr"), py::arg("widths"), py::arg("device_id") = 0);
+    m_scores.def(
+        "snr_boxcar_3d_max",
+        [](const PyArrayT<float>& arr, const PyArrayT<size_t>& widths, int device_id) {
+            if (arr.ndim() != 3 || widths.ndim() != 1) {
+                throw std::runtime_error(
+                    "Input array must be 3-dimensional, widths must be 1-dimensional");
+            }
+            if (arr.shape(0) == 0 || arr.shape(1) == 0 || widths.size() == 0) {
+                throw std::runtime_error("Input arrays cannot be empty");
+            }
+            const auto nprof = static_cast<size_t>(arr.shape(0));
+            const auto nbins = static_cast<size_t>(arr.shape(2));
+            py::array_t<float> out(static_cast<py::ssize_t>(nprof));
+            check(loki_snr_boxcar_3d_host(arr.data(), nprof, nbins, widths.data(),
+                                          static_cast<size_t>(widths.size()), out.mutable_data(),
+                                          1, device_id));
+            return out;
+        },
+        py::arg("arr"), py::arg("widths"), py::arg("device_id") = 0);
+    m_scores.def(
+        "snr_boxcar_2d_max",
+        [](const PyArrayT<float>& arr, const PyArrayT<size_t>& widths, float stdnoise,
+           int device_id) {
+            if (arr.ndim() != 2 || widths.ndim() != 1) {
+                throw std::runtime_error(
+                    "Input array must be 2-dimensional, widths must be 1-dimensional");
+            }
+            if (arr.shape(0) == 0 || arr.shape(1) == 0 || widths.size() == 0) {
+                throw std::runtime_error("Input arrays cannot be empty");
+            }
+            const auto nprof = static_cast<size_t>(arr.shape(0));
+            const auto nbins = static_cast<size_t>(arr.shape(1));
+            py::array_t<float> out(static_cast<py::ssize_t>(nprof));
+            check(loki_snr_boxcar_2d_max_host(arr.data(), nprof, nbins, widths.data(),
+                                              static_cast<size_t>(widths.size()), stdnoise,
+                                              out.mutable_data(), device_id));
+            return out;
+        },
+        py::arg("arr"), py::arg("widths"), py::arg("stdnoise") = 1.0F, py::arg("device_id") = 0);
+
+    // ------------------------------------------------------------------- fold
+    auto m_fold = m.def_submodule("fold", "Fold submodule");
+    m_fold.def(
+        "compute_brute_fold_time_cuda",
+        [](const PyArrayT<float>& ts_e, const PyArrayT<float>& ts_v,
+           const PyArrayT<double>& freq_arr, size_t segment_len, size_t nbins, double tsamp,
+           double t_ref, int device_id) {
+            if (ts_v.size() != ts_e.size()) {
+                throw std::runtime_error("BruteFold::Impl::execute: ts_v must have size nsamps");
+            }
+            const auto nsamps = static_cast<size_t>(ts_e.size());
+            if (segment_len == 0 || nsamps % segment_len != 0) {
+                throw std::runtime_error(
+                    "BruteFold::Impl: Number of samples is not a multiple of segment length");
+            }
+            const size_t n = (nsamps / segment_len) * static_cast<size_t>(freq_arr.size()) * 2 * nbins;
+            py::array_t<float> out(static_cast<py::ssize_t>(n));
+            check(loki_brute_fold_time_host(ts_e.data(), ts_v.data(), nsamps, freq_arr.data(),
+                                            static_cast<size_t>(freq_arr.size()), segment_len,
+                                            nbins, tsamp, t_ref, out.mutable_data(), device_id));
+            return out;
+        },
+        py::arg("ts_e"), py::arg("ts_v"), py::arg("freq_arr"), py::arg("segment_len"),
+        py::arg("nbins"), py::arg("tsamp"), py::arg("t_ref") = 0.0F, py::arg("device_id") = 0);
+
+    // -------------------------------------------------------------------- ffa
+    auto m_ffa = m.def_submodule("ffa", "FFA submodule");
+    py::class_<FFA>(m_ffa, "FFACUDA")
+        .def(py::init([](const Config& cfg, int device_id) {
+                 return std::make_unique<FFA>(cfg, device_id, LOKI_FOLD_TIME);
+             }),
+             py::arg("cfg"), py::arg("device_id") = 0)
+        .def_property_readonly("plan", &FFA::plan)
+        .def("execute", &FFA::execute, py::arg("ts_e"), py::arg("ts_v"), py::arg("fold"))
+        .def("execute_device", &FFA::execute_device, py::arg("ts_e_ptr"), py::arg("ts_v_ptr"),
+             py::arg("fold_ptr"), py::arg("fold_len"), py::arg("stream") = 0)
+        .def("scores_device", &FFA::scores_device, py::arg("ts_e_ptr"), py::arg("ts_v_ptr"),
+             py::arg("widths"), py::arg("scores_ptr"), py::arg("scores_len"),
+             py::arg("find_max") = false, py::arg("stream") = 0)
+        .def("scores", &FFA::scores_host, py::arg("ts_e"), py::arg("ts_v"), py::arg("widths"),
+             py::arg("find_max") = false);
+
+    m_ffa.def(
+        "compute_ffa_time_cuda",
+        [](const PyArrayT<float>& ts_e, const PyArrayT<float>& ts_v, const Config& cfg,
+           int device_id, bool /*quiet*/) {
+            check_nsamps(ts_e, ts_v, cfg);
+            FFA ffa(cfg, device_id, LOKI_FOLD_TIME);
+            const size_t buf = ffa.plan().info().buffer_size;
+            PyArrayT<float> fold(static_cast<py::ssize_t>(buf));
+            ffa.execute(ts_e, ts_v, fold);
+            fold.resize({static_cast<py::ssize_t>(ffa.plan().info().fold_size)});
+            return py::make_tuple(fold, PlanTime(cfg));
+        },
+        py::arg("ts_e"), py::arg("ts_v"), py::arg("cfg"), py::arg("device_id") = 0,
+        py::arg("quiet") = false);
+    m_ffa.def(
+        "compute_ffa_scores_cuda",
+        [](const PyArrayT<float>& ts_e, const PyArrayT<float>& ts_v, const Config& cfg,
+           int device_id, bool /*quiet*/, std::optional<PyArrayT<size_t>> widths) {
+            check_nsamps(ts_e, ts_v, cfg);
+            if (cfg.params().use_fourier != 0) {
+                throw std::runtime_error(
+                    "compute_ffa_scores_cuda: the Fourier-domain path is not available in this "
+                    "build; use use_fourier=False");
+            }
+            FFA ffa(cfg, device_id, LOKI_FOLD_TIME);
+            PyArrayT<size_t> w = widths.has_value() ? *widths : PyArrayT<size_t>(cfg.score_widths());
+            auto scores = ffa.scores_host(ts_e, ts_v, w, false);
+            return py::make_tuple(scores, PlanTime(cfg));
+        },
+        py::arg("ts_e"), py::arg("ts_v"), py::arg("cfg"), py::arg("device_id") = 0,
+        py::arg("quiet") = false, py::arg("widths") = std::nullopt);
+
+    m.def("device_count", [] {
+        int n = 0;
+        check(loki_device_count(&n));
+        return n;
+    });
+}

@@ -1,0 +1,22 @@
+"""``loki.libculoki``-compatible GPU modules (ffa, fold, scores) on gfx950.
+
+Mirrors src/loki_python_cuda.cpp:18-203 and src/loki_templates_cuda.cuh:21-51: the same function
+and class names, argument names, defaults and return types.  ``FFACUDA`` keeps the reference's
+name so that callers switch backends by changing the import only.
+"""
+
+from __future__ import annotations
+
+from types import SimpleNamespace
+
+from . import _loki_amd as _n
+
+ffa = _n.ffa
+fold = _n.fold
+scores = SimpleNamespace(
+    snr_boxcar_3d=_n.scores.snr_boxcar_3d,
+    snr_boxcar_3d_max=_n.scores.snr_boxcar_3d_max,
+    snr_boxcar_2d_max=_n.scores.snr_boxcar_2d_max,
+)
+
+__all__ = ["ffa", "fold", "scores"]

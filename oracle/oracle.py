@@ -1,0 +1,209 @@
+"""ctypes wrappers around oracle/liboracle.so and whole-pipeline drivers.
+
+TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).  The drivers mirror the
+level loop of ffa.cpp:242-360 and the compute_* entry points of
+ffa.cpp:405-466 so tests can compare the HIP product end to end.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+from .plan import FFA_COORD_DTYPE, FFA_COORD_FREQ_DTYPE, Plan, SearchConfig
+
+_HERE = Path(__file__).resolve().parent
+_LIB = None
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+_sz = C.c_size_t
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", str(_HERE)], check=True)
+
+
+def lib() -> C.CDLL:
+    global _LIB
+    if _LIB is None:
+        path = _HERE / "liboracle.so"
+        if not path.exists():
+            build()
+        L = C.CDLL(str(path))
+        L.orc_phase_idx.restype = C.c_float
+        L.orc_phase_idx.argtypes = [C.c_double, C.c_double, _sz, C.c_double]
+        L.orc_brute_fold_time.argtypes = [_f32p, _f32p, _sz, _f64p, _sz, _sz, _sz, C.c_double,
+                                          C.c_double, _f32p, C.c_int]
+        L.orc_brute_fold_time_f64.argtypes = [_f32p, _f32p, _sz, _f64p, _sz, _sz, _sz,
+                                              C.c_double, C.c_double, _f64p, C.c_int]
+        for name, p in (("orc_ffa_iter_freq", _f32p), ("orc_ffa_iter_poly", _f32p),
+                        ("orc_ffa_iter_freq_f64", _f64p), ("orc_ffa_iter_poly_f64", _f64p),
+                        ("orc_ffa_complex_iter_freq", _f32p), ("orc_ffa_complex_iter_poly", _f32p)):
+            getattr(L, name).argtypes = [p, p, C.c_void_p, _sz, _sz, _sz, _sz, C.c_int]
+        L.orc_brute_fold_fourier.argtypes = [_f32p, _f32p, _sz, _f64p, _sz, _sz, _sz, C.c_double,
+                                             C.c_double, _f32p, C.c_int]
+        L.orc_irfft.argtypes = [_f32p, _f32p, _sz, _sz, C.c_int]
+        L.orc_circular_prefix_sum.argtypes = [_f32p, _f32p, _sz, _sz]
+        L.orc_diff_max.restype = C.c_float
+        L.orc_diff_max.argtypes = [_f32p, _f32p, _sz]
+        L.orc_snr_boxcar_3d.argtypes = [_f32p, _sz, _sz, _u64p, _sz, _f32p, C.c_int, C.c_int]
+        L.orc_snr_boxcar_3d_f64.argtypes = [_f64p, _sz, _sz, _u64p, _sz, _f64p, C.c_int]
+        _LIB = L
+    return _LIB
+
+
+def _nthreads(n: int | None) -> int:
+    return int(n) if n else (os.cpu_count() or 1)
+
+
+# ---------------------------------------------------------------------------
+# kernels
+# ---------------------------------------------------------------------------
+def brute_fold_time(ts_e, ts_v, freqs, segment_len, nbins, tsamp, t_ref=0.0, nthreads=None,
+                    f64=False):
+    ts_e = np.ascontiguousarray(ts_e, np.float32)
+    ts_v = np.ascontiguousarray(ts_v, np.float32)
+    freqs = np.ascontiguousarray(freqs, np.float64)
+    nseg = ts_e.size // segment_len
+    if f64:
+        out = np.zeros(nseg * freqs.size * 2 * nbins, np.float64)
+        lib().orc_brute_fold_time_f64(ts_e, ts_v, ts_e.size, freqs, freqs.size, segment_len, nbins,
+                                      tsamp, t_ref, out, _nthreads(nthreads))
+    else:
+        out = np.zeros(nseg * freqs.size * 2 * nbins, np.float32)
+        lib().orc_brute_fold_time(ts_e, ts_v, ts_e.size, freqs, freqs.size, segment_len, nbins,
+                                  tsamp, t_ref, out, _nthreads(nthreads))
+    return out
+
+
+def brute_fold_fourier(ts_e, ts_v, freqs, segment_len, nbins, tsamp, t_ref=0.0, nthreads=None):
+    ts_e = np.ascontiguousarray(ts_e, np.float32)
+    ts_v = np.ascontiguousarray(ts_v, np.float32)
+    freqs = np.ascontiguousarray(freqs, np.float64)
+    nseg = ts_e.size // segment_len
+    nbf = nbins // 2 + 1
+    out = np.zeros(nseg * freqs.size * 2 * nbf * 2, np.float32)
+    lib().orc_brute_fold_fourier(ts_e, ts_v, ts_e.size, freqs, freqs.size, segment_len, nbins,
+                                 tsamp, t_ref, out, _nthreads(nthreads))
+    return out.view(np.complex64)
+
+
+def ffa_iter(fold_in, coords, ncur, nprev, nseg, nbins, nthreads=None, fourier=False):
+    """One merge level; dispatch on coord dtype (freq-only vs poly) and domain."""
+    coords = np.ascontiguousarray(coords)
+    freq_only = coords.dtype == FFA_COORD_FREQ_DTYPE
+    if fourier:
+        nbf = nbins // 2 + 1
+        fin = np.ascontiguousarray(fold_in).view(np.float32)
+        out = np.empty(nseg * ncur * 2 * nbf * 2, np.float32)
+        fn = lib().orc_ffa_complex_iter_freq if freq_only else lib().orc_ffa_complex_iter_poly
+        fn(fin, out, coords.ctypes.data, ncur, nprev, nseg, nbins, _nthreads(nthreads))
+        return out.view(np.complex64)
+    f64 = fold_in.dtype == np.float64
+    out = np.empty(nseg * ncur * 2 * nbins, np.float64 if f64 else np.float32)
+    if f64:
+        fn = lib().orc_ffa_iter_freq_f64 if freq_only else lib().orc_ffa_iter_poly_f64
+    else:
+        fn = lib().orc_ffa_iter_freq if freq_only else lib().orc_ffa_iter_poly
+    fn(np.ascontiguousarray(fold_in), out, coords.ctypes.data, ncur, nprev, nseg, nbins,
+       _nthreads(nthreads))
+    return out
+
+
+def irfft(fold_c, nbins, nthreads=None):
+    fin = np.ascontiguousarray(fold_c, np.complex64).view(np.float32)
+    nbf = nbins // 2 + 1
+    nfft = fin.size // (2 * nbf)
+    out = np.empty(nfft * nbins, np.float32)
+    lib().orc_irfft(fin, out, nfft, nbins, _nthreads(nthreads))
+    return out
+
+
+def circular_prefix_sum(x, nsum):
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.zeros(nsum, np.float32)
+    lib().orc_circular_prefix_sum(x, out, x.size, nsum)
+    return out
+
+
+def diff_max(x, y):
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    return float(lib().orc_diff_max(x, y, min(x.size, y.size)))
+
+
+def snr_boxcar_3d(folds, widths, nbins=None, find_max=False, nthreads=None):
+    folds = np.ascontiguousarray(folds, np.float32)
+    widths = np.ascontiguousarray(widths, np.uint64)
+    if nbins is None:
+        nbins = folds.shape[-1]
+    nprof = folds.size // (2 * nbins)
+    out = np.empty(nprof if find_max else nprof * widths.size, np.float32)
+    lib().orc_snr_boxcar_3d(folds.reshape(-1), nprof, nbins, widths, widths.size, out,
+                            int(find_max), _nthreads(nthreads))
+    return out if find_max else out.reshape(nprof, widths.size)
+
+
+def snr_boxcar_3d_f64(folds, widths, nbins, nthreads=None):
+    folds = np.ascontiguousarray(folds, np.float64)
+    widths = np.ascontiguousarray(widths, np.uint64)
+    nprof = folds.size // (2 * nbins)
+    out = np.empty(nprof * widths.size, np.float64)
+    lib().orc_snr_boxcar_3d_f64(folds.reshape(-1), nprof, nbins, widths, widths.size, out,
+                                _nthreads(nthreads))
+    return out.reshape(nprof, widths.size)
+
+
+# ---------------------------------------------------------------------------
+# drivers (ffa.cpp:242-360, 405-466)
+# ---------------------------------------------------------------------------
+def compute_ffa(ts_e, ts_v, cfg: SearchConfig, fourier: bool = False, nthreads=None,
+                f64: bool = False, return_levels: bool = False):
+    """Run brute fold + all merge levels; returns (final fold, plan[, levels])."""
+    plan = Plan(cfg, fourier=fourier)
+    coords = plan.resolve_coordinates()
+    freqs = plan.brute_fold_freqs()
+    t_ref = plan.brute_fold_tref()
+    seglen = plan.segment_lens[0]
+    if fourier:
+        if cfg.nbins > cfg.nbins_min_lossy_bf:
+            raise NotImplementedError("lossy Fourier init (nbins > nbins_min_lossy_bf)")
+        fold = brute_fold_fourier(ts_e, ts_v, freqs, seglen, cfg.nbins, cfg.tsamp, t_ref, nthreads)
+    else:
+        fold = brute_fold_time(ts_e, ts_v, freqs, seglen, cfg.nbins, cfg.tsamp, t_ref, nthreads,
+                               f64=f64)
+    levels = [fold]
+    for lvl in range(1, plan.n_levels):
+        fold = ffa_iter(fold, coords[lvl], plan.ncoords[lvl], plan.ncoords[lvl - 1],
+                        plan.nsegments[lvl], cfg.nbins, nthreads, fourier=fourier)
+        if return_levels:
+            levels.append(fold)
+    if return_levels:
+        return fold, plan, levels
+    return fold, plan
+
+
+def compute_ffa_scores(ts_e, ts_v, cfg: SearchConfig, widths=None, nthreads=None):
+    """ffa.cpp:444-466: FFA (time, or Fourier then irfft) + snr_boxcar_3d."""
+    if cfg.use_fourier:
+        fold_c, plan = compute_ffa(ts_e, ts_v, cfg, fourier=True, nthreads=nthreads)
+        fold = irfft(fold_c, cfg.nbins, nthreads)
+    else:
+        fold, plan = compute_ffa(ts_e, ts_v, cfg, fourier=False, nthreads=nthreads)
+    if plan.nsegments[-1] != 1:
+        raise ValueError("compute_ffa_scores: nsegments must be 1 for scores")
+    w = cfg.score_widths if widths is None else np.asarray(widths, np.uint64)
+    return snr_boxcar_3d(fold, w, cfg.nbins, nthreads=nthreads).reshape(-1), plan
+
+
+__all__ = [
+    "FFA_COORD_DTYPE", "FFA_COORD_FREQ_DTYPE", "Plan", "SearchConfig", "brute_fold_fourier",
+    "brute_fold_time", "build", "circular_prefix_sum", "compute_ffa", "compute_ffa_scores",
+    "diff_max", "ffa_iter", "irfft", "lib", "snr_boxcar_3d", "snr_boxcar_3d_f64",
+]
