@@ -1,0 +1,241 @@
+#!/usr/bin/env python3
+"""Benchmark of the FFA hot path on one MI355X.
+
+Workload (default): Taylor FFA over (accel, freq) + boxcar S/N, FP32, time domain --
+2^23 float32 samples at tsamp = 64 us, N(0,1) noise from a fixed seed with an injected binary
+pulsar (f0 = 200 Hz, accel = +20 m/s^2, Gaussian pulse FWHM 4 %, S/N 15); grid f 100-400 Hz,
+accel -50..+50 m/s^2 at the FFA-native spacing (eta = 1), nbins = 32, boxcar widths
+{1,2,3,4,6,8,12,16,24} (the list {1,...,32} minus widths >= nbins).
+
+One step = shift tables for every level + brute fold + all merge levels + S/N of every final
+trial, on inputs already resident in HBM.  value = final-level parameter trials per second.
+
+Prints ONE JSON line.  ``--gpus N`` with N > 1 exits: this build runs on one GPU.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from collections import defaultdict
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBPS = 8000.0
+WIDTHS = (1, 2, 3, 4, 6, 8, 12, 16, 24, 32)
+
+WORKLOADS = {
+    "taylor_ffa_accel_freq": dict(
+        nsamps=2**23, tsamp=64e-6, nbins=32, eta=1.0,
+        param_limits=[(-50.0, 50.0), (100.0, 400.0)],
+        signal=dict(f0=200.0, accel=20.0, jerk=0.0, fwhm=0.04, snr=15.0), seed=20231,
+        # CPU sample: the same grid rules (f_max = 400 Hz, bseg_brute = 1024) over the top
+        # 75 Hz of the band.
+        cpu_sample=dict(param_limits=[(-50.0, 50.0), (325.0, 400.0)], bseg_brute=1024),
+    ),
+    "ffa_freq_only": dict(
+        nsamps=2**20, tsamp=64e-6, nbins=64, eta=1.0, param_limits=[(0.5, 2.0)],
+        signal=dict(f0=1.0, accel=0.0, jerk=0.0, fwhm=0.05, snr=20.0), seed=101,
+        cpu_sample=dict(param_limits=[(0.5, 2.0)], bseg_brute=None),
+    ),
+}
+
+
+def parse() -> argparse.Namespace:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="taylor_ffa_accel_freq", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--fast-norm", action="store_true",
+                    help="hardware rsqrt profile normalisation (S/N within 1e-4, not bit-exact)")
+    ap.add_argument("--unfused", action="store_true",
+                    help="store the final fold and score it in a separate kernel")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cpu count)")
+    return ap.parse_args()
+
+
+def alg_bytes(plan, nbins, nw, kind, level, nsamps):
+    """Algorithmic bytes of one launch (FP32 time domain; 16-byte poly / 8-byte freq coords)."""
+    npar = int(plan.n_params)
+    rec = 16 if npar > 1 else 8
+    nc = [int(x) for x in plan.ncoords]
+    ns = [int(x) for x in plan.nsegments]
+    if kind == "merge":
+        return ns[level] * nc[level] * 2 * nbins * 3 * 4 + nc[level] * rec
+    if kind == "resolve":
+        return nc[level] * rec
+    if kind == "brute_fold":
+        return 2 * nsamps * 4 + ns[0] * nc[0] * 2 * nbins * 4
+    if kind == "score":
+        return nc[-1] * 2 * nbins * 4 + nc[-1] * nw * 4
+    if kind == "merge_score":  # fused last level: same algorithmic bytes as merge + score
+        return (alg_bytes(plan, nbins, nw, "merge", level, nsamps)
+                + alg_bytes(plan, nbins, nw, "score", level, nsamps))
+    raise ValueError(kind)
+
+
+def pipeline_bytes(plan, nbins, nw, nsamps):
+    tot = alg_bytes(plan, nbins, nw, "brute_fold", 0, nsamps)
+    for lvl in range(1, int(plan.n_levels)):
+        tot += alg_bytes(plan, nbins, nw, "merge", lvl, nsamps)
+    tot += alg_bytes(plan, nbins, nw, "score", 0, nsamps)
+    return tot
+
+
+def load_traffic(workload: str, kernel: str):
+    """HBM bytes per launch measured with rocprofv3 --pmc (profiles/hbm_traffic.json)."""
+    path = ROOT / "profiles" / "hbm_traffic.json"
+    if not path.exists():
+        return None
+    try:
+        data = json.loads(path.read_text())
+        return data.get(workload, {}).get(kernel)
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(wl: dict, ts_e, ts_v, widths, threads: int) -> dict:
+    """The CPU oracle (C/OpenMP restatement of the reference CPU FFA) on a bounded sample."""
+    from oracle import oracle as orc
+    from oracle.plan import Plan, SearchConfig
+
+    smp = wl["cpu_sample"]
+    ocfg = SearchConfig(wl["nsamps"], wl["tsamp"], wl["nbins"], wl["eta"], smp["param_limits"],
+                        use_fourier=False, bseg_brute=smp["bseg_brute"])
+    nc = Plan(ocfg).ncoords[-1]
+    orc.lib()
+    t0 = time.perf_counter()
+    orc.compute_ffa_scores(ts_e, ts_v, ocfg, widths=widths, nthreads=threads)
+    dt = time.perf_counter() - t0
+    lims = ", ".join(f"[{a:g}, {b:g}]" for a, b in smp["param_limits"])
+    return {
+        "value": nc / dt, "unit": "param trials/s", "cores": threads, "kind": "port",
+        "sample": f"same series and grid rules, param_limits {lims}, bseg_brute "
+                  f"{ocfg.bseg_brute}: {nc} final trials x {len(widths)} widths in {dt:.2f} s",
+    }
+
+
+def main() -> int:
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs the FFA hot path on one GPU; --gpus {args.gpus} is "
+              "not supported (multi-GPU sharding is design-only).", file=sys.stderr)
+        return 2
+    import torch
+
+    from loki_amd import libculoki, libloki
+    from loki_amd.synth import make_series
+
+    wl = WORKLOADS[args.workload]
+    nbins = wl["nbins"]
+    widths = np.array([w for w in WIDTHS if w < nbins], dtype=np.uint64)
+    sig = wl["signal"]
+    ts_e, ts_v, _amp = make_series(wl["nsamps"], wl["tsamp"], wl["seed"], f0=sig["f0"],
+                                   accel=sig["accel"], jerk=sig["jerk"], fwhm=sig["fwhm"],
+                                   snr=sig["snr"], nbins=nbins, widths=widths)
+    cfg = libloki.configs.PulsarSearchConfig(
+        nsamps=wl["nsamps"], tsamp=wl["tsamp"], nbins=nbins, eta=wl["eta"],
+        param_limits=np.array(wl["param_limits"]), use_fourier=False)
+    dev = torch.device("cuda:0")
+    ffa = libculoki.ffa.FFACUDA(cfg, device_id=0)
+    ffa.set_exact_norm(not args.fast_norm)
+    ffa.set_fused(not args.unfused)
+    plan = ffa.plan
+    nc = int(plan.ncoords[-1])
+    nw = int(widths.size)
+    d_e = torch.from_numpy(ts_e).to(dev)
+    d_v = torch.from_numpy(ts_v).to(dev)
+    scores = torch.empty(nc * nw, dtype=torch.float32, device=dev)
+    stream = torch.cuda.Stream(device=dev)
+
+    def step():
+        ffa.scores_device(d_e.data_ptr(), d_v.data_ptr(), widths, scores.data_ptr(), nc * nw,
+                          False, stream.cuda_stream)
+
+    with torch.cuda.stream(stream):
+        for _ in range(args.warmup):
+            step()
+    torch.cuda.synchronize()
+
+    ffa.set_timing(True)
+    per_kernel = defaultdict(list)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    with torch.cuda.stream(stream):
+        for _ in range(args.steps):
+            step()
+            for kind, level, ms in ffa.timings():  # waits for this step's events
+                per_kernel[(kind, level)].append(ms)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ffa.set_timing(False)
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+
+    # signal recovery check (outside the timed region)
+    best = int(torch.argmax(scores).item())
+    shape = [int(x) for x in plan.fold_shapes[-1][1:-2]]
+    idx = np.unravel_index(best // nw, shape)
+    grid = plan.params_dict
+    found = {name: float(grid[name][i]) for name, i in zip(cfg.param_names, idx)}
+    found["snr"] = float(scores[best].item())
+    found["width"] = int(widths[best % nw])
+
+    # dominant kernel and its roofline
+    avg = {k: float(np.mean(v)) for k, v in per_kernel.items()}
+    dom = max(avg, key=avg.get)
+    dom_bytes = alg_bytes(plan, nbins, nw, dom[0], dom[1], wl["nsamps"])
+    achieved = dom_bytes / (avg[dom] * 1e-3) / 1e9
+    dom_name = f"{dom[0]}_L{dom[1]}"
+    traffic = load_traffic(args.workload, dom_name)
+    kern_ms = {f"{k}_L{l}": round(v, 4) for (k, l), v in sorted(avg.items(), key=lambda x: -x[1])}
+    total_alg = pipeline_bytes(plan, nbins, nw, wl["nsamps"])
+
+    result = {
+        "metric": "FFA param trials/s + % HBM roofline",
+        "value": nc / (ms_per_step * 1e-3),
+        "unit": "param trials/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": args.workload, "nsamps": wl["nsamps"], "tsamp": wl["tsamp"],
+            "nbins": nbins, "eta": wl["eta"], "param_limits": wl["param_limits"],
+            "widths": [int(w) for w in widths], "fold_final": [int(x) for x in plan.fold_shapes[-1]],
+            "n_levels": int(plan.n_levels), "trials_per_step": nc,
+            "score_norm": "hw_rsqrt" if args.fast_norm else "exact",
+            "final_level": "unfused" if args.unfused else "fused",
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": dom_name, "achieved": achieved, "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
+            "alg_bytes_per_launch": dom_bytes, "avg_ms": avg[dom],
+        },
+        "pipeline_alg_GBps": total_alg / (ms_per_step * 1e-3) / 1e9,
+        "kernel_ms": kern_ms,
+        "found": found,
+        "injected": {"freq": sig["f0"], "accel": sig["accel"], "snr": sig["snr"]},
+    }
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        result["cpu_baseline"] = cpu_baseline(wl, ts_e, ts_v, widths, threads)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

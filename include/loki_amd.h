@@ -174,6 +174,30 @@ int loki_ffa_scores_host(loki_ffa_t ffa, const float* ts_e, const float* ts_v, s
                          const size_t* widths, size_t nwidths, float* scores, size_t scores_len,
                          int find_max);
 
+/* Per-kernel timing of the last execute on an FFA object (hipEvent pairs recorded on the
+ * launch stream around each kernel).  No reference counterpart: it replaces the reference's
+ * host-side ScopeTimer / get_brute_fold_timing (ffa.hpp:52, ffa.cpp:74,204-239).
+ * kind: 0 shift-table resolve, 1 brute fold, 2 merge, 3 score, 4 fused last level
+ * (shift tables + merge + score); level: FFA level. */
+typedef struct {
+    int kind;
+    int level;
+    float ms;
+} loki_kernel_timing;
+int loki_ffa_set_timing(loki_ffa_t ffa, int enable);
+/* Scores path: 1 (default) fuses the last level's shift tables, merge and S/N into one kernel
+ * (the final fold cube is never stored); 0 runs the unfused merge + snr_boxcar_3d.  Both give
+ * identical results.  No reference counterpart (the reference always stores the final fold,
+ * ffa_cuda.cu:670-692). */
+int loki_ffa_set_fused(loki_ffa_t ffa, int enable);
+/* Profile normalisation in the fused scores path: 1 (default) the CPU's correctly rounded
+ * e * (1/sqrt(v)), bit-identical to the reference CPU; 0 the hardware reciprocal square root
+ * (|dS/N| ~ 1e-6, the analogue of the reference CUDA path's fast-math e/sqrtf(v),
+ * score_cuda.cu:82).  No reference counterpart as a switch. */
+int loki_ffa_set_exact_norm(loki_ffa_t ffa, int exact);
+/* Waits for the recorded events; *n receives the number of timed launches. */
+int loki_ffa_timings(loki_ffa_t ffa, loki_kernel_timing* out, size_t cap, size_t* n);
+
 /* --- Brute fold (fold.hpp:78-133 BruteFoldCUDA) ------------------------ */
 /* compute_brute_fold_cuda<float> (fold_cuda.cu:260-277): fold has
  * (nsamps/segment_len) * nfreqs * 2 * nbins floats. */

@@ -23,8 +23,8 @@ OBJ = ROOT / "_build" / "obj"
 ARCH = os.environ.get("LOKI_AMD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-HIP_SOURCES = ["k_resolve.hip", "k_brutefold.hip", "k_merge.hip", "k_score.hip", "ffa.hip",
-               "capi.hip"]
+HIP_SOURCES = ["k_resolve.hip", "k_brutefold.hip", "k_merge.hip", "k_score.hip", "k_final.hip",
+               "ffa.hip", "capi.hip"]
 HOST_SOURCES = ["config.cpp", "plan.cpp"]
 
 # -ffp-contract=off: no FMA contraction anywhere (bit-exact FP64 shift tables

@@ -381,6 +381,42 @@ int loki_ffa_scores_host(loki_ffa_t ffa, const float* ts_e, const float* ts_v, s
     });
 }
 
+int loki_ffa_set_timing(loki_ffa_t ffa, int enable) {
+    return guarded([&] {
+        REQ(ffa != nullptr, "null ffa");
+        ffa->exec.set_timing(enable != 0);
+    });
+}
+
+int loki_ffa_set_fused(loki_ffa_t ffa, int enable) {
+    return guarded([&] {
+        REQ(ffa != nullptr, "null ffa");
+        ffa->exec.set_fused(enable != 0);
+    });
+}
+
+int loki_ffa_set_exact_norm(loki_ffa_t ffa, int exact) {
+    return guarded([&] {
+        REQ(ffa != nullptr, "null ffa");
+        ffa->exec.set_exact_norm(exact != 0);
+    });
+}
+
+int loki_ffa_timings(loki_ffa_t ffa, loki_kernel_timing* out, size_t cap, size_t* n) {
+    return guarded([&] {
+        REQ(ffa != nullptr && n != nullptr, "null argument");
+        loki_amd::DeviceGuard g(ffa->exec.device_id());
+        const auto t = ffa->exec.timings();
+        *n           = t.size();
+        REQ(out == nullptr || cap >= t.size(), "output buffer too small");
+        if (out != nullptr) {
+            for (size_t i = 0; i < t.size(); ++i) {
+                out[i] = {static_cast<int>(t[i].kind), t[i].level, t[i].ms};
+            }
+        }
+    });
+}
+
 int loki_brute_fold_time_host(const float* ts_e, const float* ts_v, size_t nsamps,
                               const double* freqs, size_t nfreqs, size_t segment_len,
                               size_t nbins, double tsamp, double t_ref, float* fold,

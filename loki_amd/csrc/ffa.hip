@@ -4,6 +4,7 @@
 // enqueued back to back on the caller's stream.
 #include "ffa.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <string>
 
@@ -71,6 +72,54 @@ DeviceGuard::~DeviceGuard() {
     }
 }
 
+LaunchTimer::~LaunchTimer() {
+    for (auto& sl : m_slots) {
+        (void)hipEventDestroy(sl.a);
+        (void)hipEventDestroy(sl.b);
+    }
+}
+
+LaunchTimer::LaunchTimer(LaunchTimer&& o) noexcept
+    : m_enabled(o.m_enabled), m_slots(std::move(o.m_slots)), m_used(o.m_used) {
+    o.m_slots.clear();
+    o.m_used = 0;
+}
+
+int LaunchTimer::start(KernelKind kind, int level, hipStream_t s) {
+    if (!m_enabled) {
+        return -1;
+    }
+    if (m_used == m_slots.size()) {
+        Slot sl;
+        hip_check(hipEventCreate(&sl.a), "hipEventCreate");
+        hip_check(hipEventCreate(&sl.b), "hipEventCreate");
+        m_slots.push_back(sl);
+    }
+    Slot& sl = m_slots[m_used];
+    sl.kind  = kind;
+    sl.level = level;
+    hip_check(hipEventRecord(sl.a, s), "hipEventRecord");
+    return static_cast<int>(m_used++);
+}
+
+void LaunchTimer::stop(int slot, hipStream_t s) {
+    if (slot < 0) {
+        return;
+    }
+    hip_check(hipEventRecord(m_slots[static_cast<size_t>(slot)].b, s), "hipEventRecord");
+}
+
+std::vector<KernelTiming> LaunchTimer::collect() {
+    std::vector<KernelTiming> out;
+    for (size_t i = 0; i < m_used; ++i) {
+        hip_check(hipEventSynchronize(m_slots[i].b), "hipEventSynchronize");
+        float ms = 0.0F;
+        hip_check(hipEventElapsedTime(&ms, m_slots[i].a, m_slots[i].b), "hipEventElapsedTime");
+        out.push_back({m_slots[i].kind, m_slots[i].level, ms});
+    }
+    return out;
+}
+
 namespace {
 
 kern::ResolveLevel make_level(const FFAPlan& plan, SizeType l) {
@@ -125,18 +174,21 @@ SizeType FFAExecutor::coord_record_bytes() const noexcept {
     return m_freq_only ? sizeof(FFACoordFreq) : sizeof(FFACoord);
 }
 
-void FFAExecutor::resolve_coordinates(void* coords_dev, hipStream_t s) const {
+void FFAExecutor::resolve_coordinates(void* coords_dev, hipStream_t s, SizeType last_level) {
     auto* base = static_cast<char*>(coords_dev);
-    for (SizeType l = 1; l < m_plan.n_levels(); ++l) {
+    const SizeType top = std::min(last_level, m_plan.n_levels() - 1);
+    for (SizeType l = 1; l <= top; ++l) {
         const auto lv      = make_level(m_plan, l);
         const SizeType off = m_plan.ncoords_offsets64()[l];
         const SizeType n   = m_plan.ncoords()[l];
+        const int slot     = m_timer.start(KernelKind::kResolve, static_cast<int>(l), s);
         if (m_freq_only) {
             kern::resolve_freq(lv, reinterpret_cast<FFACoordFreq*>(base) + off, n, s);
         } else {
             kern::resolve_poly(lv, reinterpret_cast<FFACoord*>(base) + off, n, s);
         }
         check_launch("resolver launch");
+        m_timer.stop(slot, s);
     }
 }
 
@@ -144,50 +196,44 @@ void FFAExecutor::prepare_brute_fold(hipStream_t s) {
     if (m_buckets_ready) {
         return;
     }
-    const auto freqs       = m_plan.compute_param_grid(0).back();
-    const SizeType nfreqs  = freqs.size();
-    const SizeType seglen  = m_plan.segment_lens()[0];
-    const SizeType nbins   = m_plan.config().get_nbins();
-    const double t_ref     = m_freq_only ? 0.0 : m_plan.tsegments()[0] / 2.0;
+    const auto freqs      = m_plan.compute_param_grid(0).back();
+    const SizeType nfreqs = freqs.size();
+    const SizeType seglen = m_plan.segment_lens()[0];
+    const SizeType nbins  = m_plan.config().get_nbins();
+    const double t_ref    = m_freq_only ? 0.0 : m_plan.tsegments()[0] / 2.0;
     LOKI_REQUIRE(seglen < (1ULL << 32) && nfreqs < (1ULL << 32), "brute fold too large");
     m_bf_freqs.reserve(nfreqs * sizeof(double));
-    m_bf_idx.reserve(nfreqs * seglen * sizeof(uint32_t));
-    m_bf_off.reserve(nfreqs * (nbins + 1) * sizeof(uint32_t));
     hip_check(hipMemcpyAsync(m_bf_freqs.get(), freqs.data(), nfreqs * sizeof(double),
                              hipMemcpyHostToDevice, s),
               "copy brute-fold frequencies");
-    kern::brute_fold_buckets(m_bf_freqs.as<double>(), static_cast<uint32_t>(nfreqs),
-                             static_cast<uint32_t>(seglen), static_cast<uint32_t>(nbins),
-                             m_plan.config().get_tsamp(), t_ref, m_bf_idx.as<uint32_t>(),
-                             m_bf_off.as<uint32_t>(), s);
-    check_launch("brute-fold bucket launch");
-    // The host vector `freqs` must outlive the async copy.
-    hip_check(hipStreamSynchronize(s), "brute-fold setup");
+    // synchronises s (so `freqs` outlives the copy)
+    m_bf = kern::build_brute_fold_lists(m_bf_freqs.as<double>(), static_cast<uint32_t>(nfreqs),
+                                        static_cast<uint32_t>(seglen),
+                                        static_cast<uint32_t>(nbins),
+                                        m_plan.config().get_tsamp(), t_ref, s);
     m_buckets_ready = true;
 }
 
-void FFAExecutor::run_levels(const float* ts_e, const float* ts_v, void* out_final,
-                             void* other, hipStream_t s) {
-    const SizeType L = m_plan.n_levels();
+float* FFAExecutor::run_levels(const float* ts_e, const float* ts_v, void* buf_even,
+                              void* buf_odd, SizeType last_level, hipStream_t s) {
+    // Level k lands in buf_even for even k and in buf_odd for odd k (the
+    // reference picks its ping-pong start side the same way so the last level
+    // lands in the caller's buffer, ffa.cpp:261-276).
+    LOKI_REQUIRE(last_level < m_plan.n_levels(), "level out of range");
+    m_timer.reset();
     m_coords.reserve(m_plan.coord_size() * coord_record_bytes());
     prepare_brute_fold(s);
-    resolve_coordinates(m_coords.get(), s);
+    resolve_coordinates(m_coords.get(), s, last_level);
 
-    // ffa.cpp:261-276: start on the side that makes the last level land in
-    // out_final.
-    const bool odd_swaps = ((L - 2) % 2) == 1;
-    auto* cur_in         = static_cast<float*>(odd_swaps ? out_final : other);
-    auto* cur_out        = static_cast<float*>(odd_swaps ? other : out_final);
-
+    auto* even = static_cast<float*>(buf_even);
+    auto* odd  = static_cast<float*>(buf_odd);
     const SizeType nbins = m_plan.config().get_nbins();
-    kern::brute_fold_time(ts_e, ts_v, m_plan.nsegments()[0],
-                          static_cast<uint32_t>(m_plan.segment_lens()[0]),
-                          static_cast<uint32_t>(m_plan.ncoords()[0]),
-                          static_cast<uint32_t>(nbins), m_bf_idx.as<uint32_t>(),
-                          m_bf_off.as<uint32_t>(), cur_in, s);
+    const int bslot      = m_timer.start(KernelKind::kBruteFold, 0, s);
+    kern::brute_fold_time(ts_e, ts_v, m_plan.nsegments()[0], m_bf, even, s);
     check_launch("brute fold launch");
+    m_timer.stop(bslot, s);
 
-    for (SizeType l = 1; l < L; ++l) {
+    for (SizeType l = 1; l <= last_level; ++l) {
         const SizeType off = m_plan.ncoords_offsets64()[l];
         const FFACoord* cp = nullptr;
         const FFACoordFreq* cfp = nullptr;
@@ -196,13 +242,23 @@ void FFAExecutor::run_levels(const float* ts_e, const float* ts_v, void* out_fin
         } else {
             cp = m_coords.as<FFACoord>() + off;
         }
-        kern::merge_time(cur_in, cur_out, cp, cfp, m_plan.ncoords()[l], m_plan.ncoords()[l - 1],
+        float* in  = (l % 2 == 1) ? even : odd;
+        float* out = (l % 2 == 1) ? odd : even;
+        const int slot = m_timer.start(KernelKind::kMerge, static_cast<int>(l), s);
+        kern::merge_time(in, out, cp, cfp, m_plan.ncoords()[l], m_plan.ncoords()[l - 1],
                          m_plan.nsegments()[l], static_cast<uint32_t>(nbins), s);
         check_launch("merge launch");
-        if (l + 1 < L) {
-            std::swap(cur_in, cur_out);
-        }
+        m_timer.stop(slot, s);
     }
+    return (last_level % 2 == 0) ? even : odd;
+}
+
+SizeType FFAExecutor::max_level_size(SizeType last_level) const {
+    SizeType m = 0;
+    for (SizeType l = 0; l <= last_level; ++l) {
+        m = std::max(m, m_plan.level_size(l));
+    }
+    return m;
 }
 
 void FFAExecutor::execute(const float* ts_e, const float* ts_v, void* fold, SizeType fold_len,
@@ -212,9 +268,19 @@ void FFAExecutor::execute(const float* ts_e, const float* ts_v, void* fold, Size
                  "FFA::execute: null buffer");
     LOKI_REQUIRE(fold_len == m_plan.buffer_size(),
                  "FFA::execute: fold must have size buffer_size");
-    const SizeType elem = sizeof(float);
-    m_internal.reserve(m_plan.buffer_size() * elem);
-    run_levels(ts_e, ts_v, fold, m_internal.get(), s);
+    const SizeType L = m_plan.n_levels();
+    m_internal.reserve(m_plan.buffer_size() * sizeof(float));
+    if ((L - 1) % 2 == 0) {
+        run_levels(ts_e, ts_v, fold, m_internal.get(), L - 1, s);
+    } else {
+        run_levels(ts_e, ts_v, m_internal.get(), fold, L - 1, s);
+    }
+}
+
+bool FFAExecutor::fused_scores_supported() const {
+    return kern::final_merge_score_supported(
+               static_cast<uint32_t>(m_plan.config().get_nbins())) &&
+           m_plan.nsegments().back() == 1;
 }
 
 void FFAExecutor::execute_scores(const float* ts_e, const float* ts_v,
@@ -223,18 +289,37 @@ void FFAExecutor::execute_scores(const float* ts_e, const float* ts_v,
     DeviceGuard g(m_device);
     LOKI_REQUIRE(m_plan.nsegments().back() == 1,
                  "compute_ffa_scores: nsegments must be 1 for scores");
+    LOKI_REQUIRE(ts_e != nullptr && ts_v != nullptr, "FFA::execute: null buffer");
+    const SizeType L       = m_plan.n_levels();
     const SizeType nbins   = m_plan.config().get_nbins();
     const SizeType ncoords = m_plan.ncoords().back();
     const auto wt          = kern::make_width_table(widths.data(), widths.size(), nbins);
     LOKI_REQUIRE(scores != nullptr, "scores buffer is null");
     LOKI_REQUIRE(scores_len == (find_max ? ncoords : ncoords * widths.size()),
                  "scores must have size ncoords * nwidths (or ncoords for the max)");
+    if (m_fused && fused_scores_supported()) {
+        // Levels 0..L-2 through HBM, the last level fused with its shift
+        // tables and the S/N: the final fold cube is never stored.
+        const SizeType need = max_level_size(L - 2) * sizeof(float);
+        m_internal.reserve(need);
+        m_result.reserve(need);
+        const float* parent =
+            run_levels(ts_e, ts_v, m_result.get(), m_internal.get(), L - 2, s);
+        const auto lv  = make_level(m_plan, L - 1);
+        const int slot = m_timer.start(KernelKind::kMergeScore, static_cast<int>(L - 1), s);
+        kern::final_merge_score(parent, ncoords, m_plan.ncoords()[L - 2], lv, wt, scores,
+                                find_max, m_exact_norm, s);
+        check_launch("final-level merge+score launch");
+        m_timer.stop(slot, s);
+        return;
+    }
     m_internal.reserve(m_plan.buffer_size() * sizeof(float));
     m_result.reserve(m_plan.buffer_size() * sizeof(float));
-    run_levels(ts_e, ts_v, m_result.get(), m_internal.get(), s);
-    kern::snr_boxcar_3d(m_result.as<float>(), ncoords, static_cast<uint32_t>(nbins), wt, scores,
-                        find_max, s);
+    const float* fin = run_levels(ts_e, ts_v, m_result.get(), m_internal.get(), L - 1, s);
+    const int slot   = m_timer.start(KernelKind::kScore, static_cast<int>(L - 1), s);
+    kern::snr_boxcar_3d(fin, ncoords, static_cast<uint32_t>(nbins), wt, scores, find_max, s);
     check_launch("score launch");
+    m_timer.stop(slot, s);
 }
 
 void brute_fold_time_device(const float* ts_e, const float* ts_v, SizeType nsamps,
@@ -244,21 +329,17 @@ void brute_fold_time_device(const float* ts_e, const float* ts_v, SizeType nsamp
     LOKI_REQUIRE(segment_len > 0 && nsamps % segment_len == 0,
                  "BruteFold: Number of samples is not a multiple of segment length");
     LOKI_REQUIRE(nbins >= 1 && nbins <= 1024, "BruteFold: nbins must be in [1, 1024]");
+    LOKI_REQUIRE(segment_len < (1ULL << 32), "BruteFold: segment too long");
     DeviceBuffer d_freqs(freqs.size() * sizeof(double));
-    DeviceBuffer d_idx(freqs.size() * segment_len * sizeof(uint32_t));
-    DeviceBuffer d_off(freqs.size() * (nbins + 1) * sizeof(uint32_t));
     hip_check(hipMemcpyAsync(d_freqs.get(), freqs.data(), freqs.size() * sizeof(double),
                              hipMemcpyHostToDevice, s),
               "copy frequencies");
-    kern::brute_fold_buckets(d_freqs.as<double>(), static_cast<uint32_t>(freqs.size()),
-                             static_cast<uint32_t>(segment_len), static_cast<uint32_t>(nbins),
-                             tsamp, t_ref, d_idx.as<uint32_t>(), d_off.as<uint32_t>(), s);
-    check_launch("brute-fold bucket launch");
-    kern::brute_fold_time(ts_e, ts_v, nsamps / segment_len, static_cast<uint32_t>(segment_len),
-                          static_cast<uint32_t>(freqs.size()), static_cast<uint32_t>(nbins),
-                          d_idx.as<uint32_t>(), d_off.as<uint32_t>(), fold, s);
+    const auto lists = kern::build_brute_fold_lists(
+        d_freqs.as<double>(), static_cast<uint32_t>(freqs.size()),
+        static_cast<uint32_t>(segment_len), static_cast<uint32_t>(nbins), tsamp, t_ref, s);
+    kern::brute_fold_time(ts_e, ts_v, nsamps / segment_len, lists, fold, s);
     check_launch("brute fold launch");
-    // Temporaries are freed on return: finish before releasing them.
+    // The lists are freed on return: finish before releasing them.
     hip_check(hipStreamSynchronize(s), "brute fold");
 }
 

@@ -9,52 +9,64 @@
 #include <span>
 #include <vector>
 
+#include "device.hpp"
+#include "kernels.hpp"
 #include "plan.hpp"
 
 namespace loki_amd {
 
-/// RAII device allocation.
-class DeviceBuffer {
-public:
-    DeviceBuffer() = default;
-    explicit DeviceBuffer(SizeType bytes);
-    ~DeviceBuffer();
-    DeviceBuffer(DeviceBuffer&& o) noexcept : m_ptr(o.m_ptr), m_bytes(o.m_bytes) {
-        o.m_ptr   = nullptr;
-        o.m_bytes = 0;
-    }
-    DeviceBuffer& operator=(DeviceBuffer&& o) noexcept;
-    DeviceBuffer(const DeviceBuffer&)            = delete;
-    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
-    void* get() const noexcept { return m_ptr; }
-    SizeType bytes() const noexcept { return m_bytes; }
-    template <typename T> T* as() const noexcept { return static_cast<T*>(m_ptr); }
-    /// Grow (never shrink) to at least `bytes`; contents are not preserved.
-    void reserve(SizeType bytes);
-
-private:
-    void* m_ptr{nullptr};
-    SizeType m_bytes{0};
+/// Kernel classes reported by the timing hooks.
+enum class KernelKind : int {
+    kResolve    = 0,
+    kBruteFold  = 1,
+    kMerge      = 2,
+    kScore      = 3,
+    kMergeScore = 4,  ///< fused last level (shift tables + merge + S/N)
 };
 
-void hip_check(hipError_t err, const char* what);
+struct KernelTiming {
+    KernelKind kind;
+    int level;
+    float ms;
+};
 
-/// Current-device guard (cuda_utils.cuh:434 CudaSetDeviceGuard).
-class DeviceGuard {
+/// hipEvent pairs recorded around each launch on the launch stream.
+class LaunchTimer {
 public:
-    explicit DeviceGuard(int device);
-    ~DeviceGuard();
-    DeviceGuard(const DeviceGuard&)            = delete;
-    DeviceGuard& operator=(const DeviceGuard&) = delete;
+    LaunchTimer() = default;
+    ~LaunchTimer();
+    LaunchTimer(const LaunchTimer&)            = delete;
+    LaunchTimer& operator=(const LaunchTimer&) = delete;
+    LaunchTimer(LaunchTimer&& o) noexcept;
+    LaunchTimer& operator=(LaunchTimer&& o) noexcept = delete;
+    bool enabled() const noexcept { return m_enabled; }
+    void set_enabled(bool on) noexcept { m_enabled = on; }
+    void reset() noexcept { m_used = 0; }
+    /// Records the start event; returns the slot to close with stop().
+    int start(KernelKind kind, int level, hipStream_t s);
+    void stop(int slot, hipStream_t s);
+    /// Waits for the recorded events and returns the elapsed times.
+    std::vector<KernelTiming> collect();
 
 private:
-    int m_prev{0};
-    bool m_changed{false};
+    struct Slot {
+        hipEvent_t a{nullptr};
+        hipEvent_t b{nullptr};
+        KernelKind kind{};
+        int level{};
+    };
+    bool m_enabled{false};
+    std::vector<Slot> m_slots;
+    size_t m_used{0};
 };
 
 class FFAExecutor {
 public:
     FFAExecutor(const SearchConfig& cfg, FoldKind kind, int device_id);
+
+    void set_timing(bool on) noexcept { m_timer.set_enabled(on); }
+    /// Per-kernel durations of the last execute()/execute_scores() call.
+    std::vector<KernelTiming> timings() { return m_timer.collect(); }
 
     const FFAPlan& plan() const noexcept { return m_plan; }
     int device_id() const noexcept { return m_device; }
@@ -70,8 +82,17 @@ public:
     void execute_scores(const float* ts_e, const float* ts_v, std::span<const SizeType> widths,
                         float* scores, SizeType scores_len, bool find_max, hipStream_t s);
 
-    /// Resolve all levels' shift tables into device memory (level 0 zeroed).
-    void resolve_coordinates(void* coords_dev, hipStream_t s) const;
+    /// Resolve the shift tables of levels 1..last_level into device memory
+    /// (at their ncoords_offsets positions).
+    void resolve_coordinates(void* coords_dev, hipStream_t s, SizeType last_level = ~SizeType{0});
+
+    /// Use the fused last-level kernel in execute_scores() when nbins allows
+    /// (default on).  Off: unfused merge + snr_boxcar_3d (same bits).
+    void set_fused(bool on) noexcept { m_fused = on; }
+    /// Profile normalisation of the fused scores path: exact (default, the
+    /// CPU's correctly rounded e * (1/sqrt(v))) or the hardware rsqrt.
+    void set_exact_norm(bool on) noexcept { m_exact_norm = on; }
+    bool fused_scores_supported() const;
 
     /// Device pointer to the last shift tables resolved by execute().
     const void* coords_device() const noexcept { return m_coords.get(); }
@@ -85,13 +106,18 @@ private:
     DeviceBuffer m_result;     // second buffer for execute_scores
     DeviceBuffer m_coords;     // all levels' shift tables
     DeviceBuffer m_bf_freqs;   // level-0 frequencies (FP64)
-    DeviceBuffer m_bf_idx;     // bucket lists
-    DeviceBuffer m_bf_off;
+    kern::BruteFoldLists m_bf; // transposed bucket lists of the brute fold
     bool m_buckets_ready{false};
+    bool m_fused{true};
+    bool m_exact_norm{true};
+    LaunchTimer m_timer;
 
     void prepare_brute_fold(hipStream_t s);
-    void run_levels(const float* ts_e, const float* ts_v, void* out_final, void* other,
-                    hipStream_t s);
+    /// Brute fold + merge levels 1..last_level; returns the buffer holding
+    /// last_level (buf_even for even levels, buf_odd for odd levels).
+    float* run_levels(const float* ts_e, const float* ts_v, void* buf_even, void* buf_odd,
+                      SizeType last_level, hipStream_t s);
+    SizeType max_level_size(SizeType last_level) const;
 };
 
 /// Standalone brute fold (fold_cuda.cu:139-164, 260-277): fold has

@@ -2,36 +2,46 @@
 //
 // The reference CPU path (fold.cpp:113-142, kernels.cpp:550-580) sums each
 // (freq, phase-bin) bucket in ascending sample order; its CUDA path uses
-// float atomics and is order-nondeterministic (kernels_cuda.cu:662-773).  Here
-// the bucket lists are built once per plan (one block per frequency, FP64
-// phases), and the fold runs one lane per (segment, freq, bin) summing its
-// bucket in ascending order -- bit-identical to the CPU order and
-// deterministic.  The loads of a bucket are independent of the running sum,
-// so they are issued in batches and only the FP32 adds form a chain.
+// float atomics and is order-nondeterministic (kernels_cuda.cu:662-773).
+// Here the bucket lists are built once per plan (FP64 phases, one block per
+// frequency) in a transposed layout list[f][k][b] = k-th sample of bin b, so
+// that lanes holding consecutive bins read consecutive words at every step.
+// The fold runs one lane per (segment, freq, bin), summing its bucket in
+// ascending order from samples staged in LDS -- bit-identical to the CPU
+// order and deterministic.
+#include <algorithm>
+#include <vector>
+
+#include "device.hpp"
 #include "grid.hpp"
 #include "kernels.hpp"
 
 namespace loki_amd::kern {
 namespace {
 
-constexpr int kChunk = 4096;  // samples whose bins are staged in LDS at once
+constexpr int kChunk    = 4096;  // samples whose bins are staged in LDS at once
+constexpr int kBlock    = 256;
+constexpr int kPairsBlk = 2048;  // (freq, bin) pairs per fold block
+constexpr int kLdsSamps = 4096;  // segments up to this length are staged in LDS
 
-// One block per frequency; thread b owns phase bin b (b < nbins).  Two passes
-// over the segment: count, then (after an exclusive scan) write the sample
-// indices of each bin in ascending order.
+// Pass 1 (count): one block per frequency, thread b counts the samples of
+// phase bin b; kmax[f] = largest bucket of frequency f.
 __global__ void __launch_bounds__(1024)
-k_bf_buckets(const double* __restrict__ freqs, uint32_t seglen, uint32_t nbins, double tsamp,
-             double t_ref, uint32_t* __restrict__ idx, uint32_t* __restrict__ off) {
+k_bf_count(const double* __restrict__ freqs, uint32_t seglen, uint32_t nbins, double tsamp,
+           double t_ref, uint32_t* __restrict__ counts, uint32_t* __restrict__ kmax) {
     __shared__ uint16_t s_bins[kChunk];
-    __shared__ uint32_t s_start[1025];
+    __shared__ uint32_t s_max;
     const uint32_t f    = blockIdx.x;
     const double freq   = freqs[f];
     const uint32_t tid  = threadIdx.x;
     const uint32_t nthr = blockDim.x;
-
+    if (tid == 0) {
+        s_max = 0;
+    }
     uint32_t count = 0;
     for (uint32_t base = 0; base < seglen; base += kChunk) {
         const uint32_t len = min(static_cast<uint32_t>(kChunk), seglen - base);
+        __syncthreads();
         for (uint32_t k = tid; k < len; k += nthr) {
             const double t = (static_cast<double>(base + k) * tsamp) - t_ref;
             s_bins[k]      = static_cast<uint16_t>(grid::phase_bin(t, freq, nbins));
@@ -42,25 +52,29 @@ k_bf_buckets(const double* __restrict__ freqs, uint32_t seglen, uint32_t nbins, 
                 count += (s_bins[k] == tid) ? 1U : 0U;
             }
         }
-        __syncthreads();
     }
     if (tid < nbins) {
-        s_start[tid + 1] = count;
+        counts[static_cast<uint64_t>(f) * nbins + tid] = count;
+        atomicMax(&s_max, count);
     }
     __syncthreads();
     if (tid == 0) {
-        s_start[0] = 0;
-        for (uint32_t b = 1; b <= nbins; ++b) {
-            s_start[b] += s_start[b - 1];
-        }
+        kmax[f] = s_max;
     }
-    __syncthreads();
-    uint32_t* __restrict__ off_f = off + static_cast<uint64_t>(f) * (nbins + 1);
-    for (uint32_t b = tid; b <= nbins; b += nthr) {
-        off_f[b] = s_start[b];
-    }
-    uint32_t* __restrict__ idx_f = idx + static_cast<uint64_t>(f) * seglen;
-    uint32_t wpos                = (tid < nbins) ? s_start[tid] : 0U;
+}
+
+// Pass 2 (fill): list[base[f] + k * nbins + b] = k-th sample (ascending) of
+// bin b of frequency f.
+__global__ void __launch_bounds__(1024)
+k_bf_fill(const double* __restrict__ freqs, uint32_t seglen, uint32_t nbins, double tsamp,
+          double t_ref, const uint64_t* __restrict__ base_f, uint32_t* __restrict__ list) {
+    __shared__ uint16_t s_bins[kChunk];
+    const uint32_t f    = blockIdx.x;
+    const double freq   = freqs[f];
+    const uint32_t tid  = threadIdx.x;
+    const uint32_t nthr = blockDim.x;
+    uint32_t* __restrict__ lst = list + base_f[f] + tid;
+    uint64_t k_out             = 0;
     for (uint32_t base = 0; base < seglen; base += kChunk) {
         const uint32_t len = min(static_cast<uint32_t>(kChunk), seglen - base);
         __syncthreads();
@@ -72,83 +86,135 @@ k_bf_buckets(const double* __restrict__ freqs, uint32_t seglen, uint32_t nbins, 
         if (tid < nbins) {
             for (uint32_t k = 0; k < len; ++k) {
                 if (s_bins[k] == tid) {
-                    idx_f[wpos++] = base + k;
+                    lst[k_out * nbins] = base + k;
+                    ++k_out;
                 }
             }
         }
     }
 }
 
-// One lane per (segment, freq, bin); consecutive lanes are consecutive bins
-// of one frequency, so the two output rows of a (segment, freq) are written
-// contiguously.
-__global__ void __launch_bounds__(256)
+// One lane per (segment, freq, bin) pair; consecutive lanes are consecutive
+// bins, so list reads and output writes are contiguous.  USE_LDS: the
+// segment's samples are staged in LDS (seglen <= kLdsSamps).
+template <bool USE_LDS>
+__global__ void __launch_bounds__(kBlock)
 k_bf_fold(const float* __restrict__ ts_e, const float* __restrict__ ts_v, uint32_t seglen,
           uint32_t nfreqs, uint32_t nbins, uint32_t blocks_per_seg,
-          const uint32_t* __restrict__ idx, const uint32_t* __restrict__ off,
-          float* __restrict__ fold) {
-    const uint64_t seg  = blockIdx.x / blocks_per_seg;
-    const uint32_t pair = (blockIdx.x % blocks_per_seg) * blockDim.x + threadIdx.x;
-    if (pair >= nfreqs * nbins) {
-        return;
+          const uint32_t* __restrict__ list, const uint64_t* __restrict__ base_f,
+          const uint32_t* __restrict__ counts, float* __restrict__ fold) {
+    __shared__ float s_e[USE_LDS ? kLdsSamps : 1];
+    __shared__ float s_v[USE_LDS ? kLdsSamps : 1];
+    const uint64_t seg   = blockIdx.x / blocks_per_seg;
+    const uint32_t pbase = (blockIdx.x % blocks_per_seg) * kPairsBlk;
+    const uint32_t npair = nfreqs * nbins;
+    const float* __restrict__ ge = ts_e + seg * seglen;
+    const float* __restrict__ gv = ts_v + seg * seglen;
+    if constexpr (USE_LDS) {
+        for (uint32_t k = threadIdx.x; k < seglen; k += kBlock) {
+            s_e[k] = ge[k];
+            s_v[k] = gv[k];
+        }
+        __syncthreads();
     }
-    const uint32_t f = pair / nbins;
-    const uint32_t b = pair - f * nbins;
-    const uint32_t* __restrict__ lst = idx + static_cast<uint64_t>(f) * seglen;
-    const uint32_t beg = off[static_cast<uint64_t>(f) * (nbins + 1) + b];
-    const uint32_t end = off[static_cast<uint64_t>(f) * (nbins + 1) + b + 1];
-    const float* __restrict__ e = ts_e + seg * seglen;
-    const float* __restrict__ v = ts_v + seg * seglen;
-    float se = 0.0F;
-    float sv = 0.0F;
-    uint32_t i = beg;
-    for (; i + 4 <= end; i += 4) {
-        const uint32_t k0 = lst[i], k1 = lst[i + 1], k2 = lst[i + 2], k3 = lst[i + 3];
-        const float e0 = e[k0], e1 = e[k1], e2 = e[k2], e3 = e[k3];
-        const float v0 = v[k0], v1 = v[k1], v2 = v[k2], v3 = v[k3];
-        se = se + e0;
-        sv = sv + v0;
-        se = se + e1;
-        sv = sv + v1;
-        se = se + e2;
-        sv = sv + v2;
-        se = se + e3;
-        sv = sv + v3;
+    const float* __restrict__ e = USE_LDS ? s_e : ge;
+    const float* __restrict__ v = USE_LDS ? s_v : gv;
+    const uint32_t pend = min(pbase + static_cast<uint32_t>(kPairsBlk), npair);
+    for (uint32_t p = pbase + threadIdx.x; p < pend; p += kBlock) {
+        const uint32_t f   = p / nbins;
+        const uint32_t b   = p - f * nbins;
+        const uint32_t cnt = counts[p];
+        const uint32_t* __restrict__ lst = list + base_f[f] + b;
+        float se = 0.0F;
+        float sv = 0.0F;
+        uint32_t k = 0;
+        for (; k + 4 <= cnt; k += 4) {
+            const uint32_t i0 = lst[static_cast<uint64_t>(k) * nbins];
+            const uint32_t i1 = lst[static_cast<uint64_t>(k + 1) * nbins];
+            const uint32_t i2 = lst[static_cast<uint64_t>(k + 2) * nbins];
+            const uint32_t i3 = lst[static_cast<uint64_t>(k + 3) * nbins];
+            const float e0 = e[i0], e1 = e[i1], e2 = e[i2], e3 = e[i3];
+            const float v0 = v[i0], v1 = v[i1], v2 = v[i2], v3 = v[i3];
+            se = se + e0;
+            sv = sv + v0;
+            se = se + e1;
+            sv = sv + v1;
+            se = se + e2;
+            sv = sv + v2;
+            se = se + e3;
+            sv = sv + v3;
+        }
+        for (; k < cnt; ++k) {
+            const uint32_t i = lst[static_cast<uint64_t>(k) * nbins];
+            se = se + e[i];
+            sv = sv + v[i];
+        }
+        float* __restrict__ out = fold + (seg * nfreqs + f) * 2 * nbins;
+        out[b]         = se;
+        out[nbins + b] = sv;
     }
-    for (; i < end; ++i) {
-        const uint32_t k = lst[i];
-        se = se + e[k];
-        sv = sv + v[k];
-    }
-    float* __restrict__ out = fold + (seg * nfreqs + f) * 2 * nbins;
-    out[b]         = se;
-    out[nbins + b] = sv;
 }
 
 }  // namespace
 
-void brute_fold_buckets(const double* freqs, uint32_t nfreqs, uint32_t seglen, uint32_t nbins,
-                        double tsamp, double t_ref, uint32_t* idx, uint32_t* off,
-                        hipStream_t s) {
+BruteFoldLists build_brute_fold_lists(const double* freqs_dev, uint32_t nfreqs, uint32_t seglen,
+                                      uint32_t nbins, double tsamp, double t_ref,
+                                      hipStream_t s) {
     LOKI_REQUIRE(nbins >= 1 && nbins <= 1024, "brute fold supports 1 <= nbins <= 1024");
     LOKI_REQUIRE(nfreqs >= 1 && seglen >= 1, "brute fold: empty frequency or segment");
+    BruteFoldLists L;
+    L.nfreqs = nfreqs;
+    L.seglen = seglen;
+    L.nbins  = nbins;
+    L.counts.reserve(static_cast<SizeType>(nfreqs) * nbins * sizeof(uint32_t));
+    DeviceBuffer kmax(static_cast<SizeType>(nfreqs) * sizeof(uint32_t));
     const uint32_t threads = ((nbins + 63) / 64) * 64;
-    k_bf_buckets<<<nfreqs, threads, 0, s>>>(freqs, seglen, nbins, tsamp, t_ref, idx, off);
+    k_bf_count<<<nfreqs, threads, 0, s>>>(freqs_dev, seglen, nbins, tsamp, t_ref,
+                                          L.counts.as<uint32_t>(), kmax.as<uint32_t>());
+    hip_check(hipGetLastError(), "brute-fold count launch");
+    std::vector<uint32_t> h_kmax(nfreqs);
+    hip_check(hipMemcpyAsync(h_kmax.data(), kmax.get(), nfreqs * sizeof(uint32_t),
+                             hipMemcpyDeviceToHost, s),
+              "copy bucket sizes");
+    hip_check(hipStreamSynchronize(s), "brute-fold count");
+    std::vector<uint64_t> h_base(nfreqs);
+    uint64_t total = 0;
+    for (uint32_t f = 0; f < nfreqs; ++f) {
+        h_base[f] = total;
+        total += static_cast<uint64_t>(h_kmax[f]) * nbins;
+    }
+    L.base.reserve(static_cast<SizeType>(nfreqs) * sizeof(uint64_t));
+    L.list.reserve(std::max<uint64_t>(total, 1) * sizeof(uint32_t));
+    hip_check(hipMemcpyAsync(L.base.get(), h_base.data(), nfreqs * sizeof(uint64_t),
+                             hipMemcpyHostToDevice, s),
+              "copy bucket offsets");
+    k_bf_fill<<<nfreqs, threads, 0, s>>>(freqs_dev, seglen, nbins, tsamp, t_ref,
+                                         L.base.as<uint64_t>(), L.list.as<uint32_t>());
+    hip_check(hipGetLastError(), "brute-fold fill launch");
+    // h_base must outlive its async copy.
+    hip_check(hipStreamSynchronize(s), "brute-fold lists");
+    return L;
 }
 
-void brute_fold_time(const float* ts_e, const float* ts_v, uint64_t nseg, uint32_t seglen,
-                     uint32_t nfreqs, uint32_t nbins, const uint32_t* idx, const uint32_t* off,
-                     float* fold, hipStream_t s) {
-    const uint64_t pairs = static_cast<uint64_t>(nfreqs) * nbins;
+void brute_fold_time(const float* ts_e, const float* ts_v, uint64_t nseg,
+                     const BruteFoldLists& L, float* fold, hipStream_t s) {
+    const uint64_t pairs = static_cast<uint64_t>(L.nfreqs) * L.nbins;
     LOKI_REQUIRE(pairs < (1ULL << 31), "brute fold: too many (freq, bin) pairs");
-    const uint32_t bps    = static_cast<uint32_t>((pairs + 255) / 256);
+    const uint32_t bps    = static_cast<uint32_t>((pairs + kPairsBlk - 1) / kPairsBlk);
     const uint64_t blocks = nseg * bps;
     LOKI_REQUIRE(blocks < (1ULL << 31), "brute fold grid too large");
     if (blocks == 0) {
         return;
     }
-    k_bf_fold<<<static_cast<unsigned>(blocks), 256, 0, s>>>(ts_e, ts_v, seglen, nfreqs, nbins,
-                                                            bps, idx, off, fold);
+    if (L.seglen <= static_cast<uint32_t>(kLdsSamps)) {
+        k_bf_fold<true><<<static_cast<unsigned>(blocks), kBlock, 0, s>>>(
+            ts_e, ts_v, L.seglen, L.nfreqs, L.nbins, bps, L.list.as<uint32_t>(),
+            L.base.as<uint64_t>(), L.counts.as<uint32_t>(), fold);
+    } else {
+        k_bf_fold<false><<<static_cast<unsigned>(blocks), kBlock, 0, s>>>(
+            ts_e, ts_v, L.seglen, L.nfreqs, L.nbins, bps, L.list.as<uint32_t>(),
+            L.base.as<uint64_t>(), L.counts.as<uint32_t>(), fold);
+    }
 }
 
 }  // namespace loki_amd::kern

@@ -1,0 +1,219 @@
+// Fused last FFA level: FP64 shift tables + merge + boxcar S/N, one pass.
+//
+// compute_ffa_scores (ffa.cpp:444-466) writes the final fold cube and reads
+// it back for snr_boxcar_3d; here the final rows never reach HBM.  Each
+// 64-lane workgroup owns 64 consecutive final coordinates:
+//   1. lane i resolves coordinate c0+i in FP64 (taylor_ffa.cpp:19-271 order,
+//      the same bits as the table kernels) into LDS;
+//   2. the wave merges the 64 rows cooperatively: nbins/4 lanes per row, each
+//      lane owning 4 phase bins of both e and v, so it forms the merged e and
+//      v (kernels.cpp:71-146: one FP32 add each) and directly the normalised
+//      profile p = e * (1/sqrt(v)) (score.cpp:139-141); loads of several rows
+//      are in flight at once;
+//   3. lane i scores row i like snr_boxcar_3d (score.cpp:95-172): sequential
+//      circular prefix sum in VGPRs + its LDS row, max window difference per
+//      width, snr = (h + b) * dmax - b * sum.
+// Every FP32 operation matches the unfused path and the CPU oracle bit for
+// bit.
+#include <cfloat>
+
+#include "merge_device.hpp"
+#include "resolve_device.hpp"
+#include "score_device.hpp"
+
+namespace loki_amd::kern {
+namespace {
+
+constexpr uint32_t kRows = 64;
+
+template <bool EXACT>
+__device__ __forceinline__ float4 normalise4(const float4 e, const float4 v) {
+    float4 p;
+    p.x = normalise<EXACT>(e.x, v.x);
+    p.y = normalise<EXACT>(e.y, v.y);
+    p.z = normalise<EXACT>(e.z, v.z);
+    p.w = normalise<EXACT>(e.w, v.w);
+    return p;
+}
+
+template <int NB, int NP, bool FINDMAX, bool EXACT>
+__global__ void __launch_bounds__(kRows)
+k_final_merge_score(const float* __restrict__ in, uint64_t ncur, uint64_t nprev,
+                    ResolveLevel lv, FinalGrid fg, WidthTable wt, float* __restrict__ scores,
+                    uint32_t nblocks) {
+    constexpr uint32_t C4  = NB / 4;        // float4 chunks per component row
+    constexpr uint32_t LPR = C4;            // lanes per row (4 bins of e and v each)
+    constexpr uint32_t RPI = kRows / LPR;   // rows per merge step
+    constexpr uint32_t NSTEP = kRows / RPI; // merge steps for a full tile
+    constexpr uint32_t U   = NSTEP >= 4 ? 4 : NSTEP;  // rows with loads in flight
+    extern __shared__ float s_p[];          // [kRows][stride]
+    __shared__ FFACoord s_coord[kRows];
+    const uint32_t stride = fg.stride;
+
+    const uint32_t bid  = xcd_remap(blockIdx.x, nblocks);
+    const uint64_t c0   = static_cast<uint64_t>(bid) * kRows;
+    const uint32_t lane = threadIdx.x;
+    const uint64_t c    = c0 + lane;
+    const uint32_t nrow =
+        static_cast<uint32_t>(ncur - c0 < kRows ? ncur - c0 : static_cast<uint64_t>(kRows));
+
+    // 1. shift tables for this tile (FP64)
+    if (c < ncur) {
+        s_coord[lane] = resolve_pair<NP>(lv, fg.dv, c);
+    }
+    __syncthreads();
+
+    // 2. merge + normalise: lane (rsub, i4) produces p[4*i4 .. 4*i4+3] of
+    //    rows rsub, rsub + RPI, ...
+    {
+        const uint32_t rsub = lane / LPR;
+        const uint32_t i4   = lane % LPR;
+#pragma unroll
+        for (uint32_t s0 = 0; s0 < NSTEP; s0 += U) {
+            float4 te[U], tv[U], he[U], hv[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) {
+                const uint32_t r = rsub + (s0 + u) * RPI;
+                if (r < nrow) {
+                    const FFACoord cd = s_coord[r];
+                    const float4* __restrict__ trow = reinterpret_cast<const float4*>(
+                        in + static_cast<uint64_t>(cd.i_tail) * (2 * NB));
+                    const float4* __restrict__ hrow = reinterpret_cast<const float4*>(
+                        in + (nprev + static_cast<uint64_t>(cd.i_head)) * (2 * NB));
+                    if constexpr (NP == 1) {
+                        te[u] = trow[i4];
+                        tv[u] = trow[C4 + i4];
+                    } else {
+                        const uint32_t St = shift_int(cd.shift_tail, NB);
+                        te[u] = rotated4<NB>(trow, i4, St);
+                        tv[u] = rotated4<NB>(trow + C4, i4, St);
+                    }
+                    const uint32_t Sh = shift_int(cd.shift_head, NB);
+                    he[u] = rotated4<NB>(hrow, i4, Sh);
+                    hv[u] = rotated4<NB>(hrow + C4, i4, Sh);
+                }
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) {
+                const uint32_t r = rsub + (s0 + u) * RPI;
+                if (r < nrow) {
+                    float4 e, v;
+                    e.x = te[u].x + he[u].x;
+                    e.y = te[u].y + he[u].y;
+                    e.z = te[u].z + he[u].z;
+                    e.w = te[u].w + he[u].w;
+                    v.x = tv[u].x + hv[u].x;
+                    v.y = tv[u].y + hv[u].y;
+                    v.z = tv[u].z + hv[u].z;
+                    v.w = tv[u].w + hv[u].w;
+                    const float4 p = normalise4<EXACT>(e, v);
+                    float* __restrict__ dst = s_p + r * stride + 4 * i4;
+                    dst[0] = p.x;
+                    dst[1] = p.y;
+                    dst[2] = p.z;
+                    dst[3] = p.w;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (c >= ncur) {
+        return;
+    }
+
+    // 3. boxcar S/N of row `lane`: the row's slots become the circular
+    //    prefix sum (n + wmax entries, stride >= n + wmax).
+    float* __restrict__ row = s_p + lane * stride;
+    float ps[NB];
+    float acc = 0.0F;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        const float x = row[j];
+        acc           = (j == 0) ? x : acc + x;
+        ps[j]         = acc;
+    }
+    const float sum = ps[NB - 1];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        row[j] = ps[j];
+    }
+    for (uint32_t j = 0; j < wt.wmax; ++j) {
+        row[NB + j] = row[j] + sum;
+    }
+    float best = -FLT_MAX;
+    for (uint32_t iw = 0; iw < wt.n; ++iw) {
+        const float dmax = window_max<NB>(row + wt.w[iw], ps);
+        const float snr  = ((wt.h[iw] + wt.b[iw]) * dmax) - (wt.b[iw] * sum);
+        if constexpr (FINDMAX) {
+            best = (best < snr) ? snr : best;
+        } else {
+            scores[c * wt.n + iw] = snr;
+        }
+    }
+    if constexpr (FINDMAX) {
+        scores[c] = best;
+    }
+}
+
+template <int NB, int NP>
+void launch_nb_np(const float* in, uint64_t ncur, uint64_t nprev, const ResolveLevel& lv,
+                  const WidthTable& wt, float* scores, bool find_max, bool exact, hipStream_t s) {
+    const uint64_t blocks = (ncur + kRows - 1) / kRows;
+    LOKI_REQUIRE(blocks < (1ULL << 31), "final-level grid too large");
+    const auto nb = static_cast<uint32_t>(blocks);
+    FinalGrid fg{};
+    for (int k = 0; k < 3; ++k) {
+        fg.dv[k] = (lv.hi[k] - lv.lo[k]) / static_cast<double>(lv.n_cur[k]);
+    }
+    fg.stride        = (static_cast<uint32_t>(NB) + wt.wmax) | 1U;  // odd: conflict-free rows
+    const size_t lds = static_cast<size_t>(kRows) * fg.stride * sizeof(float);
+    if (find_max && exact) {
+        k_final_merge_score<NB, NP, true, true>
+            <<<nb, kRows, lds, s>>>(in, ncur, nprev, lv, fg, wt, scores, nb);
+    } else if (find_max) {
+        k_final_merge_score<NB, NP, true, false>
+            <<<nb, kRows, lds, s>>>(in, ncur, nprev, lv, fg, wt, scores, nb);
+    } else if (exact) {
+        k_final_merge_score<NB, NP, false, true>
+            <<<nb, kRows, lds, s>>>(in, ncur, nprev, lv, fg, wt, scores, nb);
+    } else {
+        k_final_merge_score<NB, NP, false, false>
+            <<<nb, kRows, lds, s>>>(in, ncur, nprev, lv, fg, wt, scores, nb);
+    }
+}
+
+template <int NB>
+void launch_nb(const float* in, uint64_t ncur, uint64_t nprev, const ResolveLevel& lv,
+               const WidthTable& wt, float* scores, bool find_max, bool exact, hipStream_t s) {
+    switch (lv.np) {
+    case 1: return launch_nb_np<NB, 1>(in, ncur, nprev, lv, wt, scores, find_max, exact, s);
+    case 2: return launch_nb_np<NB, 2>(in, ncur, nprev, lv, wt, scores, find_max, exact, s);
+    case 3: return launch_nb_np<NB, 3>(in, ncur, nprev, lv, wt, scores, find_max, exact, s);
+    default: LOKI_REQUIRE(false, "final-level kernel: np must be 1, 2 or 3");
+    }
+}
+
+}  // namespace
+
+bool final_merge_score_supported(uint32_t nbins) {
+    return nbins == 8 || nbins == 16 || nbins == 32 || nbins == 64;
+}
+
+void final_merge_score(const float* in, uint64_t ncur, uint64_t nprev, const ResolveLevel& lv,
+                       const WidthTable& wt, float* scores, bool find_max, bool exact,
+                       hipStream_t s) {
+    LOKI_REQUIRE(final_merge_score_supported(lv.nbins), "final-level kernel: unsupported nbins");
+    LOKI_REQUIRE(wt.wmax < lv.nbins, "boxcar widths must be < nbins");
+    if (ncur == 0) {
+        return;
+    }
+    switch (lv.nbins) {
+    case 8: return launch_nb<8>(in, ncur, nprev, lv, wt, scores, find_max, exact, s);
+    case 16: return launch_nb<16>(in, ncur, nprev, lv, wt, scores, find_max, exact, s);
+    case 32: return launch_nb<32>(in, ncur, nprev, lv, wt, scores, find_max, exact, s);
+    case 64: return launch_nb<64>(in, ncur, nprev, lv, wt, scores, find_max, exact, s);
+    default: break;
+    }
+}
+
+}  // namespace loki_amd::kern

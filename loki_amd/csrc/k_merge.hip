@@ -15,49 +15,12 @@
 // remapped so each XCD works on a contiguous range of tiles (parent reuse
 // stays inside one L2).  All offsets are 64-bit.
 #include "kernels.hpp"
+#include "merge_device.hpp"
 
 namespace loki_amd::kern {
 namespace {
 
 constexpr int kBlock = 256;
-
-__device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t nblocks) {
-    // Workgroups are dispatched round-robin over the 8 XCDs; give XCD x the
-    // contiguous logical range [x*per(+rem), ...).
-    constexpr uint32_t kXcd = 8;
-    const uint32_t per = nblocks / kXcd;
-    const uint32_t rem = nblocks % kXcd;
-    const uint32_t xcd = bid % kXcd;
-    const uint32_t loc = bid / kXcd;
-    return xcd < rem ? xcd * (per + 1) + loc : rem * (per + 1) + (xcd - rem) * per + loc;
-}
-
-__device__ __forceinline__ float4 funnel4(const float4 a, const float4 b, uint32_t r) {
-    float4 o;
-    o.x = r == 0 ? a.x : (r == 1 ? a.y : (r == 2 ? a.z : a.w));
-    o.y = r == 0 ? a.y : (r == 1 ? a.z : (r == 2 ? a.w : b.x));
-    o.z = r == 0 ? a.z : (r == 1 ? a.w : (r == 2 ? b.x : b.y));
-    o.w = r == 0 ? a.w : (r == 1 ? b.x : (r == 2 ? b.y : b.z));
-    return o;
-}
-
-__device__ __forceinline__ uint32_t shift_int(float s, uint32_t nbins) {
-    uint32_t k = static_cast<uint32_t>(s + 0.5F);
-    return k == nbins ? 0U : k;
-}
-
-// Rotated float4 (output bins 4*i4 .. 4*i4+3) of one component row.
-template <int NB>
-__device__ __forceinline__ float4 rotated4(const float4* __restrict__ row, uint32_t i4,
-                                           uint32_t S) {
-    constexpr uint32_t C4 = NB / 4;
-    const uint32_t e0 = (4U * i4 + NB - S) & (NB - 1);
-    const uint32_t c0 = e0 >> 2;
-    const uint32_t c1 = (c0 + 1) & (C4 - 1);
-    const float4 a    = row[c0];
-    const float4 b    = row[c1];
-    return funnel4(a, b, e0 & 3U);
-}
 
 template <int NB, bool POLY, int ITERS>
 __global__ void __launch_bounds__(kBlock)
@@ -76,47 +39,62 @@ k_merge_v4(const float* __restrict__ in, float* __restrict__ out,
     const uint32_t lch  = threadIdx.x % LANES;
     const uint64_t in_seg_t = (2 * seg) * nprev;
     const uint64_t in_seg_h = (2 * seg + 1) * nprev;
+    constexpr uint32_t CPL = LPR / LANES;  // chunks per lane per row
+    // All loads of the tile are issued before any add so several rows are in
+    // flight per lane.
+    float4 tv[ITERS][CPL], hv[ITERS][CPL];
 #pragma unroll
     for (int it = 0; it < ITERS; ++it) {
         const uint64_t c = tile * TILE + static_cast<uint64_t>(it) * ROWS + rsub;
-        if (c >= ncur) {
-            break;
-        }
-        uint64_t it_row, ih_row;
-        uint32_t St, Sh;
-        if constexpr (POLY) {
-            const FFACoord cd = coords[c];
-            it_row = in_seg_t + cd.i_tail;
-            ih_row = in_seg_h + cd.i_head;
-            St     = shift_int(cd.shift_tail, NB);
-            Sh     = shift_int(cd.shift_head, NB);
-        } else {
-            const FFACoordFreq cd = cf[c];
-            it_row = in_seg_t + cd.idx;
-            ih_row = in_seg_h + cd.idx;
-            St     = 0;
-            Sh     = shift_int(cd.shift, NB);
-        }
-        const float4* __restrict__ trow = reinterpret_cast<const float4*>(in + it_row * (2 * NB));
-        const float4* __restrict__ hrow = reinterpret_cast<const float4*>(in + ih_row * (2 * NB));
-        float4* __restrict__ orow =
-            reinterpret_cast<float4*>(out + (seg * ncur + c) * (2 * NB));
-        for (uint32_t ch = lch; ch < LPR; ch += LANES) {
-            const uint32_t q  = ch / C4;
-            const uint32_t i4 = ch % C4;
-            float4 tv;
+        if (c < ncur) {
+            uint64_t it_row, ih_row;
+            uint32_t St, Sh;
             if constexpr (POLY) {
-                tv = rotated4<NB>(trow + q * C4, i4, St);
+                const FFACoord cd = coords[c];
+                it_row = in_seg_t + cd.i_tail;
+                ih_row = in_seg_h + cd.i_head;
+                St     = shift_int(cd.shift_tail, NB);
+                Sh     = shift_int(cd.shift_head, NB);
             } else {
-                tv = trow[ch];
+                const FFACoordFreq cd = cf[c];
+                it_row = in_seg_t + cd.idx;
+                ih_row = in_seg_h + cd.idx;
+                St     = 0;
+                Sh     = shift_int(cd.shift, NB);
             }
-            const float4 hv = rotated4<NB>(hrow + q * C4, i4, Sh);
-            float4 o;
-            o.x = tv.x + hv.x;
-            o.y = tv.y + hv.y;
-            o.z = tv.z + hv.z;
-            o.w = tv.w + hv.w;
-            orow[ch] = o;
+            const float4* __restrict__ trow =
+                reinterpret_cast<const float4*>(in + it_row * (2 * NB));
+            const float4* __restrict__ hrow =
+                reinterpret_cast<const float4*>(in + ih_row * (2 * NB));
+#pragma unroll
+            for (uint32_t k = 0; k < CPL; ++k) {
+                const uint32_t ch = lch + k * LANES;
+                const uint32_t q  = ch / C4;
+                const uint32_t i4 = ch % C4;
+                if constexpr (POLY) {
+                    tv[it][k] = rotated4<NB>(trow + q * C4, i4, St);
+                } else {
+                    tv[it][k] = trow[ch];
+                }
+                hv[it][k] = rotated4<NB>(hrow + q * C4, i4, Sh);
+            }
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < ITERS; ++it) {
+        const uint64_t c = tile * TILE + static_cast<uint64_t>(it) * ROWS + rsub;
+        if (c < ncur) {
+            float4* __restrict__ orow =
+                reinterpret_cast<float4*>(out + (seg * ncur + c) * (2 * NB));
+#pragma unroll
+            for (uint32_t k = 0; k < CPL; ++k) {
+                float4 o;
+                o.x = tv[it][k].x + hv[it][k].x;
+                o.y = tv[it][k].y + hv[it][k].y;
+                o.z = tv[it][k].z + hv[it][k].z;
+                o.w = tv[it][k].w + hv[it][k].w;
+                orow[lch + k * LANES] = o;
+            }
         }
     }
 }
@@ -165,7 +143,11 @@ void launch_v4(const float* in, float* out, const FFACoord* coords, const FFACoo
     constexpr uint32_t LPR   = NB / 2;
     constexpr uint32_t LANES = LPR < kBlock ? LPR : kBlock;
     constexpr uint32_t ROWS  = kBlock / LANES;
-    constexpr int ITERS      = ROWS >= 64 ? 1 : static_cast<int>(64 / ROWS);
+    constexpr uint32_t CPL   = LPR / LANES;
+    // rows per lane: up to 64 rows per block, at most 16 float4 loads in flight per lane
+    constexpr uint32_t IT_ROWS = ROWS >= 64 ? 1U : 64U / ROWS;
+    constexpr uint32_t IT_REGS = 8U / CPL;
+    constexpr int ITERS      = static_cast<int>(IT_ROWS < IT_REGS ? IT_ROWS : IT_REGS);
     constexpr uint32_t TILE  = ROWS * ITERS;
     const uint64_t tiles     = (ncur + TILE - 1) / TILE;
     const uint64_t blocks    = tiles * nseg;

@@ -12,6 +12,7 @@
 #include <cmath>
 
 #include "kernels.hpp"
+#include "score_device.hpp"
 
 namespace loki_amd::kern {
 namespace {
@@ -80,13 +81,7 @@ k_snr_reg(const float* __restrict__ folds, uint64_t nprof, WidthTable wt, float 
     }
     float best = -FLT_MAX;
     for (uint32_t iw = 0; iw < wt.n; ++iw) {
-        const float* __restrict__ hi = lds + wt.w[iw];
-        float dmax                   = -FLT_MAX;
-#pragma unroll
-        for (int j = 0; j < NB; ++j) {
-            const float d = hi[j] - ps[j];
-            dmax          = (dmax < d) ? d : dmax;
-        }
+        const float dmax = window_max<NB>(lds + wt.w[iw], ps);
         const float snr = (((wt.h[iw] + wt.b[iw]) * dmax) - (wt.b[iw] * sum)) * inv_stdnoise;
         if constexpr (FINDMAX) {
             best = (best < snr) ? snr : best;
@@ -132,8 +127,7 @@ k_snr_lds(const float* __restrict__ folds, uint64_t nprof, uint32_t nbins, Width
         const uint32_t w = wt.w[iw];
         float dmax       = -FLT_MAX;
         for (uint32_t j = 0; j < nbins; ++j) {
-            const float d = lds[j + w] - lds[j];
-            dmax          = (dmax < d) ? d : dmax;
+            dmax = __builtin_fmaxf(dmax, lds[j + w] - lds[j]);  // see window_max
         }
         const float snr = (((wt.h[iw] + wt.b[iw]) * dmax) - (wt.b[iw] * sum)) * inv_stdnoise;
         if constexpr (FINDMAX) {

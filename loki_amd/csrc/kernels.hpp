@@ -7,6 +7,7 @@
 #include <cstdint>
 
 #include "common.hpp"
+#include "device.hpp"
 #include "plan.hpp"
 
 namespace loki_amd::kern {
@@ -24,21 +25,34 @@ struct ResolveLevel {
     uint32_t np;       ///< 1 (freq), 2 (accel, freq) or 3 (jerk, accel, freq)
 };
 
+/// Extra per-level constants of the fused last-level kernel.
+struct FinalGrid {
+    double dv[3];     ///< (hi - lo) / n_cur per parameter (psr_utils.hpp:290-291)
+    uint32_t stride;  ///< LDS row stride in floats (>= nbins + wmax, odd)
+};
+
 /// Fills ncoords FFACoordFreq records (np == 1) of one level.
 void resolve_freq(const ResolveLevel& lv, FFACoordFreq* out, uint64_t ncoords, hipStream_t s);
 /// Fills ncoords FFACoord records (np == 2 or 3) of one level, tail and head.
 void resolve_poly(const ResolveLevel& lv, FFACoord* out, uint64_t ncoords, hipStream_t s);
 
-/// Bucket lists of the time-domain brute fold: for each frequency, the
-/// sample indices of each phase bin in ascending order (CSR).
-/// idx: [nfreqs * seglen] u32, off: [nfreqs * (nbins + 1)] u32.
-void brute_fold_buckets(const double* freqs, uint32_t nfreqs, uint32_t seglen, uint32_t nbins,
-                        double tsamp, double t_ref, uint32_t* idx, uint32_t* off,
-                        hipStream_t s);
-/// fold[nseg, nfreqs, 2, nbins] from the bucket lists (ascending-order sums).
-void brute_fold_time(const float* ts_e, const float* ts_v, uint64_t nseg, uint32_t seglen,
-                     uint32_t nfreqs, uint32_t nbins, const uint32_t* idx, const uint32_t* off,
-                     float* fold, hipStream_t s);
+/// Bucket lists of the time-domain brute fold, built once per plan: for
+/// each frequency f and phase bin b, the sample indices of the bin in
+/// ascending order, stored transposed: list[base[f] + k * nbins + b].
+struct BruteFoldLists {
+    uint32_t nfreqs{0};
+    uint32_t seglen{0};
+    uint32_t nbins{0};
+    DeviceBuffer counts;  ///< u32 [nfreqs * nbins]
+    DeviceBuffer base;    ///< u64 [nfreqs]
+    DeviceBuffer list;    ///< u32 [sum_f kmax_f * nbins]
+};
+/// FP64 phase bins (psr_utils.hpp:53-63) -> lists; synchronises `s`.
+BruteFoldLists build_brute_fold_lists(const double* freqs_dev, uint32_t nfreqs, uint32_t seglen,
+                                      uint32_t nbins, double tsamp, double t_ref, hipStream_t s);
+/// fold[nseg, nfreqs, 2, nbins] = ascending-order bucket sums of each segment.
+void brute_fold_time(const float* ts_e, const float* ts_v, uint64_t nseg,
+                     const BruteFoldLists& lists, float* fold, hipStream_t s);
 
 /// Time-domain merge of one level (freq-only when coords_freq != nullptr).
 void merge_time(const float* in, float* out, const FFACoord* coords,
@@ -55,6 +69,15 @@ struct WidthTable {
     float b[kMax];
 };
 WidthTable make_width_table(const SizeType* widths, SizeType nw, SizeType nbins);
+
+/// Fused last level (nsegments == 1): FP64 shift tables + merge + boxcar S/N
+/// from the parent level `in` [2, nprev, 2, nbins] straight to scores.
+bool final_merge_score_supported(uint32_t nbins);
+/// exact: p = e * (1/sqrtf(v)) with correctly rounded operations (bit-identical to the CPU);
+/// otherwise p = e * rsq(v) with the hardware reciprocal square root.
+void final_merge_score(const float* in, uint64_t ncur, uint64_t nprev, const ResolveLevel& lv,
+                       const WidthTable& wt, float* scores, bool find_max, bool exact,
+                       hipStream_t s);
 
 /// snr_boxcar_3d[_max]: folds [nprof, 2, nbins] -> scores [nprof, nw] or [nprof].
 void snr_boxcar_3d(const float* folds, uint64_t nprof, uint32_t nbins, const WidthTable& wt,

@@ -264,6 +264,21 @@ public:
                                      reinterpret_cast<float*>(scores), scores_len,
                                      find_max ? 1 : 0, reinterpret_cast<void*>(stream)));
     }
+    void set_timing(bool on) { check(loki_ffa_set_timing(m_h.get(), on ? 1 : 0)); }
+    void set_fused(bool on) { check(loki_ffa_set_fused(m_h.get(), on ? 1 : 0)); }
+    void set_exact_norm(bool on) { check(loki_ffa_set_exact_norm(m_h.get(), on ? 1 : 0)); }
+    py::list timings() {
+        size_t n = 0;
+        check(loki_ffa_timings(m_h.get(), nullptr, 0, &n));
+        std::vector<loki_kernel_timing> t(n);
+        check(loki_ffa_timings(m_h.get(), t.data(), n, &n));
+        static const char* kinds[] = {"resolve", "brute_fold", "merge", "score", "merge_score"};
+        py::list out;
+        for (const auto& x : t) {
+            out.append(py::make_tuple(kinds[x.kind >= 0 && x.kind <= 4 ? x.kind : 0], x.level, x.ms));
+        }
+        return out;
+    }
     py::array_t<float> scores_host(const PyArrayT<float>& ts_e, const PyArrayT<float>& ts_v,
                                    const PyArrayT<size_t>& widths, bool find_max) {
         const size_t nc = m_plan.info().fold_size / (2 * m_cfg.params().nbins);
@@ -540,7 +555,11 @@ PYBIND11_MODULE(_loki_amd, m) {
              py::arg("widths"), py::arg("scores_ptr"), py::arg("scores_len"),
              py::arg("find_max") = false, py::arg("stream") = 0)
         .def("scores", &FFA::scores_host, py::arg("ts_e"), py::arg("ts_v"), py::arg("widths"),
-             py::arg("find_max") = false);
+             py::arg("find_max") = false)
+        .def("set_timing", &FFA::set_timing, py::arg("enable"))
+        .def("set_fused", &FFA::set_fused, py::arg("enable"))
+        .def("set_exact_norm", &FFA::set_exact_norm, py::arg("exact"))
+        .def("timings", &FFA::timings);
 
     m_ffa.def(
         "compute_ffa_time_cuda",

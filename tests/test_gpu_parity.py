@@ -139,6 +139,41 @@ def test_ffa_scores_bit_exact(oracle, name):
     np.testing.assert_array_equal(got2, want2)
 
 
+@pytest.mark.parametrize("name", ["freq16", "acc17w", "acc20", "jerk17w", "jerk20",
+                                  "freq18_nb100", "acc_eta2_bseg"])
+@pytest.mark.parametrize("fused", [True, False])
+@pytest.mark.parametrize("find_max", [False, True])
+def test_scores_fused_and_unfused_paths(oracle, name, fused, find_max):
+    """The fused last-level kernel (FP64 shift tables + merge + S/N, no final fold in HBM) and
+    the unfused merge + snr_boxcar_3d give the oracle's scores bit for bit."""
+    from loki_amd import libculoki
+
+    cfg, ocfg = _cfgs(name)
+    ts_e, ts_v = _series(cfg.nsamps, seed=17)
+    widths = np.array([w for w in (1, 2, 3, 5, 8, 13) if w < cfg.nbins], dtype=np.uint64)
+    ffa = libculoki.ffa.FFACUDA(cfg, device_id=0)
+    ffa.set_fused(fused)
+    got = ffa.scores(ts_e, ts_v, widths, find_max=find_max)
+    want, _ = oracle.compute_ffa_scores(ts_e, ts_v, ocfg, widths=widths)
+    want = want.reshape(-1, widths.size)
+    np.testing.assert_array_equal(got, want.max(axis=1) if find_max else want.reshape(-1))
+
+
+@pytest.mark.parametrize("name", ["freq16", "acc20", "jerk17w"])
+def test_scores_fast_norm_within_tolerance(oracle, name):
+    """Hardware-rsqrt normalisation: S/N within 1e-4 of the oracle (absolute and relative)."""
+    from loki_amd import libculoki
+
+    cfg, ocfg = _cfgs(name)
+    ts_e, ts_v = _series(cfg.nsamps, seed=23)
+    widths = np.array([w for w in (1, 2, 4, 8, 16) if w < cfg.nbins], dtype=np.uint64)
+    ffa = libculoki.ffa.FFACUDA(cfg, device_id=0)
+    ffa.set_exact_norm(False)
+    got = ffa.scores(ts_e, ts_v, widths)
+    want, _ = oracle.compute_ffa_scores(ts_e, ts_v, ocfg, widths=widths)
+    np.testing.assert_allclose(got, want, rtol=1e-4, atol=1e-4)
+
+
 @pytest.mark.parametrize("nbins", [8, 16, 32, 64, 128, 100, 256])
 def test_snr_boxcar_3d_bit_exact(oracle, nbins):
     from loki_amd import libculoki
