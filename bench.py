@@ -60,7 +60,23 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--unfused", action="store_true",
                     help="store the final fold and score it in a separate kernel")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cpu count)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device-to-device copy of BYTES (3 times): the "
+                         "reference kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE")
     return ap.parse_args()
+
+
+def calibrate_copy(nbytes: int) -> int:
+    import torch
+
+    n = nbytes // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_bytes": n * 4, "launches": 3}), flush=True)
+    return 0
 
 
 def alg_bytes(plan, nbins, nw, kind, level, nsamps):
@@ -130,6 +146,8 @@ def main() -> int:
         print(f"bench.py: this build runs the FFA hot path on one GPU; --gpus {args.gpus} is "
               "not supported (multi-GPU sharding is design-only).", file=sys.stderr)
         return 2
+    if args.calibrate_copy:
+        return calibrate_copy(args.calibrate_copy)
     import torch
 
     from loki_amd import libculoki, libloki

@@ -246,7 +246,8 @@ float* FFAExecutor::run_levels(const float* ts_e, const float* ts_v, void* buf_e
         float* out = (l % 2 == 1) ? odd : even;
         const int slot = m_timer.start(KernelKind::kMerge, static_cast<int>(l), s);
         kern::merge_time(in, out, cp, cfp, m_plan.ncoords()[l], m_plan.ncoords()[l - 1],
-                         m_plan.nsegments()[l], static_cast<uint32_t>(nbins), s);
+                         m_plan.nsegments()[l], m_plan.param_counts()[l].back(),
+                         static_cast<uint32_t>(nbins), s);
         check_launch("merge launch");
         m_timer.stop(slot, s);
     }

@@ -24,7 +24,9 @@
 namespace loki_amd::kern {
 namespace {
 
-constexpr uint32_t kRows = 64;
+constexpr uint32_t kRows     = 64;
+constexpr uint32_t kRowGroup = 16;  // child rows scheduled back to back (parent reuse in L2)
+constexpr uint32_t kRun      = 16;  // frequency tiles per sequential output run
 
 template <bool EXACT>
 __device__ __forceinline__ float4 normalise4(const float4 e, const float4 v) {
@@ -40,7 +42,7 @@ template <int NB, int NP, bool FINDMAX, bool EXACT>
 __global__ void __launch_bounds__(kRows)
 k_final_merge_score(const float* __restrict__ in, uint64_t ncur, uint64_t nprev,
                     ResolveLevel lv, FinalGrid fg, WidthTable wt, float* __restrict__ scores,
-                    uint32_t nblocks) {
+                    TileSchedule ts) {
     constexpr uint32_t C4  = NB / 4;        // float4 chunks per component row
     constexpr uint32_t LPR = C4;            // lanes per row (4 bins of e and v each)
     constexpr uint32_t RPI = kRows / LPR;   // rows per merge step
@@ -50,15 +52,16 @@ k_final_merge_score(const float* __restrict__ in, uint64_t ncur, uint64_t nprev,
     __shared__ FFACoord s_coord[kRows];
     const uint32_t stride = fg.stride;
 
-    const uint32_t bid  = xcd_remap(blockIdx.x, nblocks);
-    const uint64_t c0   = static_cast<uint64_t>(bid) * kRows;
+    const Tile tl = tile_of(ts, xcd_remap(blockIdx.x, ts.nblocks), kRows);
+    if (tl.count == 0) {
+        return;  // idle block of a partial row group (before any barrier)
+    }
     const uint32_t lane = threadIdx.x;
-    const uint64_t c    = c0 + lane;
-    const uint32_t nrow =
-        static_cast<uint32_t>(ncur - c0 < kRows ? ncur - c0 : static_cast<uint64_t>(kRows));
+    const uint64_t c    = tl.c0 + lane;
+    const uint32_t nrow = tl.count;
 
     // 1. shift tables for this tile (FP64)
-    if (c < ncur) {
+    if (lane < nrow) {
         s_coord[lane] = resolve_pair<NP>(lv, fg.dv, c);
     }
     __syncthreads();
@@ -117,7 +120,7 @@ k_final_merge_score(const float* __restrict__ in, uint64_t ncur, uint64_t nprev,
         }
     }
     __syncthreads();
-    if (c >= ncur) {
+    if (lane >= nrow) {
         return;
     }
 
@@ -158,9 +161,9 @@ k_final_merge_score(const float* __restrict__ in, uint64_t ncur, uint64_t nprev,
 template <int NB, int NP>
 void launch_nb_np(const float* in, uint64_t ncur, uint64_t nprev, const ResolveLevel& lv,
                   const WidthTable& wt, float* scores, bool find_max, bool exact, hipStream_t s) {
-    const uint64_t blocks = (ncur + kRows - 1) / kRows;
-    LOKI_REQUIRE(blocks < (1ULL << 31), "final-level grid too large");
-    const auto nb = static_cast<uint32_t>(blocks);
+    // nseg == 1 at the last level; child rows sharing parents run back to back
+    const TileSchedule ts = make_schedule(ncur, lv.n_cur[2], 1, kRows, kRowGroup, kRun);
+    const uint32_t nb     = ts.nblocks;
     FinalGrid fg{};
     for (int k = 0; k < 3; ++k) {
         fg.dv[k] = (lv.hi[k] - lv.lo[k]) / static_cast<double>(lv.n_cur[k]);
@@ -169,16 +172,16 @@ void launch_nb_np(const float* in, uint64_t ncur, uint64_t nprev, const ResolveL
     const size_t lds = static_cast<size_t>(kRows) * fg.stride * sizeof(float);
     if (find_max && exact) {
         k_final_merge_score<NB, NP, true, true>
-            <<<nb, kRows, lds, s>>>(in, ncur, nprev, lv, fg, wt, scores, nb);
+            <<<nb, kRows, lds, s>>>(in, ncur, nprev, lv, fg, wt, scores, ts);
     } else if (find_max) {
         k_final_merge_score<NB, NP, true, false>
-            <<<nb, kRows, lds, s>>>(in, ncur, nprev, lv, fg, wt, scores, nb);
+            <<<nb, kRows, lds, s>>>(in, ncur, nprev, lv, fg, wt, scores, ts);
     } else if (exact) {
         k_final_merge_score<NB, NP, false, true>
-            <<<nb, kRows, lds, s>>>(in, ncur, nprev, lv, fg, wt, scores, nb);
+            <<<nb, kRows, lds, s>>>(in, ncur, nprev, lv, fg, wt, scores, ts);
     } else {
         k_final_merge_score<NB, NP, false, false>
-            <<<nb, kRows, lds, s>>>(in, ncur, nprev, lv, fg, wt, scores, nb);
+            <<<nb, kRows, lds, s>>>(in, ncur, nprev, lv, fg, wt, scores, ts);
     }
 }
 

@@ -20,21 +20,25 @@
 namespace loki_amd::kern {
 namespace {
 
-constexpr int kBlock = 256;
+constexpr int kBlock         = 256;
+constexpr uint32_t kRowGroup = 16;  // child rows scheduled back to back (parent reuse in L2)
+constexpr uint32_t kRun      = 16;  // frequency tiles per sequential output run
 
 template <int NB, bool POLY, int ITERS>
 __global__ void __launch_bounds__(kBlock)
 k_merge_v4(const float* __restrict__ in, float* __restrict__ out,
            const FFACoord* __restrict__ coords, const FFACoordFreq* __restrict__ cf,
-           uint64_t ncur, uint64_t nprev, uint32_t tiles_per_seg, uint32_t nblocks) {
+           uint64_t ncur, uint64_t nprev, TileSchedule ts) {
     constexpr uint32_t C4   = NB / 4;        // float4 chunks per component
     constexpr uint32_t LPR  = 2 * C4;        // chunks per row
     constexpr uint32_t LANES = LPR < kBlock ? LPR : kBlock;
     constexpr uint32_t ROWS = kBlock / LANES;
     constexpr uint32_t TILE = ROWS * ITERS;
-    const uint32_t bid  = xcd_remap(blockIdx.x, nblocks);
-    const uint64_t seg  = bid / tiles_per_seg;
-    const uint64_t tile = bid % tiles_per_seg;
+    const Tile tl = tile_of(ts, xcd_remap(blockIdx.x, ts.nblocks), TILE);
+    if (tl.count == 0) {
+        return;
+    }
+    const uint64_t seg  = tl.seg;
     const uint32_t rsub = threadIdx.x / LANES;
     const uint32_t lch  = threadIdx.x % LANES;
     const uint64_t in_seg_t = (2 * seg) * nprev;
@@ -45,8 +49,9 @@ k_merge_v4(const float* __restrict__ in, float* __restrict__ out,
     float4 tv[ITERS][CPL], hv[ITERS][CPL];
 #pragma unroll
     for (int it = 0; it < ITERS; ++it) {
-        const uint64_t c = tile * TILE + static_cast<uint64_t>(it) * ROWS + rsub;
-        if (c < ncur) {
+        const uint32_t local = static_cast<uint32_t>(it) * ROWS + rsub;
+        const uint64_t c     = tl.c0 + local;
+        if (local < tl.count) {
             uint64_t it_row, ih_row;
             uint32_t St, Sh;
             if constexpr (POLY) {
@@ -82,8 +87,9 @@ k_merge_v4(const float* __restrict__ in, float* __restrict__ out,
     }
 #pragma unroll
     for (int it = 0; it < ITERS; ++it) {
-        const uint64_t c = tile * TILE + static_cast<uint64_t>(it) * ROWS + rsub;
-        if (c < ncur) {
+        const uint32_t local = static_cast<uint32_t>(it) * ROWS + rsub;
+        const uint64_t c     = tl.c0 + local;
+        if (local < tl.count) {
             float4* __restrict__ orow =
                 reinterpret_cast<float4*>(out + (seg * ncur + c) * (2 * NB));
 #pragma unroll
@@ -139,7 +145,7 @@ k_merge_scalar(const float* __restrict__ in, float* __restrict__ out,
 
 template <int NB, bool POLY>
 void launch_v4(const float* in, float* out, const FFACoord* coords, const FFACoordFreq* cf,
-               uint64_t ncur, uint64_t nprev, uint64_t nseg, hipStream_t s) {
+               uint64_t ncur, uint64_t nprev, uint64_t nseg, uint64_t n_f, hipStream_t s) {
     constexpr uint32_t LPR   = NB / 2;
     constexpr uint32_t LANES = LPR < kBlock ? LPR : kBlock;
     constexpr uint32_t ROWS  = kBlock / LANES;
@@ -149,27 +155,25 @@ void launch_v4(const float* in, float* out, const FFACoord* coords, const FFACoo
     constexpr uint32_t IT_REGS = 8U / CPL;
     constexpr int ITERS      = static_cast<int>(IT_ROWS < IT_REGS ? IT_ROWS : IT_REGS);
     constexpr uint32_t TILE  = ROWS * ITERS;
-    const uint64_t tiles     = (ncur + TILE - 1) / TILE;
-    const uint64_t blocks    = tiles * nseg;
-    LOKI_REQUIRE(tiles < (1ULL << 31) && blocks < (1ULL << 31), "merge grid too large");
-    k_merge_v4<NB, POLY, ITERS><<<static_cast<unsigned>(blocks), kBlock, 0, s>>>(
-        in, out, coords, cf, ncur, nprev, static_cast<uint32_t>(tiles),
-        static_cast<uint32_t>(blocks));
+    const TileSchedule ts    = make_schedule(ncur, n_f, nseg, TILE, kRowGroup, kRun);
+    k_merge_v4<NB, POLY, ITERS><<<ts.nblocks, kBlock, 0, s>>>(in, out, coords, cf, ncur, nprev,
+                                                              ts);
 }
 
 template <bool POLY>
 void dispatch(const float* in, float* out, const FFACoord* coords, const FFACoordFreq* cf,
-              uint64_t ncur, uint64_t nprev, uint64_t nseg, uint32_t nbins, hipStream_t s) {
+              uint64_t ncur, uint64_t nprev, uint64_t nseg, uint64_t n_f, uint32_t nbins,
+              hipStream_t s) {
     switch (nbins) {
-    case 4: return launch_v4<4, POLY>(in, out, coords, cf, ncur, nprev, nseg, s);
-    case 8: return launch_v4<8, POLY>(in, out, coords, cf, ncur, nprev, nseg, s);
-    case 16: return launch_v4<16, POLY>(in, out, coords, cf, ncur, nprev, nseg, s);
-    case 32: return launch_v4<32, POLY>(in, out, coords, cf, ncur, nprev, nseg, s);
-    case 64: return launch_v4<64, POLY>(in, out, coords, cf, ncur, nprev, nseg, s);
-    case 128: return launch_v4<128, POLY>(in, out, coords, cf, ncur, nprev, nseg, s);
-    case 256: return launch_v4<256, POLY>(in, out, coords, cf, ncur, nprev, nseg, s);
-    case 512: return launch_v4<512, POLY>(in, out, coords, cf, ncur, nprev, nseg, s);
-    case 1024: return launch_v4<1024, POLY>(in, out, coords, cf, ncur, nprev, nseg, s);
+    case 4: return launch_v4<4, POLY>(in, out, coords, cf, ncur, nprev, nseg, n_f, s);
+    case 8: return launch_v4<8, POLY>(in, out, coords, cf, ncur, nprev, nseg, n_f, s);
+    case 16: return launch_v4<16, POLY>(in, out, coords, cf, ncur, nprev, nseg, n_f, s);
+    case 32: return launch_v4<32, POLY>(in, out, coords, cf, ncur, nprev, nseg, n_f, s);
+    case 64: return launch_v4<64, POLY>(in, out, coords, cf, ncur, nprev, nseg, n_f, s);
+    case 128: return launch_v4<128, POLY>(in, out, coords, cf, ncur, nprev, nseg, n_f, s);
+    case 256: return launch_v4<256, POLY>(in, out, coords, cf, ncur, nprev, nseg, n_f, s);
+    case 512: return launch_v4<512, POLY>(in, out, coords, cf, ncur, nprev, nseg, n_f, s);
+    case 1024: return launch_v4<1024, POLY>(in, out, coords, cf, ncur, nprev, nseg, n_f, s);
     default: break;
     }
     const uint64_t total = nseg * ncur * nbins;
@@ -182,7 +186,7 @@ void dispatch(const float* in, float* out, const FFACoord* coords, const FFACoor
 
 void merge_time(const float* in, float* out, const FFACoord* coords,
                 const FFACoordFreq* coords_freq, uint64_t ncur, uint64_t nprev, uint64_t nseg,
-                uint32_t nbins, hipStream_t s) {
+                uint64_t n_f, uint32_t nbins, hipStream_t s) {
     LOKI_REQUIRE(nbins >= 1, "merge: nbins must be positive");
     LOKI_REQUIRE((coords == nullptr) != (coords_freq == nullptr),
                  "merge: exactly one coordinate table");
@@ -190,9 +194,9 @@ void merge_time(const float* in, float* out, const FFACoord* coords,
         return;
     }
     if (coords != nullptr) {
-        dispatch<true>(in, out, coords, nullptr, ncur, nprev, nseg, nbins, s);
+        dispatch<true>(in, out, coords, nullptr, ncur, nprev, nseg, n_f, nbins, s);
     } else {
-        dispatch<false>(in, out, nullptr, coords_freq, ncur, nprev, nseg, nbins, s);
+        dispatch<false>(in, out, nullptr, coords_freq, ncur, nprev, nseg, n_f, nbins, s);
     }
 }
 

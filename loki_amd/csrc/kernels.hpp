@@ -54,10 +54,11 @@ BruteFoldLists build_brute_fold_lists(const double* freqs_dev, uint32_t nfreqs, 
 void brute_fold_time(const float* ts_e, const float* ts_v, uint64_t nseg,
                      const BruteFoldLists& lists, float* fold, hipStream_t s);
 
-/// Time-domain merge of one level (freq-only when coords_freq != nullptr).
+/// Time-domain merge of one level (freq-only when coords_freq != nullptr);
+/// n_f: frequency grid points of the level (coordinates are rows of n_f).
 void merge_time(const float* in, float* out, const FFACoord* coords,
                 const FFACoordFreq* coords_freq, uint64_t ncur, uint64_t nprev, uint64_t nseg,
-                uint32_t nbins, hipStream_t s);
+                uint64_t n_f, uint32_t nbins, hipStream_t s);
 
 /// Boxcar widths with the reference's template constants (score.cpp:107-116).
 struct WidthTable {

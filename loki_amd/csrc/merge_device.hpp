@@ -6,6 +6,8 @@
 
 #include <cstdint>
 
+#include "common.hpp"
+
 namespace loki_amd::kern {
 
 /// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X; remap
@@ -19,6 +21,81 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t nblocks) {
     const uint32_t xcd = bid % kXcd;
     const uint32_t loc = bid / kXcd;
     return xcd < rem ? xcd * (per + 1) + loc : rem * (per + 1) + (xcd - rem) * per + loc;
+}
+
+/// Tile schedule of a merge level.  Coordinates are rows of n_f frequencies
+/// (one row per (jerk, accel) cell, f fastest).  Neighbouring child rows map
+/// to the same parent row (the accel grid is several times finer than its
+/// parent's), so blocks are ordered
+///   (segment, row group, frequency run, row within group, tile within run):
+/// the `group` child rows that share parents are processed back to back for
+/// one run of `run` frequency tiles, so their parent rows are served from L2
+/// instead of being re-read from HBM once per child row, while each row's
+/// output is still written in long sequential runs.
+struct TileSchedule {
+    uint64_t n_f;       ///< frequencies per row
+    uint64_t n_rows;    ///< rows per segment (ncur / n_f)
+    uint32_t n_ft;      ///< frequency tiles per row
+    uint32_t n_runs;    ///< ceil(n_ft / run)
+    uint32_t run;       ///< tiles per run
+    uint32_t n_groups;  ///< ceil(n_rows / group)
+    uint32_t group;     ///< rows per group
+    uint32_t nblocks;   ///< nseg * n_groups * n_runs * group * run
+};
+
+struct Tile {
+    uint64_t seg;
+    uint64_t c0;     ///< first coordinate of the tile
+    uint32_t count;  ///< coordinates in the tile (0: idle block)
+};
+
+inline TileSchedule make_schedule(uint64_t ncur, uint64_t n_f, uint64_t nseg, uint32_t tile,
+                                  uint32_t group, uint32_t run) {
+    LOKI_REQUIRE(n_f >= 1 && ncur % n_f == 0, "merge: ncur must be a multiple of n_freq");
+    TileSchedule ts{};
+    ts.n_f      = n_f;
+    ts.n_rows   = ncur / n_f;
+    const uint64_t n_ft = (n_f + tile - 1) / tile;
+    // balanced groups of at most `group` rows (idle rows would leave whole
+    // XCD ranges of the remapped grid without work)
+    const uint64_t ngrp0 = (ts.n_rows + group - 1) / group;
+    const uint64_t grp   = (ts.n_rows + ngrp0 - 1) / ngrp0;
+    const uint64_t rn   = grp == 1 ? 1 : (n_ft < run ? n_ft : run);
+    const uint64_t n_r  = (n_ft + rn - 1) / rn;
+    const uint64_t n_g  = (ts.n_rows + grp - 1) / grp;
+    const uint64_t nb   = nseg * n_g * n_r * grp * rn;
+    LOKI_REQUIRE(nb < (1ULL << 31), "merge grid too large");
+    ts.n_ft     = static_cast<uint32_t>(n_ft);
+    ts.n_runs   = static_cast<uint32_t>(n_r);
+    ts.run      = static_cast<uint32_t>(rn);
+    ts.n_groups = static_cast<uint32_t>(n_g);
+    ts.group    = static_cast<uint32_t>(grp);
+    ts.nblocks  = static_cast<uint32_t>(nb);
+    return ts;
+}
+
+__device__ __forceinline__ Tile tile_of(const TileSchedule& ts, uint32_t t, uint32_t tile) {
+    const uint32_t per_run   = ts.group * ts.run;
+    const uint32_t per_group = ts.n_runs * per_run;
+    const uint32_t per_seg   = ts.n_groups * per_group;
+    Tile r;
+    r.seg             = t / per_seg;
+    const uint32_t q  = t % per_seg;
+    const uint32_t g  = q / per_group;
+    const uint32_t w  = q % per_group;
+    const uint32_t rb = w / per_run;
+    const uint32_t w2 = w % per_run;
+    const uint64_t row = static_cast<uint64_t>(g) * ts.group + (w2 / ts.run);
+    const uint32_t ft  = rb * ts.run + (w2 % ts.run);
+    if (row >= ts.n_rows || ft >= ts.n_ft) {
+        r.c0    = 0;
+        r.count = 0;
+        return r;
+    }
+    const uint64_t f0 = static_cast<uint64_t>(ft) * tile;
+    r.c0    = row * ts.n_f + f0;
+    r.count = static_cast<uint32_t>(ts.n_f - f0 < tile ? ts.n_f - f0 : tile);
+    return r;
 }
 
 /// Elements r..r+3 of the 8-element concatenation (a, b).
