@@ -29,6 +29,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBPS = 8000.0
+MFMA_F32_PEAK_TFLOPS = 157.3
 WIDTHS = (1, 2, 3, 4, 6, 8, 12, 16, 24, 32)
 
 WORKLOADS = {
@@ -39,6 +40,18 @@ WORKLOADS = {
         # CPU sample: the same grid rules (f_max = 400 Hz, bseg_brute = 1024) over the top
         # 75 Hz of the band.
         cpu_sample=dict(param_limits=[(-50.0, 50.0), (325.0, 400.0)], bseg_brute=1024),
+    ),
+    # Fourier-domain FFA over (fddot, fdot, f) expressed as (jerk, accel, freq) at f_min = 50 Hz:
+    # accel = c * fdot / f_min, jerk = c * fddot / f_min for |fdot| <= 1e-8 Hz/s,
+    # |fddot| <= 1e-14 Hz/s^2; complex64 folds with nbins 64 (33 harmonics).
+    "fourier_ffa_fdot_fddot": dict(
+        nsamps=2**22, tsamp=64e-6, nbins=64, eta=1.0, fourier=True,
+        param_limits=[(-299792458.0 * 1e-14 / 50.0, 299792458.0 * 1e-14 / 50.0),
+                      (-299792458.0 * 1e-8 / 50.0, 299792458.0 * 1e-8 / 50.0), (50.0, 500.0)],
+        signal=dict(f0=251.3, accel=0.0, jerk=0.0, fwhm=0.05, snr=15.0), seed=5150,
+        cpu_sample=dict(param_limits=[(-299792458.0 * 1e-14 / 50.0, 299792458.0 * 1e-14 / 50.0),
+                                      (-299792458.0 * 1e-8 / 50.0, 299792458.0 * 1e-8 / 50.0),
+                                      (496.0, 500.0)], bseg_brute=2048),
     ),
     "ffa_freq_only": dict(
         nsamps=2**20, tsamp=64e-6, nbins=64, eta=1.0, param_limits=[(0.5, 2.0)],
@@ -79,18 +92,22 @@ def calibrate_copy(nbytes: int) -> int:
     return 0
 
 
-def alg_bytes(plan, nbins, nw, kind, level, nsamps):
-    """Algorithmic bytes of one launch (FP32 time domain; 16-byte poly / 8-byte freq coords)."""
+def alg_bytes(plan, nbins, nw, kind, level, nsamps, fourier=False):
+    """Algorithmic bytes of one launch: FP32 time folds (4 B x nbins per row) or complex64
+    Fourier folds (8 B x (nbins/2+1) per row); 16-byte poly / 8-byte freq coords."""
     npar = int(plan.n_params)
     rec = 16 if npar > 1 else 8
     nc = [int(x) for x in plan.ncoords]
     ns = [int(x) for x in plan.nsegments]
+    row = 2 * (nbins // 2 + 1) * 8 if fourier else 2 * nbins * 4
     if kind == "merge":
-        return ns[level] * nc[level] * 2 * nbins * 3 * 4 + nc[level] * rec
+        return ns[level] * nc[level] * row * 3 + nc[level] * rec
     if kind == "resolve":
         return nc[level] * rec
     if kind == "brute_fold":
-        return 2 * nsamps * 4 + ns[0] * nc[0] * 2 * nbins * 4
+        return 2 * nsamps * 4 + ns[0] * nc[0] * row
+    if kind == "transform":  # final Fourier fold -> time fold
+        return nc[-1] * row + nc[-1] * 2 * nbins * 4
     if kind == "score":
         return nc[-1] * 2 * nbins * 4 + nc[-1] * nw * 4
     if kind == "merge_score":  # fused last level: same algorithmic bytes as merge + score
@@ -99,11 +116,20 @@ def alg_bytes(plan, nbins, nw, kind, level, nsamps):
     raise ValueError(kind)
 
 
-def pipeline_bytes(plan, nbins, nw, nsamps):
-    tot = alg_bytes(plan, nbins, nw, "brute_fold", 0, nsamps)
+def dft_flops(plan, nbins, nsamps):
+    """Real multiply-adds x 2 of the Fourier brute fold as a matrix product: rows (segment,
+    component) x samples x (re, im) of harmonics 1..nbins/2, per level-0 frequency."""
+    nh = nbins // 2
+    return 2.0 * (2 * nsamps) * (2 * nh) * int(plan.param_counts[0][-1])
+
+
+def pipeline_bytes(plan, nbins, nw, nsamps, fourier=False):
+    tot = alg_bytes(plan, nbins, nw, "brute_fold", 0, nsamps, fourier)
+    if fourier:
+        tot += alg_bytes(plan, nbins, nw, "transform", 0, nsamps, fourier)
     for lvl in range(1, int(plan.n_levels)):
-        tot += alg_bytes(plan, nbins, nw, "merge", lvl, nsamps)
-    tot += alg_bytes(plan, nbins, nw, "score", 0, nsamps)
+        tot += alg_bytes(plan, nbins, nw, "merge", lvl, nsamps, fourier)
+    tot += alg_bytes(plan, nbins, nw, "score", 0, nsamps, fourier)
     return tot
 
 
@@ -125,18 +151,22 @@ def cpu_baseline(wl: dict, ts_e, ts_v, widths, threads: int) -> dict:
     from oracle.plan import Plan, SearchConfig
 
     smp = wl["cpu_sample"]
+    fourier = bool(wl.get("fourier", False))
     ocfg = SearchConfig(wl["nsamps"], wl["tsamp"], wl["nbins"], wl["eta"], smp["param_limits"],
-                        use_fourier=False, bseg_brute=smp["bseg_brute"])
-    nc = Plan(ocfg).ncoords[-1]
+                        use_fourier=fourier, bseg_brute=smp["bseg_brute"])
+    nc = Plan(ocfg, fourier=fourier).ncoords[-1]
     orc.lib()
     t0 = time.perf_counter()
     orc.compute_ffa_scores(ts_e, ts_v, ocfg, widths=widths, nthreads=threads)
     dt = time.perf_counter() - t0
     lims = ", ".join(f"[{a:g}, {b:g}]" for a, b in smp["param_limits"])
+    note = (" (FP64 DFT brute fold and FP64 phase rotations: slower than an FP32 CPU path)"
+            if fourier else "")
     return {
         "value": nc / dt, "unit": "param trials/s", "cores": threads, "kind": "port",
         "sample": f"same series and grid rules, param_limits {lims}, bseg_brute "
-                  f"{ocfg.bseg_brute}: {nc} final trials x {len(widths)} widths in {dt:.2f} s",
+                  f"{ocfg.bseg_brute}: {nc} final trials x {len(widths)} widths in {dt:.2f} s"
+                  + note,
     }
 
 
@@ -160,13 +190,17 @@ def main() -> int:
     ts_e, ts_v, _amp = make_series(wl["nsamps"], wl["tsamp"], wl["seed"], f0=sig["f0"],
                                    accel=sig["accel"], jerk=sig["jerk"], fwhm=sig["fwhm"],
                                    snr=sig["snr"], nbins=nbins, widths=widths)
+    fourier = bool(wl.get("fourier", False))
     cfg = libloki.configs.PulsarSearchConfig(
         nsamps=wl["nsamps"], tsamp=wl["tsamp"], nbins=nbins, eta=wl["eta"],
-        param_limits=np.array(wl["param_limits"]), use_fourier=False)
+        param_limits=np.array(wl["param_limits"]), use_fourier=fourier)
     dev = torch.device("cuda:0")
-    ffa = libculoki.ffa.FFACUDA(cfg, device_id=0)
-    ffa.set_exact_norm(not args.fast_norm)
-    ffa.set_fused(not args.unfused)
+    if fourier:
+        ffa = libculoki.ffa.FFACOMPLEXCUDA(cfg, device_id=0)
+    else:
+        ffa = libculoki.ffa.FFACUDA(cfg, device_id=0)
+        ffa.set_exact_norm(not args.fast_norm)
+        ffa.set_fused(not args.unfused)
     plan = ffa.plan
     nc = int(plan.ncoords[-1])
     nw = int(widths.size)
@@ -210,12 +244,24 @@ def main() -> int:
     # dominant kernel and its roofline
     avg = {k: float(np.mean(v)) for k, v in per_kernel.items()}
     dom = max(avg, key=avg.get)
-    dom_bytes = alg_bytes(plan, nbins, nw, dom[0], dom[1], wl["nsamps"])
-    achieved = dom_bytes / (avg[dom] * 1e-3) / 1e9
     dom_name = f"{dom[0]}_L{dom[1]}"
     traffic = load_traffic(args.workload, dom_name)
+    if fourier and dom[0] == "brute_fold":
+        # the DFT brute fold runs on the matrix cores: FP32 MFMA roofline
+        work = dft_flops(plan, nbins, wl["nsamps"])
+        roof = {"bound": "mfma", "kernel": dom_name,
+                "achieved": work / (avg[dom] * 1e-3) / 1e12, "peak": MFMA_F32_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "traffic": traffic, "flops_per_launch": work,
+                "avg_ms": avg[dom]}
+        roof["frac"] = roof["achieved"] / MFMA_F32_PEAK_TFLOPS
+    else:
+        dom_bytes = alg_bytes(plan, nbins, nw, dom[0], dom[1], wl["nsamps"], fourier)
+        achieved = dom_bytes / (avg[dom] * 1e-3) / 1e9
+        roof = {"bound": "hbm", "kernel": dom_name, "achieved": achieved,
+                "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
+                "traffic": traffic, "alg_bytes_per_launch": dom_bytes, "avg_ms": avg[dom]}
     kern_ms = {f"{k}_L{l}": round(v, 4) for (k, l), v in sorted(avg.items(), key=lambda x: -x[1])}
-    total_alg = pipeline_bytes(plan, nbins, nw, wl["nsamps"])
+    total_alg = pipeline_bytes(plan, nbins, nw, wl["nsamps"], fourier)
 
     result = {
         "metric": "FFA param trials/s + % HBM roofline",
@@ -228,21 +274,18 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "c64" if fourier else "f32",
         "data": "synthetic",
         "config": {
             "workload": args.workload, "nsamps": wl["nsamps"], "tsamp": wl["tsamp"],
             "nbins": nbins, "eta": wl["eta"], "param_limits": wl["param_limits"],
             "widths": [int(w) for w in widths], "fold_final": [int(x) for x in plan.fold_shapes[-1]],
             "n_levels": int(plan.n_levels), "trials_per_step": nc,
+            "domain": "fourier" if fourier else "time",
             "score_norm": "hw_rsqrt" if args.fast_norm else "exact",
-            "final_level": "unfused" if args.unfused else "fused",
+            "final_level": "unfused" if (args.unfused or fourier) else "fused",
         },
-        "roofline": {
-            "bound": "hbm", "kernel": dom_name, "achieved": achieved, "peak": HBM_PEAK_GBPS,
-            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
-            "alg_bytes_per_launch": dom_bytes, "avg_ms": avg[dom],
-        },
+        "roofline": roof,
         "pipeline_alg_GBps": total_alg / (ms_per_step * 1e-3) / 1e9,
         "kernel_ms": kern_ms,
         "found": found,

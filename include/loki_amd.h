@@ -164,6 +164,15 @@ int loki_ffa_execute_device(loki_ffa_t ffa, const float* ts_e, const float* ts_v
 /* FFACUDA::execute(host spans) (ffa.hpp:125-129): synchronous. */
 int loki_ffa_execute_host(loki_ffa_t ffa, const float* ts_e, const float* ts_v, size_t nsamps,
                           void* fold, size_t fold_len);
+/* FFACUDA<complex>::execute(..., span<float> fold, stream) (ffa.hpp:138-148,
+ * ffa_cuda.cu:248-290): Fourier FFA returned to the time domain.  fold holds
+ * 2*buffer_size floats; the result is its first fold_size_time floats
+ * [nseg, ncoords, 2, nbins].  Needs a Fourier (fold_type 1) object. */
+int loki_ffa_execute_time_device(loki_ffa_t ffa, const float* ts_e, const float* ts_v,
+                                 float* fold, size_t fold_len, void* stream);
+/* The same from host spans (ffa_cuda.cu:198-246): synchronous. */
+int loki_ffa_execute_time_host(loki_ffa_t ffa, const float* ts_e, const float* ts_v,
+                               size_t nsamps, float* fold, size_t fold_len);
 /* compute_ffa_scores_cuda (ffa_cuda.cu:670-692) on resident buffers, with
  * explicit widths (widths[i] in [1, nbins)).  scores: [ncoords_final, nw]
  * or [ncoords_final] if find_max. */
@@ -178,7 +187,8 @@ int loki_ffa_scores_host(loki_ffa_t ffa, const float* ts_e, const float* ts_v, s
  * launch stream around each kernel).  No reference counterpart: it replaces the reference's
  * host-side ScopeTimer / get_brute_fold_timing (ffa.hpp:52, ffa.cpp:74,204-239).
  * kind: 0 shift-table resolve, 1 brute fold, 2 merge, 3 score, 4 fused last level
- * (shift tables + merge + score); level: FFA level. */
+ * (shift tables + merge + score), 5 real FFT between time and Fourier folds; level: FFA
+ * level. */
 typedef struct {
     int kind;
     int level;
@@ -209,6 +219,26 @@ int loki_brute_fold_time_device(const float* ts_e, const float* ts_v, size_t nsa
                                 const double* freqs_host, size_t nfreqs, size_t segment_len,
                                 size_t nbins, double tsamp, double t_ref, float* fold,
                                 void* stream);
+
+/* compute_brute_fold_cuda<ComplexTypeCUDA> (fold_cuda.cu:260-277, kernels_cuda.cu:1012-1061):
+ * fold has (nsamps/segment_len) * nfreqs * 2 * (nbins/2+1) complex64 values
+ * (interleaved float pairs); nbins in [2, 256]. */
+int loki_brute_fold_fourier_host(const float* ts_e, const float* ts_v, size_t nsamps,
+                                 const double* freqs, size_t nfreqs, size_t segment_len,
+                                 size_t nbins, double tsamp, double t_ref, float* fold,
+                                 int device_id);
+int loki_brute_fold_fourier_device(const float* ts_e, const float* ts_v, size_t nsamps,
+                                   const double* freqs_host, size_t nfreqs, size_t segment_len,
+                                   size_t nbins, double tsamp, double t_ref, float* fold,
+                                   void* stream);
+
+/* --- Real transforms (fft.hpp:180-262 irfft_batch_cuda / rfft_batch_cuda) --- */
+/* fft_cuda.cu:357-448: batch rows of n_real/2+1 complex64 -> n_real floats, scaled 1/n_real. */
+int loki_irfft_batch_device(const float* complex_in, float* real_out, size_t batch_size,
+                            size_t n_real, void* stream);
+/* fft_cuda.cu:282-355: batch rows of n_real floats -> n_real/2+1 complex64 (unnormalised). */
+int loki_rfft_batch_device(const float* real_in, float* complex_out, size_t batch_size,
+                           size_t n_real, void* stream);
 
 /* --- Scores (score.hpp:123-165 snr_boxcar_*_cuda) ---------------------- */
 /* folds [nprof, 2, nbins] -> [nprof, nw] (find_max 0) or [nprof] (1). */

@@ -9,7 +9,7 @@ The package mirrors loki's Python surface for this path:
 
 Everything goes through the C-ABI in ``include/loki_amd.h`` (``libloki_amd.so``).  There is no
 CPU fallback: importing fails loudly when the native libraries have not been built
-(``python -m loki_amd.build``).
+(``python loki_amd/build.py``).
 """
 
 from __future__ import annotations
@@ -33,7 +33,7 @@ try:
 except ImportError as exc:  # pragma: no cover - exercised only on a broken install
     raise ImportError(
         "loki_amd native library is missing or failed to load "
-        f"({exc}); build it in-tree with `python -m loki_amd.build`"
+        f"({exc}); build it in-tree with `python loki_amd/build.py`"
     ) from exc
 
 from . import libculoki, libloki  # noqa: E402

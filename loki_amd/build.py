@@ -4,7 +4,8 @@ Outputs (git-ignored, shipped with the tree):
   loki_amd/libloki_amd.so                        -- C-ABI + gfx950 HIP kernels
   loki_amd/_loki_amd.cpython-*.so                -- pybind11 bindings over the C-ABI
 
-Usage:  python -m loki_amd.build   (or __graft_entry__.build())
+Usage:  python loki_amd/build.py   (or __graft_entry__.build()); run it as a script, since
+importing the loki_amd package needs the libraries this builds.
 """
 
 from __future__ import annotations
@@ -24,6 +25,7 @@ ARCH = os.environ.get("LOKI_AMD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_SOURCES = ["k_resolve.hip", "k_brutefold.hip", "k_merge.hip", "k_score.hip", "k_final.hip",
+               "k_fourier.hip",
                "ffa.hip", "capi.hip"]
 HOST_SOURCES = ["config.cpp", "plan.cpp"]
 

@@ -338,15 +338,41 @@ int loki_ffa_execute_host(loki_ffa_t ffa, const float* ts_e, const float* ts_v, 
         const auto& pl = ffa->exec.plan();
         REQ(nsamps == pl.config().get_nsamps(), "FFA::execute: ts_e must have size nsamps");
         REQ(fold_len == pl.buffer_size(), "FFA::execute: fold must have size buffer_size");
+        const size_t elem = pl.kind() == loki_amd::FoldKind::kFourier ? 8 : 4;
         loki_amd::DeviceGuard g(ffa->exec.device_id());
         loki_amd::DeviceBuffer de(nsamps * sizeof(float)), dv(nsamps * sizeof(float));
-        loki_amd::DeviceBuffer df(fold_len * sizeof(float));
+        loki_amd::DeviceBuffer df(fold_len * elem);
         loki_amd::hip_check(hipMemcpy(de.get(), ts_e, nsamps * 4, hipMemcpyHostToDevice), "H2D");
         loki_amd::hip_check(hipMemcpy(dv.get(), ts_v, nsamps * 4, hipMemcpyHostToDevice), "H2D");
         ffa->exec.execute(de.as<float>(), dv.as<float>(), df.get(), fold_len, nullptr);
-        loki_amd::hip_check(hipMemcpy(fold, df.get(), pl.fold_size() * sizeof(float),
-                                      hipMemcpyDeviceToHost),
+        loki_amd::hip_check(hipMemcpy(fold, df.get(), pl.fold_size() * elem, hipMemcpyDeviceToHost),
                             "D2H");
+    });
+}
+
+int loki_ffa_execute_time_device(loki_ffa_t ffa, const float* ts_e, const float* ts_v,
+                                 float* fold, size_t fold_len, void* stream) {
+    return guarded([&] {
+        REQ(ffa != nullptr, "null ffa");
+        ffa->exec.execute_time(ts_e, ts_v, fold, fold_len, as_stream(stream));
+    });
+}
+
+int loki_ffa_execute_time_host(loki_ffa_t ffa, const float* ts_e, const float* ts_v,
+                               size_t nsamps, float* fold, size_t fold_len) {
+    return guarded([&] {
+        REQ(ffa != nullptr && ts_e != nullptr && ts_v != nullptr && fold != nullptr,
+            "null argument");
+        const auto& pl = ffa->exec.plan();
+        REQ(nsamps == pl.config().get_nsamps(), "FFA::execute: ts_e must have size nsamps");
+        REQ(fold_len == 2 * pl.buffer_size(), "FFA::execute: fold must have size 2*buffer_size");
+        loki_amd::DeviceGuard g(ffa->exec.device_id());
+        loki_amd::DeviceBuffer de(nsamps * 4), dv(nsamps * 4), df(fold_len * 4);
+        loki_amd::hip_check(hipMemcpy(de.get(), ts_e, nsamps * 4, hipMemcpyHostToDevice), "H2D");
+        loki_amd::hip_check(hipMemcpy(dv.get(), ts_v, nsamps * 4, hipMemcpyHostToDevice), "H2D");
+        ffa->exec.execute_time(de.as<float>(), dv.as<float>(), df.as<float>(), fold_len, nullptr);
+        loki_amd::hip_check(
+            hipMemcpy(fold, df.get(), pl.fold_size_time() * 4, hipMemcpyDeviceToHost), "D2H");
     });
 }
 
@@ -448,6 +474,62 @@ int loki_brute_fold_time_device(const float* ts_e, const float* ts_v, size_t nsa
                                          std::span<const double>(freqs_host, nfreqs),
                                          segment_len, nbins, tsamp, t_ref, fold,
                                          as_stream(stream));
+    });
+}
+
+int loki_brute_fold_fourier_host(const float* ts_e, const float* ts_v, size_t nsamps,
+                                 const double* freqs, size_t nfreqs, size_t segment_len,
+                                 size_t nbins, double tsamp, double t_ref, float* fold,
+                                 int device_id) {
+    return guarded([&] {
+        REQ(ts_e != nullptr && ts_v != nullptr && freqs != nullptr && fold != nullptr,
+            "null argument");
+        REQ(segment_len > 0 && nsamps % segment_len == 0,
+            "BruteFold: Number of samples is not a multiple of segment length");
+        loki_amd::DeviceGuard g(device_id);
+        const size_t out_len = (nsamps / segment_len) * nfreqs * 2 * (nbins / 2 + 1);
+        loki_amd::DeviceBuffer de(nsamps * 4), dv(nsamps * 4), df(out_len * 8);
+        loki_amd::hip_check(hipMemcpy(de.get(), ts_e, nsamps * 4, hipMemcpyHostToDevice), "H2D");
+        loki_amd::hip_check(hipMemcpy(dv.get(), ts_v, nsamps * 4, hipMemcpyHostToDevice), "H2D");
+        loki_amd::brute_fold_fourier_device(de.as<float>(), dv.as<float>(), nsamps,
+                                            std::span<const double>(freqs, nfreqs), segment_len,
+                                            nbins, tsamp, t_ref, df.as<float2>(), nullptr);
+        loki_amd::hip_check(hipMemcpy(fold, df.get(), out_len * 8, hipMemcpyDeviceToHost), "D2H");
+    });
+}
+
+int loki_brute_fold_fourier_device(const float* ts_e, const float* ts_v, size_t nsamps,
+                                   const double* freqs_host, size_t nfreqs, size_t segment_len,
+                                   size_t nbins, double tsamp, double t_ref, float* fold,
+                                   void* stream) {
+    return guarded([&] {
+        REQ(freqs_host != nullptr && fold != nullptr, "null argument");
+        loki_amd::brute_fold_fourier_device(ts_e, ts_v, nsamps,
+                                            std::span<const double>(freqs_host, nfreqs),
+                                            segment_len, nbins, tsamp, t_ref,
+                                            reinterpret_cast<float2*>(fold), as_stream(stream));
+    });
+}
+
+int loki_irfft_batch_device(const float* complex_in, float* real_out, size_t batch_size,
+                            size_t n_real, void* stream) {
+    return guarded([&] {
+        REQ(complex_in != nullptr && real_out != nullptr, "null argument");
+        REQ(n_real >= 1 && n_real <= 4096, "IRFFT: n_real must be in [1, 4096]");
+        loki_amd::kern::irfft_rows(reinterpret_cast<const float2*>(complex_in), real_out,
+                                   batch_size, static_cast<uint32_t>(n_real), as_stream(stream));
+        loki_amd::hip_check(hipGetLastError(), "irfft launch");
+    });
+}
+
+int loki_rfft_batch_device(const float* real_in, float* complex_out, size_t batch_size,
+                           size_t n_real, void* stream) {
+    return guarded([&] {
+        REQ(real_in != nullptr && complex_out != nullptr, "null argument");
+        REQ(n_real >= 1 && n_real <= 4096, "RFFT: n_real must be in [1, 4096]");
+        loki_amd::kern::rfft_rows(real_in, reinterpret_cast<float2*>(complex_out), batch_size,
+                                  static_cast<uint32_t>(n_real), as_stream(stream));
+        loki_amd::hip_check(hipGetLastError(), "rfft launch");
     });
 }
 

@@ -159,11 +159,16 @@ void check_launch(const char* what) { hip_check(hipGetLastError(), what); }
 }  // namespace
 
 FFAExecutor::FFAExecutor(const SearchConfig& cfg, FoldKind kind, int device_id)
-    : m_plan(cfg, kind), m_device(device_id), m_freq_only(cfg.get_nparams() == 1) {
+    : m_plan(cfg, kind),
+      m_device(device_id),
+      m_freq_only(cfg.get_nparams() == 1),
+      m_fourier(kind == FoldKind::kFourier),
+      m_lossy(kind == FoldKind::kFourier && cfg.get_nbins() > cfg.get_nbins_min_lossy_bf()) {
     DeviceGuard g(device_id);
-    LOKI_REQUIRE(kind == FoldKind::kTime,
-                 "the Fourier-domain FFA is not available in this build");
     LOKI_REQUIRE(cfg.get_nbins() <= 1024, "nbins must be <= 1024");
+    LOKI_REQUIRE(!m_fourier || m_lossy || (cfg.get_nbins() >= 2 && cfg.get_nbins() <= 256),
+                 "Fourier brute fold needs 2 <= nbins <= 256 (larger nbins use the lossy "
+                 "time-domain init: nbins > nbins_min_lossy_bf)");
     for (SizeType l = 1; l < m_plan.n_levels(); ++l) {
         LOKI_REQUIRE(m_plan.ncoords()[l - 1] < (1ULL << 32),
                      "parent grid exceeds 2^32 coordinates (u32 indices)");
@@ -206,15 +211,53 @@ void FFAExecutor::prepare_brute_fold(hipStream_t s) {
     hip_check(hipMemcpyAsync(m_bf_freqs.get(), freqs.data(), nfreqs * sizeof(double),
                              hipMemcpyHostToDevice, s),
               "copy brute-fold frequencies");
-    // synchronises s (so `freqs` outlives the copy)
-    m_bf = kern::build_brute_fold_lists(m_bf_freqs.as<double>(), static_cast<uint32_t>(nfreqs),
-                                        static_cast<uint32_t>(seglen),
-                                        static_cast<uint32_t>(nbins),
-                                        m_plan.config().get_tsamp(), t_ref, s);
+    if (!m_fourier || m_lossy) {
+        // synchronises s (so `freqs` outlives the copy)
+        m_bf = kern::build_brute_fold_lists(
+            m_bf_freqs.as<double>(), static_cast<uint32_t>(nfreqs), static_cast<uint32_t>(seglen),
+            static_cast<uint32_t>(nbins), m_plan.config().get_tsamp(), t_ref, s);
+    } else {
+        hip_check(hipStreamSynchronize(s), "copy brute-fold frequencies");
+    }
     m_buckets_ready = true;
 }
 
-float* FFAExecutor::run_levels(const float* ts_e, const float* ts_v, void* buf_even,
+void FFAExecutor::init_level(const float* ts_e, const float* ts_v, void* out, void* scratch,
+                             hipStream_t s) {
+    const SizeType nbins = m_plan.config().get_nbins();
+    const SizeType nseg  = m_plan.nsegments()[0];
+    const int bslot      = m_timer.start(KernelKind::kBruteFold, 0, s);
+    if (!m_fourier) {
+        kern::brute_fold_time(ts_e, ts_v, nseg, m_bf, static_cast<float*>(out), s);
+        check_launch("brute fold launch");
+        m_timer.stop(bslot, s);
+        return;
+    }
+    if (!m_lossy) {
+        const double t_ref = m_freq_only ? 0.0 : m_plan.tsegments()[0] / 2.0;
+        kern::brute_fold_fourier(ts_e, ts_v, nseg, static_cast<uint32_t>(m_plan.segment_lens()[0]),
+                                 m_bf_freqs.as<double>(),
+                                 static_cast<uint32_t>(m_plan.param_counts()[0].back()),
+                                 static_cast<uint32_t>(nbins), m_plan.config().get_tsamp(),
+                                 t_ref, static_cast<float2*>(out), s);
+        check_launch("Fourier brute fold launch");
+        m_timer.stop(bslot, s);
+        return;
+    }
+    // Lossy init (ffa_cuda.cu:370-398): time brute fold into the partner
+    // buffer, then a forward real DFT of every (segment, freq, component) row.
+    auto* tf = static_cast<float*>(scratch);
+    kern::brute_fold_time(ts_e, ts_v, nseg, m_bf, tf, s);
+    check_launch("brute fold launch");
+    m_timer.stop(bslot, s);
+    const SizeType nrows = nseg * m_bf.nfreqs * 2;
+    const int tslot      = m_timer.start(KernelKind::kTransform, 0, s);
+    kern::rfft_rows(tf, static_cast<float2*>(out), nrows, static_cast<uint32_t>(nbins), s);
+    check_launch("rfft launch");
+    m_timer.stop(tslot, s);
+}
+
+void* FFAExecutor::run_levels(const float* ts_e, const float* ts_v, void* buf_even,
                               void* buf_odd, SizeType last_level, hipStream_t s) {
     // Level k lands in buf_even for even k and in buf_odd for odd k (the
     // reference picks its ping-pong start side the same way so the last level
@@ -225,13 +268,10 @@ float* FFAExecutor::run_levels(const float* ts_e, const float* ts_v, void* buf_e
     prepare_brute_fold(s);
     resolve_coordinates(m_coords.get(), s, last_level);
 
-    auto* even = static_cast<float*>(buf_even);
-    auto* odd  = static_cast<float*>(buf_odd);
+    void* even           = buf_even;
+    void* odd            = buf_odd;
     const SizeType nbins = m_plan.config().get_nbins();
-    const int bslot      = m_timer.start(KernelKind::kBruteFold, 0, s);
-    kern::brute_fold_time(ts_e, ts_v, m_plan.nsegments()[0], m_bf, even, s);
-    check_launch("brute fold launch");
-    m_timer.stop(bslot, s);
+    init_level(ts_e, ts_v, even, odd, s);
 
     for (SizeType l = 1; l <= last_level; ++l) {
         const SizeType off = m_plan.ncoords_offsets64()[l];
@@ -242,12 +282,18 @@ float* FFAExecutor::run_levels(const float* ts_e, const float* ts_v, void* buf_e
         } else {
             cp = m_coords.as<FFACoord>() + off;
         }
-        float* in  = (l % 2 == 1) ? even : odd;
-        float* out = (l % 2 == 1) ? odd : even;
+        void* in       = (l % 2 == 1) ? even : odd;
+        void* out      = (l % 2 == 1) ? odd : even;
         const int slot = m_timer.start(KernelKind::kMerge, static_cast<int>(l), s);
-        kern::merge_time(in, out, cp, cfp, m_plan.ncoords()[l], m_plan.ncoords()[l - 1],
-                         m_plan.nsegments()[l], m_plan.param_counts()[l].back(),
-                         static_cast<uint32_t>(nbins), s);
+        if (m_fourier) {
+            kern::merge_fourier(static_cast<const float2*>(in), static_cast<float2*>(out), cp, cfp,
+                                m_plan.ncoords()[l], m_plan.ncoords()[l - 1],
+                                m_plan.nsegments()[l], static_cast<uint32_t>(nbins), s);
+        } else {
+            kern::merge_time(static_cast<const float*>(in), static_cast<float*>(out), cp, cfp,
+                             m_plan.ncoords()[l], m_plan.ncoords()[l - 1], m_plan.nsegments()[l],
+                             m_plan.param_counts()[l].back(), static_cast<uint32_t>(nbins), s);
+        }
         check_launch("merge launch");
         m_timer.stop(slot, s);
     }
@@ -270,7 +316,7 @@ void FFAExecutor::execute(const float* ts_e, const float* ts_v, void* fold, Size
     LOKI_REQUIRE(fold_len == m_plan.buffer_size(),
                  "FFA::execute: fold must have size buffer_size");
     const SizeType L = m_plan.n_levels();
-    m_internal.reserve(m_plan.buffer_size() * sizeof(float));
+    m_internal.reserve(m_plan.buffer_size() * elem_bytes());
     if ((L - 1) % 2 == 0) {
         run_levels(ts_e, ts_v, fold, m_internal.get(), L - 1, s);
     } else {
@@ -278,8 +324,33 @@ void FFAExecutor::execute(const float* ts_e, const float* ts_v, void* fold, Size
     }
 }
 
+void FFAExecutor::execute_time(const float* ts_e, const float* ts_v, float* fold,
+                               SizeType fold_len, hipStream_t s) {
+    DeviceGuard g(m_device);
+    LOKI_REQUIRE(m_fourier, "FFA::execute: the time-domain output overload needs a Fourier plan");
+    LOKI_REQUIRE(ts_e != nullptr && ts_v != nullptr && fold != nullptr,
+                 "FFA::execute: null buffer");
+    LOKI_REQUIRE(fold_len == 2 * m_plan.buffer_size(),
+                 "FFA::execute: fold must have size 2*buffer_size_fourier");
+    const SizeType L = m_plan.n_levels();
+    // The caller's buffer (buffer_size complex elements) is the ping-pong
+    // partner; the last level lands in the internal buffer and is returned to
+    // the time domain into the caller's buffer (ffa_cuda.cu:248-290).
+    m_internal.reserve(m_plan.buffer_size() * elem_bytes());
+    const void* fin = ((L - 1) % 2 == 0)
+                          ? run_levels(ts_e, ts_v, m_internal.get(), fold, L - 1, s)
+                          : run_levels(ts_e, ts_v, fold, m_internal.get(), L - 1, s);
+    const SizeType nbins = m_plan.config().get_nbins();
+    const int slot       = m_timer.start(KernelKind::kTransform, static_cast<int>(L - 1), s);
+    kern::irfft_rows(static_cast<const float2*>(fin), fold, m_plan.fold_size_time() / nbins,
+                     static_cast<uint32_t>(nbins), s);
+    check_launch("irfft launch");
+    m_timer.stop(slot, s);
+}
+
 bool FFAExecutor::fused_scores_supported() const {
-    return kern::final_merge_score_supported(
+    return !m_fourier &&
+           kern::final_merge_score_supported(
                static_cast<uint32_t>(m_plan.config().get_nbins())) &&
            m_plan.nsegments().back() == 1;
 }
@@ -304,8 +375,8 @@ void FFAExecutor::execute_scores(const float* ts_e, const float* ts_v,
         const SizeType need = max_level_size(L - 2) * sizeof(float);
         m_internal.reserve(need);
         m_result.reserve(need);
-        const float* parent =
-            run_levels(ts_e, ts_v, m_result.get(), m_internal.get(), L - 2, s);
+        const auto* parent = static_cast<const float*>(
+            run_levels(ts_e, ts_v, m_result.get(), m_internal.get(), L - 2, s));
         const auto lv  = make_level(m_plan, L - 1);
         const int slot = m_timer.start(KernelKind::kMergeScore, static_cast<int>(L - 1), s);
         kern::final_merge_score(parent, ncoords, m_plan.ncoords()[L - 2], lv, wt, scores,
@@ -314,10 +385,23 @@ void FFAExecutor::execute_scores(const float* ts_e, const float* ts_v,
         m_timer.stop(slot, s);
         return;
     }
-    m_internal.reserve(m_plan.buffer_size() * sizeof(float));
-    m_result.reserve(m_plan.buffer_size() * sizeof(float));
-    const float* fin = run_levels(ts_e, ts_v, m_result.get(), m_internal.get(), L - 1, s);
-    const int slot   = m_timer.start(KernelKind::kScore, static_cast<int>(L - 1), s);
+    m_internal.reserve(m_plan.buffer_size() * elem_bytes());
+    m_result.reserve(m_plan.buffer_size() * elem_bytes());
+    const void* last = run_levels(ts_e, ts_v, m_result.get(), m_internal.get(), L - 1, s);
+    const float* fin = static_cast<const float*>(last);
+    if (m_fourier) {
+        // Back to the time domain into the other ping-pong buffer: it holds
+        // buffer_size complex = 2*buffer_size floats >= fold_size_time.
+        void* other     = last == m_result.get() ? m_internal.get() : m_result.get();
+        auto* tf        = static_cast<float*>(other);
+        const int tslot = m_timer.start(KernelKind::kTransform, static_cast<int>(L - 1), s);
+        kern::irfft_rows(static_cast<const float2*>(last), tf, m_plan.fold_size_time() / nbins,
+                         static_cast<uint32_t>(nbins), s);
+        check_launch("irfft launch");
+        m_timer.stop(tslot, s);
+        fin = tf;
+    }
+    const int slot = m_timer.start(KernelKind::kScore, static_cast<int>(L - 1), s);
     kern::snr_boxcar_3d(fin, ncoords, static_cast<uint32_t>(nbins), wt, scores, find_max, s);
     check_launch("score launch");
     m_timer.stop(slot, s);
@@ -342,6 +426,29 @@ void brute_fold_time_device(const float* ts_e, const float* ts_v, SizeType nsamp
     check_launch("brute fold launch");
     // The lists are freed on return: finish before releasing them.
     hip_check(hipStreamSynchronize(s), "brute fold");
+}
+
+void brute_fold_fourier_device(const float* ts_e, const float* ts_v, SizeType nsamps,
+                               std::span<const double> freqs, SizeType segment_len,
+                               SizeType nbins, double tsamp, double t_ref, float2* fold,
+                               hipStream_t s) {
+    LOKI_REQUIRE(!freqs.empty(), "BruteFold: Frequency array is empty");
+    LOKI_REQUIRE(segment_len > 0 && nsamps % segment_len == 0,
+                 "BruteFold: Number of samples is not a multiple of segment length");
+    LOKI_REQUIRE(nbins >= 2 && nbins <= 256, "BruteFold: Fourier fold needs nbins in [2, 256]");
+    LOKI_REQUIRE(segment_len < (1ULL << 32) && freqs.size() < (1ULL << 32),
+                 "BruteFold: segment too long");
+    DeviceBuffer d_freqs(freqs.size() * sizeof(double));
+    hip_check(hipMemcpyAsync(d_freqs.get(), freqs.data(), freqs.size() * sizeof(double),
+                             hipMemcpyHostToDevice, s),
+              "copy frequencies");
+    kern::brute_fold_fourier(ts_e, ts_v, nsamps / segment_len,
+                             static_cast<uint32_t>(segment_len), d_freqs.as<double>(),
+                             static_cast<uint32_t>(freqs.size()), static_cast<uint32_t>(nbins),
+                             tsamp, t_ref, fold, s);
+    check_launch("Fourier brute fold launch");
+    // The frequencies are freed on return: finish before releasing them.
+    hip_check(hipStreamSynchronize(s), "Fourier brute fold");
 }
 
 }  // namespace loki_amd

@@ -22,6 +22,7 @@ enum class KernelKind : int {
     kMerge      = 2,
     kScore      = 3,
     kMergeScore = 4,  ///< fused last level (shift tables + merge + S/N)
+    kTransform  = 5,  ///< real FFT between time and Fourier folds
 };
 
 struct KernelTiming {
@@ -77,8 +78,15 @@ public:
     void execute(const float* ts_e, const float* ts_v, void* fold, SizeType fold_len,
                  hipStream_t s);
 
-    /// Time-domain FFA + boxcar S/N of the final level (nsegments must be 1):
-    /// scores [ncoords_final, nw] (or [ncoords_final] when find_max).
+    /// Fourier FFA returned to the time domain (ffa_cuda.cu:248-290): fold
+    /// has 2 * buffer_size() floats; the result is its first
+    /// fold_size_time() floats, [nseg, ncoords, 2, nbins].
+    void execute_time(const float* ts_e, const float* ts_v, float* fold, SizeType fold_len,
+                      hipStream_t s);
+
+    /// FFA + boxcar S/N of the final level (nsegments must be 1): scores
+    /// [ncoords_final, nw] (or [ncoords_final] when find_max).  A Fourier plan
+    /// returns its final level to the time domain first (ffa_cuda.cu:670-692).
     void execute_scores(const float* ts_e, const float* ts_v, std::span<const SizeType> widths,
                         float* scores, SizeType scores_len, bool find_max, hipStream_t s);
 
@@ -102,6 +110,8 @@ private:
     FFAPlan m_plan;
     int m_device;
     bool m_freq_only;
+    bool m_fourier;            // complex folds (float2 elements)
+    bool m_lossy;              // Fourier init = time brute fold + rfft (ffa_cuda.cu:342-353)
     DeviceBuffer m_internal;   // ping-pong partner (buffer_size elements)
     DeviceBuffer m_result;     // second buffer for execute_scores
     DeviceBuffer m_coords;     // all levels' shift tables
@@ -115,9 +125,12 @@ private:
     void prepare_brute_fold(hipStream_t s);
     /// Brute fold + merge levels 1..last_level; returns the buffer holding
     /// last_level (buf_even for even levels, buf_odd for odd levels).
-    float* run_levels(const float* ts_e, const float* ts_v, void* buf_even, void* buf_odd,
-                      SizeType last_level, hipStream_t s);
+    void* run_levels(const float* ts_e, const float* ts_v, void* buf_even, void* buf_odd,
+                     SizeType last_level, hipStream_t s);
+    void init_level(const float* ts_e, const float* ts_v, void* out, void* scratch,
+                    hipStream_t s);
     SizeType max_level_size(SizeType last_level) const;
+    SizeType elem_bytes() const noexcept { return m_fourier ? 2 * sizeof(float) : sizeof(float); }
 };
 
 /// Standalone brute fold (fold_cuda.cu:139-164, 260-277): fold has
@@ -125,5 +138,11 @@ private:
 void brute_fold_time_device(const float* ts_e, const float* ts_v, SizeType nsamps,
                             std::span<const double> freqs, SizeType segment_len, SizeType nbins,
                             double tsamp, double t_ref, float* fold, hipStream_t s);
+/// Standalone Fourier brute fold: fold has nsamps/segment_len * nfreqs * 2 *
+/// (nbins/2+1) float2 elements.  Synchronises `s`.
+void brute_fold_fourier_device(const float* ts_e, const float* ts_v, SizeType nsamps,
+                               std::span<const double> freqs, SizeType segment_len,
+                               SizeType nbins, double tsamp, double t_ref, float2* fold,
+                               hipStream_t s);
 
 }  // namespace loki_amd

@@ -60,6 +60,25 @@ void merge_time(const float* in, float* out, const FFACoord* coords,
                 const FFACoordFreq* coords_freq, uint64_t ncur, uint64_t nprev, uint64_t nseg,
                 uint64_t n_f, uint32_t nbins, hipStream_t s);
 
+/// Fourier brute fold: fold [nseg, nfreqs, 2, nbins/2+1] complex (float2) of
+/// exp(-2 pi i m f (k tsamp - t_ref)) sums; FP32 matrix-core DFT with FP64
+/// phases, harmonic 0 by FP64 reduction.  nbins in [2, 256].
+void brute_fold_fourier(const float* ts_e, const float* ts_v, uint64_t nseg, uint32_t seglen,
+                        const double* freqs_dev, uint32_t nfreqs, uint32_t nbins, double tsamp,
+                        double t_ref, float2* fold, hipStream_t s);
+
+/// Fourier-domain merge of one level (freq-only when coords_freq != nullptr):
+/// phase rotation by the fractional shifts, rows of [2, nbins/2+1] float2.
+void merge_fourier(const float2* in, float2* out, const FFACoord* coords,
+                   const FFACoordFreq* coords_freq, uint64_t ncur, uint64_t nprev, uint64_t nseg,
+                   uint32_t nbins, hipStream_t s);
+
+/// Inverse real DFT of nrows rows of nbins/2+1 float2 into rows of nbins
+/// floats, scaled by 1/nbins (C2R + normalisation).
+void irfft_rows(const float2* in, float* out, uint64_t nrows, uint32_t nbins, hipStream_t s);
+/// Forward real DFT (unnormalised) of nrows rows of nbins floats.
+void rfft_rows(const float* in, float2* out, uint64_t nrows, uint32_t nbins, hipStream_t s);
+
 /// Boxcar widths with the reference's template constants (score.cpp:107-116).
 struct WidthTable {
     static constexpr uint32_t kMax = 64;

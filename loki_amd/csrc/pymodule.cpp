@@ -2,8 +2,10 @@
 // argument names/defaults and return types of loki's pybind modules:
 //   libloki:   configs.PulsarSearchConfig, plans.FFAPlan{Base,Time,Fourier},
 //              scores.generate_box_width_trials   (src/loki_python.cpp:71-520)
-//   libculoki: ffa.FFACUDA / compute_ffa_time_cuda / compute_ffa_scores_cuda,
-//              fold.compute_brute_fold_time_cuda, scores.snr_boxcar_*
+//   libculoki: ffa.FFACUDA / FFACOMPLEXCUDA / compute_ffa_time_cuda /
+//              compute_ffa_fourier_cuda / compute_ffa_fourier_return_to_time_cuda /
+//              compute_ffa_scores_cuda, fold.compute_brute_fold_{time,fourier}_cuda,
+//              scores.snr_boxcar_*
 //              (src/loki_python_cuda.cpp:18-203, loki_templates_cuda.cuh:21-51)
 // C-ABI failures raise RuntimeError with the library's message, the same
 // Python exception type pybind11 gives for the reference's C++ exceptions.
@@ -11,6 +13,7 @@
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
+#include <complex>
 #include <cstdint>
 #include <map>
 #include <memory>
@@ -234,20 +237,46 @@ struct PlanFourier : Plan {
 class FFA {
 public:
     FFA(const Config& cfg, int device_id, int fold_type)
-        : m_cfg(cfg), m_device(device_id), m_plan(cfg, fold_type) {
+        : m_cfg(cfg), m_device(device_id), m_fold_type(fold_type), m_plan(cfg, fold_type) {
         loki_ffa_t h = nullptr;
         check(loki_ffa_create(cfg.handle(), fold_type, device_id, &h));
         m_h.reset(h, loki_ffa_destroy);
     }
     const Plan& plan() const { return m_plan; }
+    /// Time plan: float fold of buffer_size.  Fourier plan: float fold of
+    /// 2*buffer_size, the final level returned to the time domain.
     void execute(const PyArrayT<float>& ts_e, const PyArrayT<float>& ts_v,
                  PyArrayT<float>& fold) {
+        if (ts_v.size() != ts_e.size()) {
+            throw std::runtime_error("FFA::execute: ts_v must have size nsamps");
+        }
+        if (m_fold_type == LOKI_FOLD_FOURIER) {
+            check(loki_ffa_execute_time_host(m_h.get(), ts_e.data(), ts_v.data(),
+                                             static_cast<size_t>(ts_e.size()),
+                                             fold.mutable_data(),
+                                             static_cast<size_t>(fold.size())));
+            return;
+        }
+        check(loki_ffa_execute_host(m_h.get(), ts_e.data(), ts_v.data(),
+                                    static_cast<size_t>(ts_e.size()), fold.mutable_data(),
+                                    static_cast<size_t>(fold.size())));
+    }
+    /// Fourier plan: complex64 fold of buffer_size.
+    void execute_complex(const PyArrayT<float>& ts_e, const PyArrayT<float>& ts_v,
+                         PyArrayT<std::complex<float>>& fold) {
         if (ts_v.size() != ts_e.size()) {
             throw std::runtime_error("FFA::execute: ts_v must have size nsamps");
         }
         check(loki_ffa_execute_host(m_h.get(), ts_e.data(), ts_v.data(),
                                     static_cast<size_t>(ts_e.size()), fold.mutable_data(),
                                     static_cast<size_t>(fold.size())));
+    }
+    void execute_time_device(uintptr_t ts_e, uintptr_t ts_v, uintptr_t fold, size_t fold_len,
+                             uintptr_t stream) {
+        check(loki_ffa_execute_time_device(m_h.get(), reinterpret_cast<const float*>(ts_e),
+                                           reinterpret_cast<const float*>(ts_v),
+                                           reinterpret_cast<float*>(fold), fold_len,
+                                           reinterpret_cast<void*>(stream)));
     }
     void execute_device(uintptr_t ts_e, uintptr_t ts_v, uintptr_t fold, size_t fold_len,
                         uintptr_t stream) {
@@ -272,16 +301,18 @@ public:
         check(loki_ffa_timings(m_h.get(), nullptr, 0, &n));
         std::vector<loki_kernel_timing> t(n);
         check(loki_ffa_timings(m_h.get(), t.data(), n, &n));
-        static const char* kinds[] = {"resolve", "brute_fold", "merge", "score", "merge_score"};
+        static const char* kinds[] = {"resolve", "brute_fold", "merge",
+                                      "score",   "merge_score", "transform"};
         py::list out;
         for (const auto& x : t) {
-            out.append(py::make_tuple(kinds[x.kind >= 0 && x.kind <= 4 ? x.kind : 0], x.level, x.ms));
+            out.append(
+                py::make_tuple(kinds[x.kind >= 0 && x.kind <= 5 ? x.kind : 0], x.level, x.ms));
         }
         return out;
     }
     py::array_t<float> scores_host(const PyArrayT<float>& ts_e, const PyArrayT<float>& ts_v,
                                    const PyArrayT<size_t>& widths, bool find_max) {
-        const size_t nc = m_plan.info().fold_size / (2 * m_cfg.params().nbins);
+        const size_t nc = m_plan.info().fold_size_time / (2 * m_cfg.params().nbins);
         const size_t n  = find_max ? nc : nc * static_cast<size_t>(widths.size());
         py::array_t<float> out(static_cast<py::ssize_t>(n));
         check(loki_ffa_scores_host(m_h.get(), ts_e.data(), ts_v.data(),
@@ -291,11 +322,20 @@ public:
         return out;
     }
 
+    int fold_type() const { return m_fold_type; }
+
 private:
     Config m_cfg;
     int m_device;
+    int m_fold_type;
     Plan m_plan;
     std::shared_ptr<loki_ffa_s> m_h;
+};
+
+/// Distinct C++ type for the complex-fold class (pybind11 binds one Python
+/// class per C++ type).
+struct FFAComplex : FFA {
+    FFAComplex(const Config& cfg, int device_id) : FFA(cfg, device_id, LOKI_FOLD_FOURIER) {}
 };
 
 void check_nsamps(const PyArrayT<float>& ts_e, const PyArrayT<float>& ts_v, const Config& cfg) {
@@ -540,6 +580,31 @@ PYBIND11_MODULE(_loki_amd, m) {
         py::arg("ts_e"), py::arg("ts_v"), py::arg("freq_arr"), py::arg("segment_len"),
         py::arg("nbins"), py::arg("tsamp"), py::arg("t_ref") = 0.0F, py::arg("device_id") = 0);
 
+    m_fold.def(
+        "compute_brute_fold_fourier_cuda",
+        [](const PyArrayT<float>& ts_e, const PyArrayT<float>& ts_v,
+           const PyArrayT<double>& freq_arr, size_t segment_len, size_t nbins, double tsamp,
+           double t_ref, int device_id) {
+            if (ts_v.size() != ts_e.size()) {
+                throw std::runtime_error("BruteFold::Impl::execute: ts_v must have size nsamps");
+            }
+            const auto nsamps = static_cast<size_t>(ts_e.size());
+            if (segment_len == 0 || nsamps % segment_len != 0) {
+                throw std::runtime_error(
+                    "BruteFold::Impl: Number of samples is not a multiple of segment length");
+            }
+            const size_t n = (nsamps / segment_len) * static_cast<size_t>(freq_arr.size()) * 2 *
+                             (nbins / 2 + 1);
+            py::array_t<std::complex<float>> out(static_cast<py::ssize_t>(n));
+            check(loki_brute_fold_fourier_host(
+                ts_e.data(), ts_v.data(), nsamps, freq_arr.data(),
+                static_cast<size_t>(freq_arr.size()), segment_len, nbins, tsamp, t_ref,
+                reinterpret_cast<float*>(out.mutable_data()), device_id));
+            return out;
+        },
+        py::arg("ts_e"), py::arg("ts_v"), py::arg("freq_arr"), py::arg("segment_len"),
+        py::arg("nbins"), py::arg("tsamp"), py::arg("t_ref") = 0.0F, py::arg("device_id") = 0);
+
     // -------------------------------------------------------------------- ffa
     auto m_ffa = m.def_submodule("ffa", "FFA submodule");
     py::class_<FFA>(m_ffa, "FFACUDA")
@@ -560,6 +625,23 @@ PYBIND11_MODULE(_loki_amd, m) {
         .def("set_fused", &FFA::set_fused, py::arg("enable"))
         .def("set_exact_norm", &FFA::set_exact_norm, py::arg("exact"))
         .def("timings", &FFA::timings);
+    py::class_<FFAComplex>(m_ffa, "FFACOMPLEXCUDA")
+        .def(py::init<const Config&, int>(), py::arg("cfg"), py::arg("device_id") = 0)
+        .def_property_readonly("plan", &FFA::plan)
+        .def("execute", &FFA::execute_complex, py::arg("ts_e"), py::arg("ts_v"), py::arg("fold"))
+        .def("execute", &FFA::execute, py::arg("ts_e"), py::arg("ts_v"), py::arg("fold"))
+        .def("execute_device", &FFA::execute_device, py::arg("ts_e_ptr"), py::arg("ts_v_ptr"),
+             py::arg("fold_ptr"), py::arg("fold_len"), py::arg("stream") = 0)
+        .def("execute_time_device", &FFA::execute_time_device, py::arg("ts_e_ptr"),
+             py::arg("ts_v_ptr"), py::arg("fold_ptr"), py::arg("fold_len"),
+             py::arg("stream") = 0)
+        .def("scores_device", &FFA::scores_device, py::arg("ts_e_ptr"), py::arg("ts_v_ptr"),
+             py::arg("widths"), py::arg("scores_ptr"), py::arg("scores_len"),
+             py::arg("find_max") = false, py::arg("stream") = 0)
+        .def("scores", &FFA::scores_host, py::arg("ts_e"), py::arg("ts_v"), py::arg("widths"),
+             py::arg("find_max") = false)
+        .def("set_timing", &FFA::set_timing, py::arg("enable"))
+        .def("timings", &FFA::timings);
 
     m_ffa.def(
         "compute_ffa_time_cuda",
@@ -576,16 +658,41 @@ PYBIND11_MODULE(_loki_amd, m) {
         py::arg("ts_e"), py::arg("ts_v"), py::arg("cfg"), py::arg("device_id") = 0,
         py::arg("quiet") = false);
     m_ffa.def(
+        "compute_ffa_fourier_cuda",
+        [](const PyArrayT<float>& ts_e, const PyArrayT<float>& ts_v, const Config& cfg,
+           int device_id, bool /*quiet*/) {
+            check_nsamps(ts_e, ts_v, cfg);
+            FFA ffa(cfg, device_id, LOKI_FOLD_FOURIER);
+            const size_t buf = ffa.plan().info().buffer_size;
+            PyArrayT<std::complex<float>> fold(static_cast<py::ssize_t>(buf));
+            ffa.execute_complex(ts_e, ts_v, fold);
+            fold.resize({static_cast<py::ssize_t>(ffa.plan().info().fold_size)});
+            return py::make_tuple(fold, PlanFourier(cfg));
+        },
+        py::arg("ts_e"), py::arg("ts_v"), py::arg("cfg"), py::arg("device_id") = 0,
+        py::arg("quiet") = false);
+    m_ffa.def(
+        "compute_ffa_fourier_return_to_time_cuda",
+        [](const PyArrayT<float>& ts_e, const PyArrayT<float>& ts_v, const Config& cfg,
+           int device_id, bool /*quiet*/) {
+            check_nsamps(ts_e, ts_v, cfg);
+            FFA ffa(cfg, device_id, LOKI_FOLD_FOURIER);
+            const size_t buf = ffa.plan().info().buffer_size_time;
+            PyArrayT<float> fold(static_cast<py::ssize_t>(buf));
+            ffa.execute(ts_e, ts_v, fold);
+            fold.resize({static_cast<py::ssize_t>(ffa.plan().info().fold_size_time)});
+            return py::make_tuple(fold, PlanTime(cfg));
+        },
+        py::arg("ts_e"), py::arg("ts_v"), py::arg("cfg"), py::arg("device_id") = 0,
+        py::arg("quiet") = false);
+    m_ffa.def(
         "compute_ffa_scores_cuda",
         [](const PyArrayT<float>& ts_e, const PyArrayT<float>& ts_v, const Config& cfg,
            int device_id, bool /*quiet*/, std::optional<PyArrayT<size_t>> widths) {
             check_nsamps(ts_e, ts_v, cfg);
-            if (cfg.params().use_fourier != 0) {
-                throw std::runtime_error(
-                    "compute_ffa_scores_cuda: the Fourier-domain path is not available in this "
-                    "build; use use_fourier=False");
-            }
-            FFA ffa(cfg, device_id, LOKI_FOLD_TIME);
+            // ffa_cuda.cu:676-680: Fourier FFA returned to the time domain, or the time FFA
+            FFA ffa(cfg, device_id,
+                    cfg.params().use_fourier != 0 ? LOKI_FOLD_FOURIER : LOKI_FOLD_TIME);
             PyArrayT<size_t> w = widths.has_value() ? *widths : PyArrayT<size_t>(cfg.score_widths());
             auto scores = ffa.scores_host(ts_e, ts_v, w, false);
             return py::make_tuple(scores, PlanTime(cfg));

@@ -125,6 +125,13 @@ def irfft(fold_c, nbins, nthreads=None):
     return out
 
 
+def rfft(fold_t, nbins):
+    """Forward real DFT of rows of nbins floats (FP64 numpy, rounded to complex64): the
+    unnormalised R2C transform of fft_cuda.cu:282-355 / fft.cpp:375-500."""
+    x = np.ascontiguousarray(fold_t, np.float32).reshape(-1, nbins).astype(np.float64)
+    return np.fft.rfft(x, axis=1).astype(np.complex64).reshape(-1)
+
+
 def circular_prefix_sum(x, nsum):
     x = np.ascontiguousarray(x, np.float32)
     out = np.zeros(nsum, np.float32)
@@ -171,9 +178,11 @@ def compute_ffa(ts_e, ts_v, cfg: SearchConfig, fourier: bool = False, nthreads=N
     freqs = plan.brute_fold_freqs()
     t_ref = plan.brute_fold_tref()
     seglen = plan.segment_lens[0]
-    if fourier:
-        if cfg.nbins > cfg.nbins_min_lossy_bf:
-            raise NotImplementedError("lossy Fourier init (nbins > nbins_min_lossy_bf)")
+    if fourier and cfg.nbins > cfg.nbins_min_lossy_bf:
+        # lossy init (ffa.cpp:180-192, 207-234): time brute fold, then rfft
+        fold_t = brute_fold_time(ts_e, ts_v, freqs, seglen, cfg.nbins, cfg.tsamp, t_ref, nthreads)
+        fold = rfft(fold_t, cfg.nbins)
+    elif fourier:
         fold = brute_fold_fourier(ts_e, ts_v, freqs, seglen, cfg.nbins, cfg.tsamp, t_ref, nthreads)
     else:
         fold = brute_fold_time(ts_e, ts_v, freqs, seglen, cfg.nbins, cfg.tsamp, t_ref, nthreads,
@@ -205,5 +214,5 @@ def compute_ffa_scores(ts_e, ts_v, cfg: SearchConfig, widths=None, nthreads=None
 __all__ = [
     "FFA_COORD_DTYPE", "FFA_COORD_FREQ_DTYPE", "Plan", "SearchConfig", "brute_fold_fourier",
     "brute_fold_time", "build", "circular_prefix_sum", "compute_ffa", "compute_ffa_scores",
-    "diff_max", "ffa_iter", "irfft", "lib", "snr_boxcar_3d", "snr_boxcar_3d_f64",
+    "diff_max", "ffa_iter", "irfft", "lib", "rfft", "snr_boxcar_3d", "snr_boxcar_3d_f64",
 ]

@@ -322,6 +322,56 @@ def test_fourier_merge_equals_time_merge_for_integer_shifts(oracle):
     np.testing.assert_allclose(back, t_out, rtol=1e-5, atol=1e-5)
 
 
+@pytest.mark.parametrize("freq_only", [False, True])
+def test_fourier_merge_fractional_shift_theorem(oracle, freq_only):
+    """Fractional shifts: for band-limited rows x_j = sum_m a_m cos(2 pi m j / n + phi_m)
+    (m < n/2), the merge equals rfft of x_tail(j - s_t) + x_head(j - s_h) evaluated
+    analytically (the shift theorem), so a rounded shift, a sign flip or n vs n/2+1 in the
+    twiddle fails here even where the integer-shift identity holds."""
+    from oracle.plan import FFA_COORD_DTYPE, FFA_COORD_FREQ_DTYPE
+
+    rng = np.random.default_rng(11)
+    nbins, nprev, ncur, nseg = 32, 3, 6, 2
+    j = np.arange(nbins)
+    amp = rng.standard_normal((2 * nseg, nprev, 2, 15))
+    phi = rng.uniform(0, 2 * np.pi, (2 * nseg, nprev, 2, 15))
+    m = np.arange(1, 16)
+
+    def rows(shift):
+        arg = 2 * np.pi * m[:, None] * (j[None, :] - shift) / nbins
+        return np.einsum("...m,...mj->...j", amp, np.cos(arg[None, None, None] + phi[..., None]))
+
+    x = rows(0.0)  # [2*nseg, nprev, 2, nbins]
+    f_in = np.fft.rfft(x, axis=-1).astype(np.complex64)
+    if freq_only:
+        coords = np.zeros(ncur, FFA_COORD_FREQ_DTYPE)
+        coords["idx"] = rng.integers(0, nprev, ncur)
+        coords["shift"] = rng.uniform(0, nbins, ncur).astype(np.float32)
+        it, ih = coords["idx"], coords["idx"]
+        st, sh = np.zeros(ncur), coords["shift"].astype(np.float64)
+    else:
+        coords = np.zeros(ncur, FFA_COORD_DTYPE)
+        coords["i_tail"] = rng.integers(0, nprev, ncur)
+        coords["i_head"] = rng.integers(0, nprev, ncur)
+        coords["shift_tail"] = rng.uniform(0, nbins, ncur).astype(np.float32)
+        coords["shift_head"] = rng.uniform(0, nbins, ncur).astype(np.float32)
+        it, ih = coords["i_tail"], coords["i_head"]
+        st = coords["shift_tail"].astype(np.float64)
+        sh = coords["shift_head"].astype(np.float64)
+    got = oracle.ffa_iter(f_in.reshape(-1), coords, ncur, nprev, nseg, nbins, fourier=True)
+    got = got.reshape(nseg, ncur, 2, nbins // 2 + 1)
+    for s in range(nseg):
+        for c in range(ncur):
+            arg_t = 2 * np.pi * m[:, None] * (j[None, :] - st[c]) / nbins
+            arg_h = 2 * np.pi * m[:, None] * (j[None, :] - sh[c]) / nbins
+            xt = np.einsum("qm,qmj->qj", amp[2 * s, it[c]],
+                           np.cos(arg_t[None] + phi[2 * s, it[c]][..., None]))
+            xh = np.einsum("qm,qmj->qj", amp[2 * s + 1, ih[c]],
+                           np.cos(arg_h[None] + phi[2 * s + 1, ih[c]][..., None]))
+            want = np.fft.rfft(xt + xh, axis=-1)
+            np.testing.assert_allclose(got[s, c], want, rtol=0, atol=2e-4)
+
+
 # ---------------------------------------------------------------------------
 # Fourier helpers against numpy / direct sums
 # ---------------------------------------------------------------------------
