@@ -37,9 +37,9 @@ WORKLOADS = {
         nsamps=2**23, tsamp=64e-6, nbins=32, eta=1.0,
         param_limits=[(-50.0, 50.0), (100.0, 400.0)],
         signal=dict(f0=200.0, accel=20.0, jerk=0.0, fwhm=0.04, snr=15.0), seed=20231,
-        # CPU sample: the same grid rules (f_max = 400 Hz, bseg_brute = 1024) over the top
-        # 75 Hz of the band.
-        cpu_sample=dict(param_limits=[(-50.0, 50.0), (325.0, 400.0)], bseg_brute=1024),
+        # CPU baseline on the whole workload: ~10 s on 16 threads, ~75 GB of host memory
+        # (two fold levels, the shift tables and the scores).
+        cpu_sample=dict(param_limits=[(-50.0, 50.0), (100.0, 400.0)], bseg_brute=1024),
     ),
     # Fourier-domain FFA over (fddot, fdot, f) expressed as (jerk, accel, freq) at f_min = 50 Hz:
     # accel = c * fdot / f_min, jerk = c * fddot / f_min for |fdot| <= 1e-8 Hz/s,
@@ -72,6 +72,10 @@ def parse() -> argparse.Namespace:
                     help="hardware rsqrt profile normalisation (S/N within 1e-4, not bit-exact)")
     ap.add_argument("--unfused", action="store_true",
                     help="store the final fold and score it in a separate kernel")
+    ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"],
+                    help="fold storage precision of levels 0..L-2 (adds stay FP32)")
+    ap.add_argument("--no-storage-compare", action="store_true",
+                    help="skip timing the FP16 / BF16 storage variants beside the main run")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cpu count)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES (3 times): the "
@@ -201,6 +205,7 @@ def main() -> int:
         ffa = libculoki.ffa.FFACUDA(cfg, device_id=0)
         ffa.set_exact_norm(not args.fast_norm)
         ffa.set_fused(not args.unfused)
+        ffa.set_fold_storage(args.storage)
     plan = ffa.plan
     nc = int(plan.ncoords[-1])
     nw = int(widths.size)
@@ -231,6 +236,35 @@ def main() -> int:
     t1 = time.perf_counter()
     ffa.set_timing(False)
     ms_per_step = (t1 - t0) * 1e3 / args.steps
+
+    # The same step with the other fold storage precisions (same series, same stream).
+    storage_cmp = None
+    if not fourier and not args.no_storage_compare:
+        storage_cmp = {args.storage: ms_per_step}
+        for st in ("f32", "f16", "bf16"):
+            if st == args.storage:
+                continue
+            ffa.set_fold_storage(st)
+            with torch.cuda.stream(stream):
+                step()
+            torch.cuda.synchronize()
+            ta = time.perf_counter()
+            with torch.cuda.stream(stream):
+                for _ in range(args.steps):
+                    step()
+            torch.cuda.synchronize()
+            storage_cmp[st] = (time.perf_counter() - ta) * 1e3 / args.steps
+        ffa.set_fold_storage(args.storage)
+        with torch.cuda.stream(stream):
+            step()  # scores of the main run's storage for the recovery check
+        torch.cuda.synchronize()
+        storage_cmp = {
+            "ms_per_step": {k: round(v, 4) for k, v in storage_cmp.items()},
+            "speedup_vs_f32": {k: round(storage_cmp["f32"] / v, 4)
+                               for k, v in storage_cmp.items()},
+            "note": "levels 0..L-2 stored at that precision, every add in FP32; S/N error vs "
+                    "FP64 is stated by tests/test_gpu_parity.py::test_storage_snr_error_vs_fp64",
+        }
 
     # signal recovery check (outside the timed region)
     best = int(torch.argmax(scores).item())
@@ -282,12 +316,14 @@ def main() -> int:
             "widths": [int(w) for w in widths], "fold_final": [int(x) for x in plan.fold_shapes[-1]],
             "n_levels": int(plan.n_levels), "trials_per_step": nc,
             "domain": "fourier" if fourier else "time",
+            "fold_storage": "c64" if fourier else args.storage,
             "score_norm": "hw_rsqrt" if args.fast_norm else "exact",
             "final_level": "unfused" if (args.unfused or fourier) else "fused",
         },
         "roofline": roof,
         "pipeline_alg_GBps": total_alg / (ms_per_step * 1e-3) / 1e9,
         "kernel_ms": kern_ms,
+        "storage_compare": storage_cmp,
         "found": found,
         "injected": {"freq": sig["f0"], "accel": sig["accel"], "snr": sig["snr"]},
     }

@@ -205,6 +205,17 @@ int loki_ffa_set_fused(loki_ffa_t ffa, int enable);
  * (|dS/N| ~ 1e-6, the analogue of the reference CUDA path's fast-math e/sqrtf(v),
  * score_cuda.cu:82).  No reference counterpart as a switch. */
 int loki_ffa_set_exact_norm(loki_ffa_t ffa, int exact);
+/* Storage precision of the intermediate fold levels 0..L-2 of a time-domain FFA object:
+ * LOKI_STORAGE_F32 (default), LOKI_STORAGE_F16 or LOKI_STORAGE_BF16.  Each add is done in FP32
+ * and each stored value is rounded once (round to nearest even); level l is stored at the exact
+ * scale 2^-(k0 + l), k0 = floor(log2(segment_len / nbins)); the final level is returned and
+ * scored in FP32.  FP16 assumes inputs of order unity per sample (bucket sums beyond
+ * 65504 * 2^k0 overflow).  No reference counterpart (the reference stores FP32 or complex64
+ * folds, plans.hpp:151-201). */
+#define LOKI_STORAGE_F32 0
+#define LOKI_STORAGE_F16 1
+#define LOKI_STORAGE_BF16 2
+int loki_ffa_set_fold_storage(loki_ffa_t ffa, int storage);
 /* Waits for the recorded events; *n receives the number of timed launches. */
 int loki_ffa_timings(loki_ffa_t ffa, loki_kernel_timing* out, size_t cap, size_t* n);
 

@@ -428,6 +428,16 @@ int loki_ffa_set_exact_norm(loki_ffa_t ffa, int exact) {
     });
 }
 
+int loki_ffa_set_fold_storage(loki_ffa_t ffa, int storage) {
+    return guarded([&] {
+        REQ(ffa != nullptr, "null ffa");
+        REQ(storage == LOKI_STORAGE_F32 || storage == LOKI_STORAGE_F16 ||
+                storage == LOKI_STORAGE_BF16,
+            "fold storage must be 0 (f32), 1 (f16) or 2 (bf16)");
+        ffa->exec.set_fold_storage(static_cast<loki_amd::FoldStorage>(storage));
+    });
+}
+
 int loki_ffa_timings(loki_ffa_t ffa, loki_kernel_timing* out, size_t cap, size_t* n) {
     return guarded([&] {
         REQ(ffa != nullptr && n != nullptr, "null argument");

@@ -173,6 +173,11 @@ FFAExecutor::FFAExecutor(const SearchConfig& cfg, FoldKind kind, int device_id)
         LOKI_REQUIRE(m_plan.ncoords()[l - 1] < (1ULL << 32),
                      "parent grid exceeds 2^32 coordinates (u32 indices)");
     }
+    // level-0 scale of reduced-precision storage: bucket sums of about
+    // seglen/nbins samples are stored near unit magnitude
+    for (SizeType per = m_plan.segment_lens()[0] / cfg.get_nbins(); per > 1; per >>= 1) {
+        ++m_k0;
+    }
 }
 
 SizeType FFAExecutor::coord_record_bytes() const noexcept {
@@ -222,13 +227,20 @@ void FFAExecutor::prepare_brute_fold(hipStream_t s) {
     m_buckets_ready = true;
 }
 
+void FFAExecutor::set_fold_storage(FoldStorage st) {
+    LOKI_REQUIRE(!m_fourier || st == FoldStorage::kF32,
+                 "reduced-precision fold storage is available for the time-domain FFA only");
+    m_storage = st;
+}
+
 void FFAExecutor::init_level(const float* ts_e, const float* ts_v, void* out, void* scratch,
-                             hipStream_t s) {
+                             bool fp32_out, hipStream_t s) {
     const SizeType nbins = m_plan.config().get_nbins();
     const SizeType nseg  = m_plan.nsegments()[0];
     const int bslot      = m_timer.start(KernelKind::kBruteFold, 0, s);
     if (!m_fourier) {
-        kern::brute_fold_time(ts_e, ts_v, nseg, m_bf, static_cast<float*>(out), s);
+        const FoldStorage so = fp32_out ? FoldStorage::kF32 : m_storage;
+        kern::brute_fold_time(ts_e, ts_v, nseg, m_bf, out, so, std::ldexp(1.0F, -m_k0), s);
         check_launch("brute fold launch");
         m_timer.stop(bslot, s);
         return;
@@ -247,7 +259,7 @@ void FFAExecutor::init_level(const float* ts_e, const float* ts_v, void* out, vo
     // Lossy init (ffa_cuda.cu:370-398): time brute fold into the partner
     // buffer, then a forward real DFT of every (segment, freq, component) row.
     auto* tf = static_cast<float*>(scratch);
-    kern::brute_fold_time(ts_e, ts_v, nseg, m_bf, tf, s);
+    kern::brute_fold_time(ts_e, ts_v, nseg, m_bf, tf, FoldStorage::kF32, 1.0F, s);
     check_launch("brute fold launch");
     m_timer.stop(bslot, s);
     const SizeType nrows = nseg * m_bf.nfreqs * 2;
@@ -258,7 +270,8 @@ void FFAExecutor::init_level(const float* ts_e, const float* ts_v, void* out, vo
 }
 
 void* FFAExecutor::run_levels(const float* ts_e, const float* ts_v, void* buf_even,
-                              void* buf_odd, SizeType last_level, hipStream_t s) {
+                              void* buf_odd, SizeType last_level, bool last_fp32,
+                              hipStream_t s) {
     // Level k lands in buf_even for even k and in buf_odd for odd k (the
     // reference picks its ping-pong start side the same way so the last level
     // lands in the caller's buffer, ffa.cpp:261-276).
@@ -271,7 +284,7 @@ void* FFAExecutor::run_levels(const float* ts_e, const float* ts_v, void* buf_ev
     void* even           = buf_even;
     void* odd            = buf_odd;
     const SizeType nbins = m_plan.config().get_nbins();
-    init_level(ts_e, ts_v, even, odd, s);
+    init_level(ts_e, ts_v, even, odd, last_fp32 && last_level == 0, s);
 
     for (SizeType l = 1; l <= last_level; ++l) {
         const SizeType off = m_plan.ncoords_offsets64()[l];
@@ -290,9 +303,15 @@ void* FFAExecutor::run_levels(const float* ts_e, const float* ts_v, void* buf_ev
                                 m_plan.ncoords()[l], m_plan.ncoords()[l - 1],
                                 m_plan.nsegments()[l], static_cast<uint32_t>(nbins), s);
         } else {
-            kern::merge_time(static_cast<const float*>(in), static_cast<float*>(out), cp, cfp,
-                             m_plan.ncoords()[l], m_plan.ncoords()[l - 1], m_plan.nsegments()[l],
-                             m_plan.param_counts()[l].back(), static_cast<uint32_t>(nbins), s);
+            // reduced precision: (tail + head) * 0.5 keeps level l at 2^-(k0 + l); a last
+            // level returned in FP32 is unscaled by 2^(k0 + l - 1) instead
+            const bool to32      = last_fp32 && l == last_level;
+            const FoldStorage so = to32 ? FoldStorage::kF32 : m_storage;
+            const float scale =
+                to32 ? std::ldexp(1.0F, m_k0 + static_cast<int>(l) - 1) : 0.5F;
+            kern::merge_time(in, out, cp, cfp, m_plan.ncoords()[l], m_plan.ncoords()[l - 1],
+                             m_plan.nsegments()[l], m_plan.param_counts()[l].back(),
+                             static_cast<uint32_t>(nbins), m_storage, so, scale, s);
         }
         check_launch("merge launch");
         m_timer.stop(slot, s);
@@ -318,9 +337,9 @@ void FFAExecutor::execute(const float* ts_e, const float* ts_v, void* fold, Size
     const SizeType L = m_plan.n_levels();
     m_internal.reserve(m_plan.buffer_size() * elem_bytes());
     if ((L - 1) % 2 == 0) {
-        run_levels(ts_e, ts_v, fold, m_internal.get(), L - 1, s);
+        run_levels(ts_e, ts_v, fold, m_internal.get(), L - 1, true, s);
     } else {
-        run_levels(ts_e, ts_v, m_internal.get(), fold, L - 1, s);
+        run_levels(ts_e, ts_v, m_internal.get(), fold, L - 1, true, s);
     }
 }
 
@@ -338,8 +357,8 @@ void FFAExecutor::execute_time(const float* ts_e, const float* ts_v, float* fold
     // the time domain into the caller's buffer (ffa_cuda.cu:248-290).
     m_internal.reserve(m_plan.buffer_size() * elem_bytes());
     const void* fin = ((L - 1) % 2 == 0)
-                          ? run_levels(ts_e, ts_v, m_internal.get(), fold, L - 1, s)
-                          : run_levels(ts_e, ts_v, fold, m_internal.get(), L - 1, s);
+                          ? run_levels(ts_e, ts_v, m_internal.get(), fold, L - 1, true, s)
+                          : run_levels(ts_e, ts_v, fold, m_internal.get(), L - 1, true, s);
     const SizeType nbins = m_plan.config().get_nbins();
     const int slot       = m_timer.start(KernelKind::kTransform, static_cast<int>(L - 1), s);
     kern::irfft_rows(static_cast<const float2*>(fin), fold, m_plan.fold_size_time() / nbins,
@@ -372,22 +391,24 @@ void FFAExecutor::execute_scores(const float* ts_e, const float* ts_v,
     if (m_fused && fused_scores_supported()) {
         // Levels 0..L-2 through HBM, the last level fused with its shift
         // tables and the S/N: the final fold cube is never stored.
-        const SizeType need = max_level_size(L - 2) * sizeof(float);
+        const SizeType need = max_level_size(L - 2) * storage_bytes(m_storage);
         m_internal.reserve(need);
         m_result.reserve(need);
-        const auto* parent = static_cast<const float*>(
-            run_levels(ts_e, ts_v, m_result.get(), m_internal.get(), L - 2, s));
+        const void* parent =
+            run_levels(ts_e, ts_v, m_result.get(), m_internal.get(), L - 2, false, s);
         const auto lv  = make_level(m_plan, L - 1);
         const int slot = m_timer.start(KernelKind::kMergeScore, static_cast<int>(L - 1), s);
-        kern::final_merge_score(parent, ncoords, m_plan.ncoords()[L - 2], lv, wt, scores,
-                                find_max, m_exact_norm, s);
+        kern::final_merge_score(parent, m_storage, std::ldexp(1.0F, m_k0 + static_cast<int>(L) - 2),
+                                ncoords, m_plan.ncoords()[L - 2], lv, wt, scores, find_max,
+                                m_exact_norm, s);
         check_launch("final-level merge+score launch");
         m_timer.stop(slot, s);
         return;
     }
     m_internal.reserve(m_plan.buffer_size() * elem_bytes());
     m_result.reserve(m_plan.buffer_size() * elem_bytes());
-    const void* last = run_levels(ts_e, ts_v, m_result.get(), m_internal.get(), L - 1, s);
+    const void* last =
+        run_levels(ts_e, ts_v, m_result.get(), m_internal.get(), L - 1, true, s);
     const float* fin = static_cast<const float*>(last);
     if (m_fourier) {
         // Back to the time domain into the other ping-pong buffer: it holds
@@ -422,7 +443,8 @@ void brute_fold_time_device(const float* ts_e, const float* ts_v, SizeType nsamp
     const auto lists = kern::build_brute_fold_lists(
         d_freqs.as<double>(), static_cast<uint32_t>(freqs.size()),
         static_cast<uint32_t>(segment_len), static_cast<uint32_t>(nbins), tsamp, t_ref, s);
-    kern::brute_fold_time(ts_e, ts_v, nsamps / segment_len, lists, fold, s);
+    kern::brute_fold_time(ts_e, ts_v, nsamps / segment_len, lists, fold, FoldStorage::kF32, 1.0F,
+                          s);
     check_launch("brute fold launch");
     // The lists are freed on return: finish before releasing them.
     hip_check(hipStreamSynchronize(s), "brute fold");

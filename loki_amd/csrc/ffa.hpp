@@ -12,6 +12,7 @@
 #include "device.hpp"
 #include "kernels.hpp"
 #include "plan.hpp"
+#include "storage.hpp"
 
 namespace loki_amd {
 
@@ -100,6 +101,10 @@ public:
     /// Profile normalisation of the fused scores path: exact (default, the
     /// CPU's correctly rounded e * (1/sqrt(v))) or the hardware rsqrt.
     void set_exact_norm(bool on) noexcept { m_exact_norm = on; }
+    /// Storage precision of fold levels 0..L-2 (time domain only); the final
+    /// level is always returned and scored in FP32.  Default FP32.
+    void set_fold_storage(FoldStorage st);
+    FoldStorage fold_storage() const noexcept { return m_storage; }
     bool fused_scores_supported() const;
 
     /// Device pointer to the last shift tables resolved by execute().
@@ -120,15 +125,19 @@ private:
     bool m_buckets_ready{false};
     bool m_fused{true};
     bool m_exact_norm{true};
+    FoldStorage m_storage{FoldStorage::kF32};
+    int m_k0{0};  // level l of a reduced-precision tree is stored at scale 2^-(k0 + l)
     LaunchTimer m_timer;
 
     void prepare_brute_fold(hipStream_t s);
     /// Brute fold + merge levels 1..last_level; returns the buffer holding
     /// last_level (buf_even for even levels, buf_odd for odd levels).
+    /// last_fp32: last_level is stored in FP32 at its true scale even when
+    /// the tree uses reduced-precision storage.
     void* run_levels(const float* ts_e, const float* ts_v, void* buf_even, void* buf_odd,
-                     SizeType last_level, hipStream_t s);
+                     SizeType last_level, bool last_fp32, hipStream_t s);
     void init_level(const float* ts_e, const float* ts_v, void* out, void* scratch,
-                    hipStream_t s);
+                    bool fp32_out, hipStream_t s);
     SizeType max_level_size(SizeType last_level) const;
     SizeType elem_bytes() const noexcept { return m_fourier ? 2 * sizeof(float) : sizeof(float); }
 };

@@ -8,8 +8,10 @@
 // that lanes holding consecutive bins read consecutive words at every step.
 // The fold runs one lane per (segment, freq, bin), summing its bucket in
 // ascending order from samples staged in LDS -- bit-identical to the CPU
-// order and deterministic.
+// order and deterministic.  Reduced-precision storage rounds each FP32 bucket
+// sum once (storage.hpp).
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "device.hpp"
@@ -97,12 +99,14 @@ k_bf_fill(const double* __restrict__ freqs, uint32_t seglen, uint32_t nbins, dou
 // One lane per (segment, freq, bin) pair; consecutive lanes are consecutive
 // bins, so list reads and output writes are contiguous.  USE_LDS: the
 // segment's samples are staged in LDS (seglen <= kLdsSamps).
-template <bool USE_LDS>
+// TO: stored element type of the fold (float, or _Float16 / __bf16 with each
+// FP32 sum multiplied by `scale` before rounding).
+template <bool USE_LDS, typename TO>
 __global__ void __launch_bounds__(kBlock)
 k_bf_fold(const float* __restrict__ ts_e, const float* __restrict__ ts_v, uint32_t seglen,
           uint32_t nfreqs, uint32_t nbins, uint32_t blocks_per_seg,
           const uint32_t* __restrict__ list, const uint64_t* __restrict__ base_f,
-          const uint32_t* __restrict__ counts, float* __restrict__ fold) {
+          const uint32_t* __restrict__ counts, TO* __restrict__ fold, float scale) {
     __shared__ float s_e[USE_LDS ? kLdsSamps : 1];
     __shared__ float s_v[USE_LDS ? kLdsSamps : 1];
     const uint64_t seg   = blockIdx.x / blocks_per_seg;
@@ -149,9 +153,14 @@ k_bf_fold(const float* __restrict__ ts_e, const float* __restrict__ ts_v, uint32
             se = se + e[i];
             sv = sv + v[i];
         }
-        float* __restrict__ out = fold + (seg * nfreqs + f) * 2 * nbins;
-        out[b]         = se;
-        out[nbins + b] = sv;
+        TO* __restrict__ out = fold + (seg * nfreqs + f) * 2 * nbins;
+        if constexpr (std::is_same_v<TO, float>) {
+            out[b]         = se;
+            out[nbins + b] = sv;
+        } else {
+            out[b]         = narrow1<TO>(se * scale);
+            out[nbins + b] = narrow1<TO>(sv * scale);
+        }
     }
 }
 
@@ -196,8 +205,27 @@ BruteFoldLists build_brute_fold_lists(const double* freqs_dev, uint32_t nfreqs, 
     return L;
 }
 
+namespace {
+
+template <typename TO>
+void launch_fold(const float* ts_e, const float* ts_v, uint64_t blocks, uint32_t bps,
+                 const BruteFoldLists& L, TO* fold, float scale, hipStream_t s) {
+    if (L.seglen <= static_cast<uint32_t>(kLdsSamps)) {
+        k_bf_fold<true, TO><<<static_cast<unsigned>(blocks), kBlock, 0, s>>>(
+            ts_e, ts_v, L.seglen, L.nfreqs, L.nbins, bps, L.list.as<uint32_t>(),
+            L.base.as<uint64_t>(), L.counts.as<uint32_t>(), fold, scale);
+    } else {
+        k_bf_fold<false, TO><<<static_cast<unsigned>(blocks), kBlock, 0, s>>>(
+            ts_e, ts_v, L.seglen, L.nfreqs, L.nbins, bps, L.list.as<uint32_t>(),
+            L.base.as<uint64_t>(), L.counts.as<uint32_t>(), fold, scale);
+    }
+}
+
+}  // namespace
+
 void brute_fold_time(const float* ts_e, const float* ts_v, uint64_t nseg,
-                     const BruteFoldLists& L, float* fold, hipStream_t s) {
+                     const BruteFoldLists& L, void* fold, FoldStorage so, float scale,
+                     hipStream_t s) {
     const uint64_t pairs = static_cast<uint64_t>(L.nfreqs) * L.nbins;
     LOKI_REQUIRE(pairs < (1ULL << 31), "brute fold: too many (freq, bin) pairs");
     const uint32_t bps    = static_cast<uint32_t>((pairs + kPairsBlk - 1) / kPairsBlk);
@@ -206,14 +234,13 @@ void brute_fold_time(const float* ts_e, const float* ts_v, uint64_t nseg,
     if (blocks == 0) {
         return;
     }
-    if (L.seglen <= static_cast<uint32_t>(kLdsSamps)) {
-        k_bf_fold<true><<<static_cast<unsigned>(blocks), kBlock, 0, s>>>(
-            ts_e, ts_v, L.seglen, L.nfreqs, L.nbins, bps, L.list.as<uint32_t>(),
-            L.base.as<uint64_t>(), L.counts.as<uint32_t>(), fold);
-    } else {
-        k_bf_fold<false><<<static_cast<unsigned>(blocks), kBlock, 0, s>>>(
-            ts_e, ts_v, L.seglen, L.nfreqs, L.nbins, bps, L.list.as<uint32_t>(),
-            L.base.as<uint64_t>(), L.counts.as<uint32_t>(), fold);
+    switch (so) {
+    case FoldStorage::kF32:
+        return launch_fold(ts_e, ts_v, blocks, bps, L, static_cast<float*>(fold), scale, s);
+    case FoldStorage::kF16:
+        return launch_fold(ts_e, ts_v, blocks, bps, L, static_cast<_Float16*>(fold), scale, s);
+    case FoldStorage::kBF16:
+        return launch_fold(ts_e, ts_v, blocks, bps, L, static_cast<__bf16*>(fold), scale, s);
     }
 }
 

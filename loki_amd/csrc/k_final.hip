@@ -14,8 +14,11 @@
 //      circular prefix sum in VGPRs + its LDS row, max window difference per
 //      width, snr = (h + b) * dmax - b * sum.
 // Every FP32 operation matches the unfused path and the CPU oracle bit for
-// bit.
+// bit.  With FP16/BF16 parents (storage.hpp) the loads are 8-byte chunks
+// widened to FP32 and the merged sums are unscaled exactly before step 2's
+// normalisation.
 #include <cfloat>
+#include <type_traits>
 
 #include "merge_device.hpp"
 #include "resolve_device.hpp"
@@ -38,11 +41,14 @@ __device__ __forceinline__ float4 normalise4(const float4 e, const float4 v) {
     return p;
 }
 
-template <int NB, int NP, bool FINDMAX, bool EXACT>
+// TI: stored element type of the parent level; unless float, the merged e and
+// v are multiplied by `unscale` (2^(k0 + L - 2), exact) before normalisation.
+template <int NB, int NP, bool FINDMAX, bool EXACT, typename TI>
 __global__ void __launch_bounds__(kRows)
-k_final_merge_score(const float* __restrict__ in, uint64_t ncur, uint64_t nprev,
+k_final_merge_score(const TI* __restrict__ in, uint64_t ncur, uint64_t nprev,
                     ResolveLevel lv, FinalGrid fg, WidthTable wt, float* __restrict__ scores,
-                    TileSchedule ts) {
+                    TileSchedule ts, float unscale) {
+    using CI = chunk4_t<TI>;
     constexpr uint32_t C4  = NB / 4;        // float4 chunks per component row
     constexpr uint32_t LPR = C4;            // lanes per row (4 bins of e and v each)
     constexpr uint32_t RPI = kRows / LPR;   // rows per merge step
@@ -79,21 +85,21 @@ k_final_merge_score(const float* __restrict__ in, uint64_t ncur, uint64_t nprev,
                 const uint32_t r = rsub + (s0 + u) * RPI;
                 if (r < nrow) {
                     const FFACoord cd = s_coord[r];
-                    const float4* __restrict__ trow = reinterpret_cast<const float4*>(
+                    const CI* __restrict__ trow = reinterpret_cast<const CI*>(
                         in + static_cast<uint64_t>(cd.i_tail) * (2 * NB));
-                    const float4* __restrict__ hrow = reinterpret_cast<const float4*>(
+                    const CI* __restrict__ hrow = reinterpret_cast<const CI*>(
                         in + (nprev + static_cast<uint64_t>(cd.i_head)) * (2 * NB));
                     if constexpr (NP == 1) {
-                        te[u] = trow[i4];
-                        tv[u] = trow[C4 + i4];
+                        te[u] = widen4<TI>(trow[i4]);
+                        tv[u] = widen4<TI>(trow[C4 + i4]);
                     } else {
                         const uint32_t St = shift_int(cd.shift_tail, NB);
-                        te[u] = rotated4<NB>(trow, i4, St);
-                        tv[u] = rotated4<NB>(trow + C4, i4, St);
+                        te[u] = rotated4_st<NB, TI>(trow, i4, St);
+                        tv[u] = rotated4_st<NB, TI>(trow + C4, i4, St);
                     }
                     const uint32_t Sh = shift_int(cd.shift_head, NB);
-                    he[u] = rotated4<NB>(hrow, i4, Sh);
-                    hv[u] = rotated4<NB>(hrow + C4, i4, Sh);
+                    he[u] = rotated4_st<NB, TI>(hrow, i4, Sh);
+                    hv[u] = rotated4_st<NB, TI>(hrow + C4, i4, Sh);
                 }
             }
 #pragma unroll
@@ -109,6 +115,16 @@ k_final_merge_score(const float* __restrict__ in, uint64_t ncur, uint64_t nprev,
                     v.y = tv[u].y + hv[u].y;
                     v.z = tv[u].z + hv[u].z;
                     v.w = tv[u].w + hv[u].w;
+                    if constexpr (!std::is_same_v<TI, float>) {
+                        e.x *= unscale;
+                        e.y *= unscale;
+                        e.z *= unscale;
+                        e.w *= unscale;
+                        v.x *= unscale;
+                        v.y *= unscale;
+                        v.z *= unscale;
+                        v.w *= unscale;
+                    }
                     const float4 p = normalise4<EXACT>(e, v);
                     float* __restrict__ dst = s_p + r * stride + 4 * i4;
                     dst[0] = p.x;
@@ -158,9 +174,10 @@ k_final_merge_score(const float* __restrict__ in, uint64_t ncur, uint64_t nprev,
     }
 }
 
-template <int NB, int NP>
-void launch_nb_np(const float* in, uint64_t ncur, uint64_t nprev, const ResolveLevel& lv,
-                  const WidthTable& wt, float* scores, bool find_max, bool exact, hipStream_t s) {
+template <int NB, int NP, typename TI>
+void launch_nb_np(const TI* in, float unscale, uint64_t ncur, uint64_t nprev,
+                  const ResolveLevel& lv, const WidthTable& wt, float* scores, bool find_max,
+                  bool exact, hipStream_t s) {
     // nseg == 1 at the last level; child rows sharing parents run back to back
     const TileSchedule ts = make_schedule(ncur, lv.n_cur[2], 1, kRows, kRowGroup, kRun);
     const uint32_t nb     = ts.nblocks;
@@ -171,28 +188,46 @@ void launch_nb_np(const float* in, uint64_t ncur, uint64_t nprev, const ResolveL
     fg.stride        = (static_cast<uint32_t>(NB) + wt.wmax) | 1U;  // odd: conflict-free rows
     const size_t lds = static_cast<size_t>(kRows) * fg.stride * sizeof(float);
     if (find_max && exact) {
-        k_final_merge_score<NB, NP, true, true>
-            <<<nb, kRows, lds, s>>>(in, ncur, nprev, lv, fg, wt, scores, ts);
+        k_final_merge_score<NB, NP, true, true, TI>
+            <<<nb, kRows, lds, s>>>(in, ncur, nprev, lv, fg, wt, scores, ts, unscale);
     } else if (find_max) {
-        k_final_merge_score<NB, NP, true, false>
-            <<<nb, kRows, lds, s>>>(in, ncur, nprev, lv, fg, wt, scores, ts);
+        k_final_merge_score<NB, NP, true, false, TI>
+            <<<nb, kRows, lds, s>>>(in, ncur, nprev, lv, fg, wt, scores, ts, unscale);
     } else if (exact) {
-        k_final_merge_score<NB, NP, false, true>
-            <<<nb, kRows, lds, s>>>(in, ncur, nprev, lv, fg, wt, scores, ts);
+        k_final_merge_score<NB, NP, false, true, TI>
+            <<<nb, kRows, lds, s>>>(in, ncur, nprev, lv, fg, wt, scores, ts, unscale);
     } else {
-        k_final_merge_score<NB, NP, false, false>
-            <<<nb, kRows, lds, s>>>(in, ncur, nprev, lv, fg, wt, scores, ts);
+        k_final_merge_score<NB, NP, false, false, TI>
+            <<<nb, kRows, lds, s>>>(in, ncur, nprev, lv, fg, wt, scores, ts, unscale);
     }
 }
 
-template <int NB>
-void launch_nb(const float* in, uint64_t ncur, uint64_t nprev, const ResolveLevel& lv,
-               const WidthTable& wt, float* scores, bool find_max, bool exact, hipStream_t s) {
+template <int NB, typename TI>
+void launch_nb(const TI* in, float unscale, uint64_t ncur, uint64_t nprev,
+               const ResolveLevel& lv, const WidthTable& wt, float* scores, bool find_max,
+               bool exact, hipStream_t s) {
     switch (lv.np) {
-    case 1: return launch_nb_np<NB, 1>(in, ncur, nprev, lv, wt, scores, find_max, exact, s);
-    case 2: return launch_nb_np<NB, 2>(in, ncur, nprev, lv, wt, scores, find_max, exact, s);
-    case 3: return launch_nb_np<NB, 3>(in, ncur, nprev, lv, wt, scores, find_max, exact, s);
+    case 1:
+        return launch_nb_np<NB, 1>(in, unscale, ncur, nprev, lv, wt, scores, find_max, exact, s);
+    case 2:
+        return launch_nb_np<NB, 2>(in, unscale, ncur, nprev, lv, wt, scores, find_max, exact, s);
+    case 3:
+        return launch_nb_np<NB, 3>(in, unscale, ncur, nprev, lv, wt, scores, find_max, exact, s);
     default: LOKI_REQUIRE(false, "final-level kernel: np must be 1, 2 or 3");
+    }
+}
+
+template <typename TI>
+void launch_typed(const void* in, float unscale, uint64_t ncur, uint64_t nprev,
+                  const ResolveLevel& lv, const WidthTable& wt, float* scores, bool find_max,
+                  bool exact, hipStream_t s) {
+    const auto* p = static_cast<const TI*>(in);
+    switch (lv.nbins) {
+    case 8: return launch_nb<8>(p, unscale, ncur, nprev, lv, wt, scores, find_max, exact, s);
+    case 16: return launch_nb<16>(p, unscale, ncur, nprev, lv, wt, scores, find_max, exact, s);
+    case 32: return launch_nb<32>(p, unscale, ncur, nprev, lv, wt, scores, find_max, exact, s);
+    case 64: return launch_nb<64>(p, unscale, ncur, nprev, lv, wt, scores, find_max, exact, s);
+    default: break;
     }
 }
 
@@ -202,20 +237,22 @@ bool final_merge_score_supported(uint32_t nbins) {
     return nbins == 8 || nbins == 16 || nbins == 32 || nbins == 64;
 }
 
-void final_merge_score(const float* in, uint64_t ncur, uint64_t nprev, const ResolveLevel& lv,
-                       const WidthTable& wt, float* scores, bool find_max, bool exact,
-                       hipStream_t s) {
+void final_merge_score(const void* in, FoldStorage si, float unscale, uint64_t ncur,
+                       uint64_t nprev, const ResolveLevel& lv, const WidthTable& wt,
+                       float* scores, bool find_max, bool exact, hipStream_t s) {
     LOKI_REQUIRE(final_merge_score_supported(lv.nbins), "final-level kernel: unsupported nbins");
     LOKI_REQUIRE(wt.wmax < lv.nbins, "boxcar widths must be < nbins");
     if (ncur == 0) {
         return;
     }
-    switch (lv.nbins) {
-    case 8: return launch_nb<8>(in, ncur, nprev, lv, wt, scores, find_max, exact, s);
-    case 16: return launch_nb<16>(in, ncur, nprev, lv, wt, scores, find_max, exact, s);
-    case 32: return launch_nb<32>(in, ncur, nprev, lv, wt, scores, find_max, exact, s);
-    case 64: return launch_nb<64>(in, ncur, nprev, lv, wt, scores, find_max, exact, s);
-    default: break;
+    switch (si) {
+    case FoldStorage::kF32:
+        return launch_typed<float>(in, unscale, ncur, nprev, lv, wt, scores, find_max, exact, s);
+    case FoldStorage::kF16:
+        return launch_typed<_Float16>(in, unscale, ncur, nprev, lv, wt, scores, find_max, exact,
+                                      s);
+    case FoldStorage::kBF16:
+        return launch_typed<__bf16>(in, unscale, ncur, nprev, lv, wt, scores, find_max, exact, s);
     }
 }
 

@@ -13,7 +13,12 @@
 // dwordx4 per side and funnels them; the parent rows of a block's tile of
 // consecutive coordinates overlap heavily and are served by L1/L2.  Blocks are
 // remapped so each XCD works on a contiguous range of tiles (parent reuse
-// stays inside one L2).  All offsets are 64-bit.
+// stays inside one L2).  All offsets are 64-bit.  With FP16/BF16 storage
+// (storage.hpp) a lane reads 4 elements as 8 bytes and rotates them with a
+// 16-bit funnel shift; the add stays in FP32.
+#include <tuple>
+#include <type_traits>
+
 #include "kernels.hpp"
 #include "merge_device.hpp"
 
@@ -24,12 +29,21 @@ constexpr int kBlock         = 256;
 constexpr uint32_t kRowGroup = 16;  // child rows scheduled back to back (parent reuse in L2)
 constexpr uint32_t kRun      = 16;  // frequency tiles per sequential output run
 
-template <int NB, bool POLY, int ITERS>
+// TI/TO: stored element types of the parent and child levels (float, or
+// _Float16 / __bf16 for reduced-precision storage).  Unless both are float the
+// FP32 sum is multiplied by `scale` before it is rounded to TO (0.5 between
+// reduced-precision levels, 2^(k0 + l) when the level is returned as FP32).
+template <typename TI, typename TO>
+constexpr bool kScaled = !(std::is_same_v<TI, float> && std::is_same_v<TO, float>);
+
+template <int NB, bool POLY, int ITERS, typename TI, typename TO>
 __global__ void __launch_bounds__(kBlock)
-k_merge_v4(const float* __restrict__ in, float* __restrict__ out,
+k_merge_v4(const TI* __restrict__ in, TO* __restrict__ out,
            const FFACoord* __restrict__ coords, const FFACoordFreq* __restrict__ cf,
-           uint64_t ncur, uint64_t nprev, TileSchedule ts) {
-    constexpr uint32_t C4   = NB / 4;        // float4 chunks per component
+           uint64_t ncur, uint64_t nprev, TileSchedule ts, float scale) {
+    using CI = chunk4_t<TI>;
+    using CO = chunk4_t<TO>;
+    constexpr uint32_t C4   = NB / 4;        // 4-element chunks per component
     constexpr uint32_t LPR  = 2 * C4;        // chunks per row
     constexpr uint32_t LANES = LPR < kBlock ? LPR : kBlock;
     constexpr uint32_t ROWS = kBlock / LANES;
@@ -67,21 +81,19 @@ k_merge_v4(const float* __restrict__ in, float* __restrict__ out,
                 St     = 0;
                 Sh     = shift_int(cd.shift, NB);
             }
-            const float4* __restrict__ trow =
-                reinterpret_cast<const float4*>(in + it_row * (2 * NB));
-            const float4* __restrict__ hrow =
-                reinterpret_cast<const float4*>(in + ih_row * (2 * NB));
+            const CI* __restrict__ trow = reinterpret_cast<const CI*>(in + it_row * (2 * NB));
+            const CI* __restrict__ hrow = reinterpret_cast<const CI*>(in + ih_row * (2 * NB));
 #pragma unroll
             for (uint32_t k = 0; k < CPL; ++k) {
                 const uint32_t ch = lch + k * LANES;
                 const uint32_t q  = ch / C4;
                 const uint32_t i4 = ch % C4;
                 if constexpr (POLY) {
-                    tv[it][k] = rotated4<NB>(trow + q * C4, i4, St);
+                    tv[it][k] = rotated4_st<NB, TI>(trow + q * C4, i4, St);
                 } else {
-                    tv[it][k] = trow[ch];
+                    tv[it][k] = widen4<TI>(trow[ch]);
                 }
-                hv[it][k] = rotated4<NB>(hrow + q * C4, i4, Sh);
+                hv[it][k] = rotated4_st<NB, TI>(hrow + q * C4, i4, Sh);
             }
         }
     }
@@ -90,8 +102,7 @@ k_merge_v4(const float* __restrict__ in, float* __restrict__ out,
         const uint32_t local = static_cast<uint32_t>(it) * ROWS + rsub;
         const uint64_t c     = tl.c0 + local;
         if (local < tl.count) {
-            float4* __restrict__ orow =
-                reinterpret_cast<float4*>(out + (seg * ncur + c) * (2 * NB));
+            CO* __restrict__ orow = reinterpret_cast<CO*>(out + (seg * ncur + c) * (2 * NB));
 #pragma unroll
             for (uint32_t k = 0; k < CPL; ++k) {
                 float4 o;
@@ -99,18 +110,24 @@ k_merge_v4(const float* __restrict__ in, float* __restrict__ out,
                 o.y = tv[it][k].y + hv[it][k].y;
                 o.z = tv[it][k].z + hv[it][k].z;
                 o.w = tv[it][k].w + hv[it][k].w;
-                orow[lch + k * LANES] = o;
+                if constexpr (kScaled<TI, TO>) {
+                    o.x *= scale;
+                    o.y *= scale;
+                    o.z *= scale;
+                    o.w *= scale;
+                }
+                orow[lch + k * LANES] = narrow4<TO>(o);
             }
         }
     }
 }
 
 // Any nbins: one lane per output element pair (e and v of bin j).
-template <bool POLY>
+template <bool POLY, typename TI, typename TO>
 __global__ void __launch_bounds__(kBlock)
-k_merge_scalar(const float* __restrict__ in, float* __restrict__ out,
+k_merge_scalar(const TI* __restrict__ in, TO* __restrict__ out,
                const FFACoord* __restrict__ coords, const FFACoordFreq* __restrict__ cf,
-               uint64_t ncur, uint64_t nprev, uint64_t nseg, uint32_t nbins) {
+               uint64_t ncur, uint64_t nprev, uint64_t nseg, uint32_t nbins, float scale) {
     const uint64_t total = nseg * ncur * nbins;
     for (uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
          t += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
@@ -135,69 +152,99 @@ k_merge_scalar(const float* __restrict__ in, float* __restrict__ out,
         }
         const uint32_t jt = j < St ? j + nbins - St : j - St;
         const uint32_t jh = j < Sh ? j + nbins - Sh : j - Sh;
-        const float* __restrict__ tr = in + it_row * 2 * nbins;
-        const float* __restrict__ hr = in + ih_row * 2 * nbins;
-        float* __restrict__ o        = out + (seg * ncur + c) * 2 * nbins;
-        o[j]         = tr[jt] + hr[jh];
-        o[nbins + j] = tr[nbins + jt] + hr[nbins + jh];
+        const TI* __restrict__ tr = in + it_row * 2 * nbins;
+        const TI* __restrict__ hr = in + ih_row * 2 * nbins;
+        TO* __restrict__ o        = out + (seg * ncur + c) * 2 * nbins;
+        float oe = widen1(tr[jt]) + widen1(hr[jh]);
+        float ov = widen1(tr[nbins + jt]) + widen1(hr[nbins + jh]);
+        if constexpr (kScaled<TI, TO>) {
+            oe *= scale;
+            ov *= scale;
+        }
+        o[j]         = narrow1<TO>(oe);
+        o[nbins + j] = narrow1<TO>(ov);
     }
 }
 
-template <int NB, bool POLY>
-void launch_v4(const float* in, float* out, const FFACoord* coords, const FFACoordFreq* cf,
-               uint64_t ncur, uint64_t nprev, uint64_t nseg, uint64_t n_f, hipStream_t s) {
+template <int NB, bool POLY, typename TI, typename TO>
+void launch_v4(const TI* in, TO* out, const FFACoord* coords, const FFACoordFreq* cf,
+               uint64_t ncur, uint64_t nprev, uint64_t nseg, uint64_t n_f, float scale,
+               hipStream_t s) {
     constexpr uint32_t LPR   = NB / 2;
     constexpr uint32_t LANES = LPR < kBlock ? LPR : kBlock;
     constexpr uint32_t ROWS  = kBlock / LANES;
     constexpr uint32_t CPL   = LPR / LANES;
-    // rows per lane: up to 64 rows per block, at most 16 float4 loads in flight per lane
+    // rows per lane: up to 64 rows per block, at most 16 chunk loads in flight per lane
     constexpr uint32_t IT_ROWS = ROWS >= 64 ? 1U : 64U / ROWS;
     constexpr uint32_t IT_REGS = 8U / CPL;
     constexpr int ITERS      = static_cast<int>(IT_ROWS < IT_REGS ? IT_ROWS : IT_REGS);
     constexpr uint32_t TILE  = ROWS * ITERS;
     const TileSchedule ts    = make_schedule(ncur, n_f, nseg, TILE, kRowGroup, kRun);
-    k_merge_v4<NB, POLY, ITERS><<<ts.nblocks, kBlock, 0, s>>>(in, out, coords, cf, ncur, nprev,
-                                                              ts);
+    k_merge_v4<NB, POLY, ITERS, TI, TO><<<ts.nblocks, kBlock, 0, s>>>(in, out, coords, cf, ncur,
+                                                                      nprev, ts, scale);
 }
 
-template <bool POLY>
-void dispatch(const float* in, float* out, const FFACoord* coords, const FFACoordFreq* cf,
+template <bool POLY, typename TI, typename TO>
+void dispatch(const TI* in, TO* out, const FFACoord* coords, const FFACoordFreq* cf,
               uint64_t ncur, uint64_t nprev, uint64_t nseg, uint64_t n_f, uint32_t nbins,
-              hipStream_t s) {
+              float scale, hipStream_t s) {
     switch (nbins) {
-    case 4: return launch_v4<4, POLY>(in, out, coords, cf, ncur, nprev, nseg, n_f, s);
-    case 8: return launch_v4<8, POLY>(in, out, coords, cf, ncur, nprev, nseg, n_f, s);
-    case 16: return launch_v4<16, POLY>(in, out, coords, cf, ncur, nprev, nseg, n_f, s);
-    case 32: return launch_v4<32, POLY>(in, out, coords, cf, ncur, nprev, nseg, n_f, s);
-    case 64: return launch_v4<64, POLY>(in, out, coords, cf, ncur, nprev, nseg, n_f, s);
-    case 128: return launch_v4<128, POLY>(in, out, coords, cf, ncur, nprev, nseg, n_f, s);
-    case 256: return launch_v4<256, POLY>(in, out, coords, cf, ncur, nprev, nseg, n_f, s);
-    case 512: return launch_v4<512, POLY>(in, out, coords, cf, ncur, nprev, nseg, n_f, s);
-    case 1024: return launch_v4<1024, POLY>(in, out, coords, cf, ncur, nprev, nseg, n_f, s);
+    case 4: return launch_v4<4, POLY>(in, out, coords, cf, ncur, nprev, nseg, n_f, scale, s);
+    case 8: return launch_v4<8, POLY>(in, out, coords, cf, ncur, nprev, nseg, n_f, scale, s);
+    case 16: return launch_v4<16, POLY>(in, out, coords, cf, ncur, nprev, nseg, n_f, scale, s);
+    case 32: return launch_v4<32, POLY>(in, out, coords, cf, ncur, nprev, nseg, n_f, scale, s);
+    case 64: return launch_v4<64, POLY>(in, out, coords, cf, ncur, nprev, nseg, n_f, scale, s);
+    case 128: return launch_v4<128, POLY>(in, out, coords, cf, ncur, nprev, nseg, n_f, scale, s);
+    case 256: return launch_v4<256, POLY>(in, out, coords, cf, ncur, nprev, nseg, n_f, scale, s);
+    case 512: return launch_v4<512, POLY>(in, out, coords, cf, ncur, nprev, nseg, n_f, scale, s);
+    case 1024:
+        return launch_v4<1024, POLY>(in, out, coords, cf, ncur, nprev, nseg, n_f, scale, s);
     default: break;
     }
     const uint64_t total = nseg * ncur * nbins;
     const uint64_t want  = (total + kBlock - 1) / kBlock;
     const unsigned grid  = static_cast<unsigned>(want < 65536 * 8 ? want : 65536 * 8);
-    k_merge_scalar<POLY><<<grid, kBlock, 0, s>>>(in, out, coords, cf, ncur, nprev, nseg, nbins);
+    k_merge_scalar<POLY, TI, TO><<<grid, kBlock, 0, s>>>(in, out, coords, cf, ncur, nprev, nseg,
+                                                         nbins, scale);
+}
+
+template <typename TI, typename TO>
+void merge_typed(const void* in, void* out, const FFACoord* coords,
+                 const FFACoordFreq* coords_freq, uint64_t ncur, uint64_t nprev, uint64_t nseg,
+                 uint64_t n_f, uint32_t nbins, float scale, hipStream_t s) {
+    const auto* pi = static_cast<const TI*>(in);
+    auto* po       = static_cast<TO*>(out);
+    if (coords != nullptr) {
+        dispatch<true>(pi, po, coords, nullptr, ncur, nprev, nseg, n_f, nbins, scale, s);
+    } else {
+        dispatch<false>(pi, po, nullptr, coords_freq, ncur, nprev, nseg, n_f, nbins, scale, s);
+    }
 }
 
 }  // namespace
 
-void merge_time(const float* in, float* out, const FFACoord* coords,
+void merge_time(const void* in, void* out, const FFACoord* coords,
                 const FFACoordFreq* coords_freq, uint64_t ncur, uint64_t nprev, uint64_t nseg,
-                uint64_t n_f, uint32_t nbins, hipStream_t s) {
+                uint64_t n_f, uint32_t nbins, FoldStorage si, FoldStorage so, float scale,
+                hipStream_t s) {
     LOKI_REQUIRE(nbins >= 1, "merge: nbins must be positive");
     LOKI_REQUIRE((coords == nullptr) != (coords_freq == nullptr),
                  "merge: exactly one coordinate table");
+    LOKI_REQUIRE(si == so || so == FoldStorage::kF32, "merge: unsupported storage pair");
     if (ncur == 0 || nseg == 0) {
         return;
     }
-    if (coords != nullptr) {
-        dispatch<true>(in, out, coords, nullptr, ncur, nprev, nseg, n_f, nbins, s);
-    } else {
-        dispatch<false>(in, out, nullptr, coords_freq, ncur, nprev, nseg, n_f, nbins, s);
+    const auto args = std::make_tuple(in, out, coords, coords_freq, ncur, nprev, nseg, n_f, nbins,
+                                      scale, s);
+    if (si == FoldStorage::kF32) {
+        return std::apply(merge_typed<float, float>, args);
     }
+    if (si == FoldStorage::kF16) {
+        return so == FoldStorage::kF16 ? std::apply(merge_typed<_Float16, _Float16>, args)
+                                       : std::apply(merge_typed<_Float16, float>, args);
+    }
+    return so == FoldStorage::kBF16 ? std::apply(merge_typed<__bf16, __bf16>, args)
+                                    : std::apply(merge_typed<__bf16, float>, args);
 }
 
 }  // namespace loki_amd::kern

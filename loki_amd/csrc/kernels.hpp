@@ -9,6 +9,7 @@
 #include "common.hpp"
 #include "device.hpp"
 #include "plan.hpp"
+#include "storage.hpp"
 
 namespace loki_amd::kern {
 
@@ -50,15 +51,22 @@ struct BruteFoldLists {
 /// FP64 phase bins (psr_utils.hpp:53-63) -> lists; synchronises `s`.
 BruteFoldLists build_brute_fold_lists(const double* freqs_dev, uint32_t nfreqs, uint32_t seglen,
                                       uint32_t nbins, double tsamp, double t_ref, hipStream_t s);
-/// fold[nseg, nfreqs, 2, nbins] = ascending-order bucket sums of each segment.
+/// fold[nseg, nfreqs, 2, nbins] = ascending-order bucket sums of each segment
+/// (FP32), stored as `so`; unless so is FP32 each sum is multiplied by
+/// `scale` before rounding.
 void brute_fold_time(const float* ts_e, const float* ts_v, uint64_t nseg,
-                     const BruteFoldLists& lists, float* fold, hipStream_t s);
+                     const BruteFoldLists& lists, void* fold, FoldStorage so, float scale,
+                     hipStream_t s);
 
 /// Time-domain merge of one level (freq-only when coords_freq != nullptr);
 /// n_f: frequency grid points of the level (coordinates are rows of n_f).
-void merge_time(const float* in, float* out, const FFACoord* coords,
+/// si/so: storage of the parent/child level (so == si, or FP32 to return a
+/// reduced-precision tree to full precision); unless both are FP32 the FP32
+/// sum is multiplied by `scale` before rounding to so.
+void merge_time(const void* in, void* out, const FFACoord* coords,
                 const FFACoordFreq* coords_freq, uint64_t ncur, uint64_t nprev, uint64_t nseg,
-                uint64_t n_f, uint32_t nbins, hipStream_t s);
+                uint64_t n_f, uint32_t nbins, FoldStorage si, FoldStorage so, float scale,
+                hipStream_t s);
 
 /// Fourier brute fold: fold [nseg, nfreqs, 2, nbins/2+1] complex (float2) of
 /// exp(-2 pi i m f (k tsamp - t_ref)) sums; FP32 matrix-core DFT with FP64
@@ -95,9 +103,11 @@ WidthTable make_width_table(const SizeType* widths, SizeType nw, SizeType nbins)
 bool final_merge_score_supported(uint32_t nbins);
 /// exact: p = e * (1/sqrtf(v)) with correctly rounded operations (bit-identical to the CPU);
 /// otherwise p = e * rsq(v) with the hardware reciprocal square root.
-void final_merge_score(const float* in, uint64_t ncur, uint64_t nprev, const ResolveLevel& lv,
-                       const WidthTable& wt, float* scores, bool find_max, bool exact,
-                       hipStream_t s);
+/// si: storage of the parent level; unless FP32 the merged e and v are
+/// multiplied by `unscale` (exact power of two) before normalisation.
+void final_merge_score(const void* in, FoldStorage si, float unscale, uint64_t ncur,
+                       uint64_t nprev, const ResolveLevel& lv, const WidthTable& wt,
+                       float* scores, bool find_max, bool exact, hipStream_t s);
 
 /// snr_boxcar_3d[_max]: folds [nprof, 2, nbins] -> scores [nprof, nw] or [nprof].
 void snr_boxcar_3d(const float* folds, uint64_t nprof, uint32_t nbins, const WidthTable& wt,

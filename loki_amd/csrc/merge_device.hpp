@@ -5,8 +5,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "common.hpp"
+#include "storage.hpp"
 
 namespace loki_amd::kern {
 
@@ -127,6 +129,22 @@ __device__ __forceinline__ float4 rotated4(const float4* __restrict__ row, uint3
     const float4 a    = row[c0];
     const float4 b    = row[c1];
     return funnel4(a, b, e0 & 3U);
+}
+
+/// rotated4 for any storage type, widened to FP32: row points at the first
+/// chunk of one component row of NB stored elements.
+template <int NB, typename T>
+__device__ __forceinline__ float4 rotated4_st(const chunk4_t<T>* __restrict__ row, uint32_t i4,
+                                              uint32_t S) {
+    if constexpr (std::is_same_v<T, float>) {
+        return rotated4<NB>(row, i4, S);
+    } else {
+        constexpr uint32_t C4 = NB / 4;
+        const uint32_t e0 = (4U * i4 + NB - S) & (NB - 1);
+        const uint32_t c0 = e0 >> 2;
+        const uint32_t c1 = (c0 + 1) & (C4 - 1);
+        return widen4<T>(funnel4_16(row[c0], row[c1], e0 & 3U));
+    }
 }
 
 }  // namespace loki_amd::kern

@@ -296,6 +296,19 @@ public:
     void set_timing(bool on) { check(loki_ffa_set_timing(m_h.get(), on ? 1 : 0)); }
     void set_fused(bool on) { check(loki_ffa_set_fused(m_h.get(), on ? 1 : 0)); }
     void set_exact_norm(bool on) { check(loki_ffa_set_exact_norm(m_h.get(), on ? 1 : 0)); }
+    void set_fold_storage(const std::string& st) {
+        int k = -1;
+        if (st == "f32" || st == "float32") {
+            k = LOKI_STORAGE_F32;
+        } else if (st == "f16" || st == "float16") {
+            k = LOKI_STORAGE_F16;
+        } else if (st == "bf16" || st == "bfloat16") {
+            k = LOKI_STORAGE_BF16;
+        } else {
+            throw std::runtime_error("fold_storage must be 'f32', 'f16' or 'bf16'");
+        }
+        check(loki_ffa_set_fold_storage(m_h.get(), k));
+    }
     py::list timings() {
         size_t n = 0;
         check(loki_ffa_timings(m_h.get(), nullptr, 0, &n));
@@ -624,6 +637,7 @@ PYBIND11_MODULE(_loki_amd, m) {
         .def("set_timing", &FFA::set_timing, py::arg("enable"))
         .def("set_fused", &FFA::set_fused, py::arg("enable"))
         .def("set_exact_norm", &FFA::set_exact_norm, py::arg("exact"))
+        .def("set_fold_storage", &FFA::set_fold_storage, py::arg("storage"))
         .def("timings", &FFA::timings);
     py::class_<FFAComplex>(m_ffa, "FFACOMPLEXCUDA")
         .def(py::init<const Config&, int>(), py::arg("cfg"), py::arg("device_id") = 0)
@@ -646,9 +660,10 @@ PYBIND11_MODULE(_loki_amd, m) {
     m_ffa.def(
         "compute_ffa_time_cuda",
         [](const PyArrayT<float>& ts_e, const PyArrayT<float>& ts_v, const Config& cfg,
-           int device_id, bool /*quiet*/) {
+           int device_id, bool /*quiet*/, const std::string& fold_storage) {
             check_nsamps(ts_e, ts_v, cfg);
             FFA ffa(cfg, device_id, LOKI_FOLD_TIME);
+            ffa.set_fold_storage(fold_storage);
             const size_t buf = ffa.plan().info().buffer_size;
             PyArrayT<float> fold(static_cast<py::ssize_t>(buf));
             ffa.execute(ts_e, ts_v, fold);
@@ -656,7 +671,7 @@ PYBIND11_MODULE(_loki_amd, m) {
             return py::make_tuple(fold, PlanTime(cfg));
         },
         py::arg("ts_e"), py::arg("ts_v"), py::arg("cfg"), py::arg("device_id") = 0,
-        py::arg("quiet") = false);
+        py::arg("quiet") = false, py::arg("fold_storage") = "f32");
     m_ffa.def(
         "compute_ffa_fourier_cuda",
         [](const PyArrayT<float>& ts_e, const PyArrayT<float>& ts_v, const Config& cfg,
@@ -688,17 +703,20 @@ PYBIND11_MODULE(_loki_amd, m) {
     m_ffa.def(
         "compute_ffa_scores_cuda",
         [](const PyArrayT<float>& ts_e, const PyArrayT<float>& ts_v, const Config& cfg,
-           int device_id, bool /*quiet*/, std::optional<PyArrayT<size_t>> widths) {
+           int device_id, bool /*quiet*/, std::optional<PyArrayT<size_t>> widths,
+           const std::string& fold_storage) {
             check_nsamps(ts_e, ts_v, cfg);
             // ffa_cuda.cu:676-680: Fourier FFA returned to the time domain, or the time FFA
             FFA ffa(cfg, device_id,
                     cfg.params().use_fourier != 0 ? LOKI_FOLD_FOURIER : LOKI_FOLD_TIME);
+            ffa.set_fold_storage(fold_storage);
             PyArrayT<size_t> w = widths.has_value() ? *widths : PyArrayT<size_t>(cfg.score_widths());
             auto scores = ffa.scores_host(ts_e, ts_v, w, false);
             return py::make_tuple(scores, PlanTime(cfg));
         },
         py::arg("ts_e"), py::arg("ts_v"), py::arg("cfg"), py::arg("device_id") = 0,
-        py::arg("quiet") = false, py::arg("widths") = std::nullopt);
+        py::arg("quiet") = false, py::arg("widths") = std::nullopt,
+        py::arg("fold_storage") = "f32");
 
     m.def("device_count", [] {
         int n = 0;

@@ -1,0 +1,83 @@
+// Fold storage precision.  Folds of levels 0..L-2 may be stored as FP16 or
+// BF16; every add is done in FP32 and each stored value is rounded once
+// (round to nearest even).  Level l is stored at the exact power-of-two scale
+// 2^-(k0 + l): a merge is then (tail + head) * 0.5, which keeps the stored
+// magnitudes of order one (no FP16 overflow of the growing sums) and is
+// exact, so unscaling the last level by 2^(k0 + L - 2) restores the true
+// values without rounding.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace loki_amd {
+
+enum class FoldStorage : int { kF32 = 0, kF16 = 1, kBF16 = 2 };
+
+inline uint32_t storage_bytes(FoldStorage s) { return s == FoldStorage::kF32 ? 4U : 2U; }
+
+namespace kern {
+
+/// Raw chunk of 4 consecutive stored elements.
+template <typename T> struct Chunk4;
+template <> struct Chunk4<float> { using type = float4; };
+template <> struct Chunk4<_Float16> { using type = uint2; };
+template <> struct Chunk4<__bf16> { using type = uint2; };
+template <typename T> using chunk4_t = typename Chunk4<T>::type;
+
+__device__ __forceinline__ float f16_lo(uint32_t w) {
+    return static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(w & 0xFFFFU)));
+}
+__device__ __forceinline__ float f16_hi(uint32_t w) {
+    return static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(w >> 16)));
+}
+__device__ __forceinline__ uint32_t f16_bits(float x) {
+    return static_cast<uint32_t>(__builtin_bit_cast(uint16_t, static_cast<_Float16>(x)));
+}
+__device__ __forceinline__ float bf16_lo(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float bf16_hi(uint32_t w) { return __uint_as_float(w & 0xFFFF0000U); }
+__device__ __forceinline__ uint32_t bf16_bits(float x) {
+    return static_cast<uint32_t>(__builtin_bit_cast(uint16_t, static_cast<__bf16>(x)));
+}
+
+/// 4 stored elements -> FP32 (exact).
+template <typename T> __device__ __forceinline__ float4 widen4(const chunk4_t<T> c);
+template <> __device__ __forceinline__ float4 widen4<float>(const float4 c) { return c; }
+template <> __device__ __forceinline__ float4 widen4<_Float16>(const uint2 c) {
+    return make_float4(f16_lo(c.x), f16_hi(c.x), f16_lo(c.y), f16_hi(c.y));
+}
+template <> __device__ __forceinline__ float4 widen4<__bf16>(const uint2 c) {
+    return make_float4(bf16_lo(c.x), bf16_hi(c.x), bf16_lo(c.y), bf16_hi(c.y));
+}
+
+/// FP32 -> 4 stored elements (round to nearest even).
+template <typename T> __device__ __forceinline__ chunk4_t<T> narrow4(const float4 v);
+template <> __device__ __forceinline__ float4 narrow4<float>(const float4 v) { return v; }
+template <> __device__ __forceinline__ uint2 narrow4<_Float16>(const float4 v) {
+    return make_uint2(f16_bits(v.x) | (f16_bits(v.y) << 16), f16_bits(v.z) | (f16_bits(v.w) << 16));
+}
+template <> __device__ __forceinline__ uint2 narrow4<__bf16>(const float4 v) {
+    return make_uint2(bf16_bits(v.x) | (bf16_bits(v.y) << 16),
+                      bf16_bits(v.z) | (bf16_bits(v.w) << 16));
+}
+
+/// One stored element <-> FP32.
+template <typename T> __device__ __forceinline__ float widen1(T x) {
+    return static_cast<float>(x);
+}
+template <typename T> __device__ __forceinline__ T narrow1(float x) { return static_cast<T>(x); }
+
+/// Elements r..r+3 of the 8 16-bit elements (a, b) (little-endian halves):
+/// one select level on 32-bit words, then a 0- or 16-bit funnel shift.
+__device__ __forceinline__ uint2 funnel4_16(const uint2 a, const uint2 b, uint32_t r) {
+    const bool k      = (r & 2U) != 0;
+    const uint32_t sh = (r & 1U) * 16U;
+    const uint32_t w0 = k ? a.y : a.x;
+    const uint32_t w1 = k ? b.x : a.y;
+    const uint32_t w2 = k ? b.y : b.x;
+    return make_uint2(__builtin_amdgcn_alignbit(w1, w0, sh), __builtin_amdgcn_alignbit(w2, w1, sh));
+}
+
+}  // namespace kern
+}  // namespace loki_amd

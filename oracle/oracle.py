@@ -170,9 +170,44 @@ def snr_boxcar_3d_f64(folds, widths, nbins, nthreads=None):
 # ---------------------------------------------------------------------------
 # drivers (ffa.cpp:242-360, 405-466)
 # ---------------------------------------------------------------------------
+def round_storage(x, storage: str):
+    """Round FP32 values to the stored precision (round to nearest even) and return them as
+    the exactly equal FP32 values: 'f32' (unchanged), 'f16' (numpy's IEEE binary16
+    conversion) or 'bf16' (the upper 16 bits of the FP32 pattern, rounded to nearest even)."""
+    x = np.ascontiguousarray(x, np.float32)
+    if storage == "f32":
+        return x
+    if storage == "f16":
+        with np.errstate(over="ignore"):  # beyond 65504 rounds to inf, as on the GPU
+            return x.astype(np.float16).astype(np.float32)
+    if storage == "bf16":
+        u = x.view(np.uint32).astype(np.uint64)
+        u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+        return (u & 0xFFFFFFFF).astype(np.uint32).view(np.float32)
+    raise ValueError(f"unknown storage {storage!r}")
+
+
+def storage_k0(cfg: SearchConfig, plan: Plan) -> int:
+    """Level-0 scale exponent of reduced-precision storage: floor(log2(seglen // nbins))."""
+    per, k0 = plan.segment_lens[0] // cfg.nbins, 0
+    while per > 1:
+        per >>= 1
+        k0 += 1
+    return k0
+
+
 def compute_ffa(ts_e, ts_v, cfg: SearchConfig, fourier: bool = False, nthreads=None,
-                f64: bool = False, return_levels: bool = False):
-    """Run brute fold + all merge levels; returns (final fold, plan[, levels])."""
+                f64: bool = False, return_levels: bool = False, storage: str = "f32"):
+    """Run brute fold + all merge levels; returns (final fold, plan[, levels]).
+
+    storage 'f16' / 'bf16' (time domain): levels 0..L-2 are stored at that precision, each
+    value rounded once from its FP32 sum at the exact scale 2^-(k0 + l) (merge = (t + h) * 0.5);
+    the last level is returned in FP32 at its true scale, (t + h) * 2^(k0 + L - 2).
+    """
+    if storage != "f32":
+        if fourier or f64:
+            raise ValueError("reduced-precision storage applies to the FP32 time-domain FFA")
+        return _compute_ffa_storage(ts_e, ts_v, cfg, nthreads, storage)
     plan = Plan(cfg, fourier=fourier)
     coords = plan.resolve_coordinates()
     freqs = plan.brute_fold_freqs()
@@ -198,13 +233,37 @@ def compute_ffa(ts_e, ts_v, cfg: SearchConfig, fourier: bool = False, nthreads=N
     return fold, plan
 
 
-def compute_ffa_scores(ts_e, ts_v, cfg: SearchConfig, widths=None, nthreads=None):
+def _compute_ffa_storage(ts_e, ts_v, cfg: SearchConfig, nthreads, storage: str):
+    plan = Plan(cfg)
+    coords = plan.resolve_coordinates()
+    freqs = plan.brute_fold_freqs()
+    t_ref = plan.brute_fold_tref()
+    k0 = storage_k0(cfg, plan)
+    fold = brute_fold_time(ts_e, ts_v, freqs, plan.segment_lens[0], cfg.nbins, cfg.tsamp, t_ref,
+                           nthreads)
+    last = plan.n_levels - 1
+    if last == 0:
+        return fold, plan
+    fold = round_storage(fold * np.float32(2.0 ** -k0), storage)
+    for lvl in range(1, plan.n_levels):
+        out = ffa_iter(fold, coords[lvl], plan.ncoords[lvl], plan.ncoords[lvl - 1],
+                       plan.nsegments[lvl], cfg.nbins, nthreads)
+        if lvl < last:
+            fold = round_storage(out * np.float32(0.5), storage)
+        else:
+            fold = out * np.float32(2.0 ** (k0 + lvl - 1))
+    return fold, plan
+
+
+def compute_ffa_scores(ts_e, ts_v, cfg: SearchConfig, widths=None, nthreads=None,
+                       storage: str = "f32"):
     """ffa.cpp:444-466: FFA (time, or Fourier then irfft) + snr_boxcar_3d."""
     if cfg.use_fourier:
         fold_c, plan = compute_ffa(ts_e, ts_v, cfg, fourier=True, nthreads=nthreads)
         fold = irfft(fold_c, cfg.nbins, nthreads)
     else:
-        fold, plan = compute_ffa(ts_e, ts_v, cfg, fourier=False, nthreads=nthreads)
+        fold, plan = compute_ffa(ts_e, ts_v, cfg, fourier=False, nthreads=nthreads,
+                                 storage=storage)
     if plan.nsegments[-1] != 1:
         raise ValueError("compute_ffa_scores: nsegments must be 1 for scores")
     w = cfg.score_widths if widths is None else np.asarray(widths, np.uint64)
@@ -214,5 +273,6 @@ def compute_ffa_scores(ts_e, ts_v, cfg: SearchConfig, widths=None, nthreads=None
 __all__ = [
     "FFA_COORD_DTYPE", "FFA_COORD_FREQ_DTYPE", "Plan", "SearchConfig", "brute_fold_fourier",
     "brute_fold_time", "build", "circular_prefix_sum", "compute_ffa", "compute_ffa_scores",
-    "diff_max", "ffa_iter", "irfft", "lib", "rfft", "snr_boxcar_3d", "snr_boxcar_3d_f64",
+    "diff_max", "ffa_iter", "irfft", "lib", "rfft", "round_storage", "snr_boxcar_3d",
+    "snr_boxcar_3d_f64", "storage_k0",
 ]

@@ -259,3 +259,70 @@ def test_errors_raise_runtime_error():
         libculoki.ffa.FFACUDA(cfg, device_id=99)
     with pytest.raises(RuntimeError, match="widths"):
         libculoki.scores.snr_boxcar_3d(np.ones((4, 2, 8), np.float32), np.array([8], np.uint64))
+
+
+# ---------------------------------------------------------------------------
+# Reduced-precision fold storage: bit-exact against the oracle's emulation
+# (same FP32 sums, each stored value rounded once to FP16 / BF16 at the scale
+# 2^-(k0 + l)); the S/N error against FP64 is stated in test_storage_snr_error.
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+@pytest.mark.parametrize("name", ["freq16", "acc17w", "acc20", "jerk17w", "freq18_nb100",
+                                  "acc_eta2_bseg"])
+def test_reduced_storage_fold_bit_exact(oracle, name, storage):
+    from loki_amd import libculoki
+
+    cfg, ocfg = _cfgs(name)
+    ts_e, ts_v = _series(cfg.nsamps, seed=31)
+    got, _ = libculoki.ffa.compute_ffa_time_cuda(ts_e, ts_v, cfg, fold_storage=storage)
+    want, _ = oracle.compute_ffa(ts_e, ts_v, ocfg, storage=storage)
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+@pytest.mark.parametrize("name", ["freq16", "acc17w", "acc20", "jerk17w", "jerk20"])
+@pytest.mark.parametrize("fused", [True, False])
+def test_reduced_storage_scores_bit_exact(oracle, name, storage, fused):
+    from loki_amd import libculoki
+
+    cfg, ocfg = _cfgs(name)
+    ts_e, ts_v = _series(cfg.nsamps, seed=32)
+    widths = np.array([w for w in (1, 2, 3, 4, 6, 8, 12, 16, 24) if w < cfg.nbins], np.uint64)
+    ffa = libculoki.ffa.FFACUDA(cfg, device_id=0)
+    ffa.set_fold_storage(storage)
+    ffa.set_fused(fused)
+    got = ffa.scores(ts_e, ts_v, widths)
+    want, _ = oracle.compute_ffa_scores(ts_e, ts_v, ocfg, widths=widths, storage=storage)
+    np.testing.assert_array_equal(got, want)
+
+
+# Stated S/N error of each storage precision against the FP64 oracle (FP64 folds and FP64
+# scoring), on series with an injected pulsar (S/N 15 at the best width).  FP32 storage is
+# ~1e-6 (FP32 rounding only); FP16 keeps 11 significand bits, so a strong candidate carries
+# about S/N * 2^-12 per rounding (~5e-3 at S/N 15); BF16 (8 bits) about 8x more.
+STORAGE_SNR_BOUND = {"f32": 2e-5, "f16": 1e-2, "bf16": 8e-2}
+
+
+@pytest.mark.parametrize("name", ["acc20", "jerk20"])
+def test_storage_snr_error_vs_fp64(oracle, name):
+    from loki_amd import libculoki
+    from loki_amd.synth import make_series
+
+    cfg, ocfg = _cfgs(name)
+    nbins = cfg.nbins
+    widths = np.array([w for w in (1, 2, 3, 4, 6, 8, 12, 16, 24, 32) if w < nbins], np.uint64)
+    lims = CASES[name][2]
+    ts_e, ts_v, _ = make_series(cfg.nsamps, TSAMP, 77, f0=lims[-1][0] + 0.37, fwhm=0.05,
+                                snr=15.0, nbins=nbins, widths=widths)
+    fold64, _ = oracle.compute_ffa(ts_e, ts_v, ocfg, f64=True)
+    snr64 = oracle.snr_boxcar_3d_f64(fold64, widths, nbins).reshape(-1)
+    ffa = libculoki.ffa.FFACUDA(cfg, device_id=0)
+    for storage, bound in STORAGE_SNR_BOUND.items():
+        ffa.set_fold_storage(storage)
+        got = ffa.scores(ts_e, ts_v, widths).astype(np.float64)
+        d = np.abs(got - snr64)
+        top = np.argsort(snr64)[-100:]
+        print(f"{name} {storage}: max |dS/N| {d.max():.2e}, p99 {np.percentile(d, 99):.2e}, "
+              f"median {np.median(d):.2e}; top-100 candidates max {d[top].max():.2e} "
+              f"(S/N up to {snr64.max():.1f})")
+        assert d.max() <= bound

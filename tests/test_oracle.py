@@ -466,3 +466,40 @@ def test_f32_and_f64_oracle_folds_agree(oracle):
     f32, _ = oracle.compute_ffa(ts_e, ts_v, cfg)
     f64, _ = oracle.compute_ffa(ts_e, ts_v, cfg, f64=True)
     np.testing.assert_allclose(f32, f64, rtol=1e-4, atol=1e-3)
+
+
+# ---------------------------------------------------------------------------
+# Reduced-precision storage emulation
+# ---------------------------------------------------------------------------
+def test_round_storage_known_values(oracle):
+    """Round to nearest even at 11 (FP16) and 8 (BF16) significand bits."""
+    x = np.array([1.0, 1.0 + 2.0**-11, 1.0 + 3 * 2.0**-11, 1.0 + 2.0**-8, 1.0 + 3 * 2.0**-8,
+                  65504.0, 65520.0, -2.5, 2.0**-24, 3.0e38], np.float32)
+    f16 = oracle.round_storage(x, "f16")
+    np.testing.assert_array_equal(
+        f16[:8], np.array([1.0, 1.0, 1.0 + 2 * 2.0**-10, 1.0 + 2.0**-8, 1.0 + 3 * 2.0**-8,
+                           65504.0, np.inf, -2.5], np.float32))
+    assert f16[8] == 2.0**-24          # smallest FP16 subnormal
+    bf = oracle.round_storage(x, "bf16")
+    np.testing.assert_array_equal(
+        bf[:6], np.array([1.0, 1.0, 1.0, 1.0, 1.0 + 4 * 2.0**-8, 65536.0], np.float32))
+    assert bf[7] == -2.5 and np.isfinite(bf[9])
+
+
+def test_reduced_storage_merge_is_scaled_sum(oracle):
+    """Level l of the emulated tree equals round((t + h) * 0.5) of the level below, and the
+    returned last level equals the unrounded sum times 2^(k0 + L - 2)."""
+    from oracle.plan import Plan, SearchConfig
+
+    cfg = SearchConfig(2**14, 64e-6, 16, 1.0, [(8.0, 12.0)], use_fourier=False)
+    rng = np.random.default_rng(5)
+    ts_e = rng.standard_normal(2**14, dtype=np.float32)
+    ts_v = np.ones(2**14, np.float32)
+    plan = Plan(cfg)
+    k0 = oracle.storage_k0(cfg, plan)
+    assert k0 == int(np.log2(plan.segment_lens[0] // cfg.nbins))
+    f16, _ = oracle.compute_ffa(ts_e, ts_v, cfg, storage="f16")
+    f32, _ = oracle.compute_ffa(ts_e, ts_v, cfg)
+    # FP16 storage perturbs the FP32 result by about 2^-12 relative per level
+    assert np.max(np.abs(f16 - f32)) <= 2.0**-12 * plan.n_levels * np.max(np.abs(f32))
+    assert not np.array_equal(f16, f32)
