@@ -34,6 +34,7 @@ extern "C" {
 typedef struct loki_config_s* loki_config_t;
 typedef struct loki_plan_s* loki_plan_t;
 typedef struct loki_ffa_s* loki_ffa_t;
+typedef struct loki_batch_s* loki_batch_t;
 
 /* PulsarSearchConfig constructor arguments (configs.hpp:25-47).
  * param_limits: nparams (min, max) pairs, order (..., jerk, accel, freq).
@@ -218,6 +219,49 @@ int loki_ffa_set_exact_norm(loki_ffa_t ffa, int exact);
 int loki_ffa_set_fold_storage(loki_ffa_t ffa, int storage);
 /* Waits for the recorded events; *n receives the number of timed launches. */
 int loki_ffa_timings(loki_ffa_t ffa, loki_kernel_timing* out, size_t cap, size_t* n);
+
+/* --- Batched multi-series FFA ------------------------------------------
+ * No reference counterpart: the reference searches series one FFACUDA object at a time, with
+ * host synchronisations per level (ffa_freq_sweep_cuda.cu:122-136, kernels_cuda.cu:1081-1082).
+ * A batch holds many independent frequency-only time-domain searches (uneven lengths, grids
+ * and nbins in {8, 16, 32, 64, 128}) and advances all of them with one grouped launch per
+ * stage and nbins class, captured once in a HIP graph.  Each series' scores are bit-identical
+ * to loki_ffa_scores_* on its own config.  Inputs are copied into the batch's own device slots
+ * (loki_batch_series) before loki_batch_execute. */
+typedef struct {
+    uint32_t merge_group; /* child rows of a merge scheduled back to back */
+    uint32_t merge_run;   /* frequency tiles per sequential run */
+    uint32_t final_group; /* the same for the fused last levels */
+    uint32_t final_run;
+} loki_batch_schedule;
+/* widths: boxcar widths for every series (those >= nbins dropped); NULL/0: each config's own
+ * scoring widths. */
+int loki_batch_create(const loki_config_t* cfgs, size_t nseries, const size_t* widths,
+                      size_t nwidths, int device_id, loki_batch_t* out);
+void loki_batch_destroy(loki_batch_t batch);
+int loki_batch_info(loki_batch_t batch, size_t* nseries, size_t* total_trials,
+                    size_t* device_bytes, size_t* launches);
+int loki_batch_plan(loki_batch_t batch, size_t i, loki_plan_t* out_plan_view); /* borrowed */
+/* Device slots of series i: ts_e / ts_v (nsamps floats each, to be filled by the caller) and
+ * scores [ncoords_final, nwidths]. */
+int loki_batch_series(loki_batch_t batch, size_t i, float** ts_e, float** ts_v, float** scores,
+                      size_t* scores_len, size_t* nwidths);
+int loki_batch_widths(loki_batch_t batch, size_t i, size_t* out, size_t cap, size_t* n);
+/* Host copies in and out of series i's slots (synchronous; scores wait for the device). */
+int loki_batch_set_input_host(loki_batch_t batch, size_t i, const float* ts_e,
+                              const float* ts_v, size_t nsamps);
+int loki_batch_scores_host(loki_batch_t batch, size_t i, float* out, size_t len);
+/* Device-to-device copy of resident series into slot i, asynchronous on `stream`. */
+int loki_batch_set_input_device(loki_batch_t batch, size_t i, const float* ts_e,
+                                const float* ts_v, size_t nsamps, void* stream);
+/* Runs every series on `stream` (asynchronous): replays the captured graph (default) or
+ * enqueues the grouped launches directly. */
+int loki_batch_execute(loki_batch_t batch, void* stream);
+int loki_batch_set_graph(loki_batch_t batch, int enable);
+int loki_batch_set_schedule(loki_batch_t batch, const loki_batch_schedule* schedule);
+/* Times candidate tile schedules on the batch itself (synchronous) and keeps the fastest. */
+int loki_batch_autotune(loki_batch_t batch, void* stream, int repeats,
+                        loki_batch_schedule* chosen);
 
 /* --- Brute fold (fold.hpp:78-133 BruteFoldCUDA) ------------------------ */
 /* compute_brute_fold_cuda<float> (fold_cuda.cu:260-277): fold has

@@ -26,7 +26,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_SOURCES = ["k_resolve.hip", "k_brutefold.hip", "k_merge.hip", "k_score.hip", "k_final.hip",
                "k_fourier.hip",
-               "ffa.hip", "capi.hip"]
+               "ffa.hip", "batch.hip", "capi.hip"]
 HOST_SOURCES = ["config.cpp", "plan.cpp"]
 
 # -ffp-contract=off: no FMA contraction anywhere (bit-exact FP64 shift tables

@@ -1,0 +1,423 @@
+// Batched multi-series FFA: arenas, task tables, grouped launches, graph
+// capture and the schedule autotuner.
+#include "batch.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <string>
+
+namespace loki_amd {
+namespace {
+
+template <typename T>
+SizeType append(std::vector<uint8_t>& blob, const std::vector<T>& v) {
+    const SizeType off = (blob.size() + 255) / 256 * 256;  // 256-byte aligned tables
+    blob.resize(off + v.size() * sizeof(T));
+    if (!v.empty()) {
+        std::memcpy(blob.data() + off, v.data(), v.size() * sizeof(T));
+    }
+    return off;
+}
+
+SizeType ceil_div(SizeType a, SizeType b) { return (a + b - 1) / b; }
+
+}  // namespace
+
+FFABatch::FFABatch(const std::vector<SearchConfig>& cfgs, const std::vector<SizeType>& widths,
+                   int device_id)
+    : m_device(device_id) {
+    LOKI_REQUIRE(!cfgs.empty(), "FFABatch: no series");
+    DeviceGuard g(device_id);
+    SizeType in_total = 0, fold_total = 0, coord_total = 0, score_total = 0;
+    m_series.reserve(cfgs.size());
+    for (const auto& cfg : cfgs) {
+        Series sr{FFAPlan(cfg, FoldKind::kTime), {}, {}, 0, {0, 0}, 0, 0, 0};
+        const auto& pl = sr.plan;
+        const SizeType nb = cfg.get_nbins();
+        LOKI_REQUIRE(cfg.get_nparams() == 1, "FFABatch: frequency-only searches only");
+        LOKI_REQUIRE(nb == 8 || nb == 16 || nb == 32 || nb == 64 || nb == 128,
+                     "FFABatch: nbins must be 8, 16, 32, 64 or 128");
+        sr.nlevels = pl.n_levels();
+        LOKI_REQUIRE(sr.nlevels >= 2, "FFABatch: levels must be greater than or equal to 2");
+        LOKI_REQUIRE(pl.nsegments().back() == 1, "compute_ffa_scores: nsegments must be 1 for scores");
+        if (widths.empty()) {
+            sr.widths = cfg.get_scoring_widths();
+        } else {
+            for (SizeType w : widths) {
+                if (w >= 1 && w < nb) {
+                    sr.widths.push_back(w);
+                }
+            }
+        }
+        LOKI_REQUIRE(!sr.widths.empty(), "FFABatch: no boxcar width below nbins");
+        for (SizeType l = 1; l < sr.nlevels; ++l) {
+            LOKI_REQUIRE(pl.ncoords()[l - 1] < (1ULL << 32), "FFABatch: grid exceeds 2^32");
+        }
+        SizeType buf = 0;
+        for (SizeType l = 0; l + 1 < sr.nlevels; ++l) {
+            buf = std::max(buf, pl.level_size(l));
+        }
+        sr.in_off      = in_total;
+        in_total      += 2 * ((cfg.get_nsamps() + 63) / 64 * 64);
+        sr.buf_off[0]  = fold_total;
+        sr.buf_off[1]  = fold_total + (buf + 63) / 64 * 64;
+        fold_total    += 2 * ((buf + 63) / 64 * 64);
+        sr.coord_off   = coord_total;
+        coord_total   += pl.coord_size();
+        sr.score_off   = score_total;
+        score_total   += (pl.ncoords().back() * sr.widths.size() + 63) / 64 * 64;
+        m_series.push_back(std::move(sr));
+    }
+    m_in.reserve(in_total * sizeof(float));
+    m_fold.reserve(fold_total * sizeof(float));
+    m_coords.reserve(std::max<SizeType>(coord_total, 1) * sizeof(FFACoordFreq));
+    m_scores.reserve(score_total * sizeof(float));
+    // brute-fold bucket lists (FP64 phases), once per series
+    hipStream_t s = nullptr;
+    for (auto& sr : m_series) {
+        const auto freqs   = sr.plan.compute_param_grid(0).back();
+        const auto& cfg    = sr.plan.config();
+        const SizeType nfr = freqs.size();
+        LOKI_REQUIRE(sr.plan.segment_lens()[0] < (1ULL << 32) && nfr < (1ULL << 32),
+                     "FFABatch: brute fold too large");
+        m_freqs.reserve(nfr * sizeof(double));
+        hip_check(hipMemcpy(m_freqs.get(), freqs.data(), nfr * sizeof(double),
+                            hipMemcpyHostToDevice),
+                  "copy brute-fold frequencies");
+        sr.lists = kern::build_brute_fold_lists(
+            m_freqs.as<double>(), static_cast<uint32_t>(nfr),
+            static_cast<uint32_t>(sr.plan.segment_lens()[0]), static_cast<uint32_t>(cfg.get_nbins()),
+            cfg.get_tsamp(), 0.0, s);
+    }
+    build_tasks();
+}
+
+FFABatch::~FFABatch() { drop_graph(); }
+
+float* FFABatch::ts_e(SizeType i) const { return m_in.as<float>() + m_series.at(i).in_off; }
+
+float* FFABatch::ts_v(SizeType i) const {
+    const auto& sr = m_series.at(i);
+    return m_in.as<float>() + sr.in_off + (sr.plan.config().get_nsamps() + 63) / 64 * 64;
+}
+
+float* FFABatch::scores(SizeType i) const {
+    return m_scores.as<float>() + m_series.at(i).score_off;
+}
+
+SizeType FFABatch::scores_len(SizeType i) const {
+    const auto& sr = m_series.at(i);
+    return sr.plan.ncoords().back() * sr.widths.size();
+}
+
+SizeType FFABatch::total_trials() const noexcept {
+    SizeType n = 0;
+    for (const auto& sr : m_series) {
+        n += sr.plan.ncoords().back();
+    }
+    return n;
+}
+
+SizeType FFABatch::device_bytes() const noexcept {
+    return m_in.bytes() + m_fold.bytes() + m_coords.bytes() + m_scores.bytes() +
+           m_tasks.bytes();
+}
+
+void FFABatch::set_schedule(const BatchSchedule& sc) {
+    LOKI_REQUIRE(sc.merge_group >= 1 && sc.merge_run >= 1 && sc.final_group >= 1 &&
+                     sc.final_run >= 1,
+                 "FFABatch: schedule knobs must be positive");
+    m_sched = sc;
+    DeviceGuard g(m_device);
+    build_tasks();
+}
+
+void FFABatch::set_graph(bool on) {
+    m_use_graph = on;
+    if (!on) {
+        drop_graph();
+    }
+}
+
+void FFABatch::drop_graph() {
+    if (m_exec != nullptr) {
+        (void)hipGraphExecDestroy(m_exec);
+        m_exec = nullptr;
+    }
+    if (m_graph != nullptr) {
+        (void)hipGraphDestroy(m_graph);
+        m_graph = nullptr;
+    }
+}
+
+void FFABatch::build_tasks() {
+    drop_graph();
+    m_launches.clear();
+    std::vector<uint8_t> blob;
+    float* fold        = m_fold.as<float>();
+    auto* coords       = m_coords.as<FFACoordFreq>();
+    auto buf           = [&](const Series& sr, SizeType level) {
+        return fold + sr.buf_off[level % 2];
+    };
+
+    // 1. shift tables of levels 1..L-2 (the last level resolves in-kernel)
+    {
+        std::vector<kern::ResolveTask> tasks;
+        uint32_t block0 = 0;
+        for (const auto& sr : m_series) {
+            for (SizeType l = 1; l + 1 < sr.nlevels; ++l) {
+                const SizeType n = sr.plan.ncoords()[l];
+                tasks.push_back({resolve_level(sr.plan, l),
+                                 coords + sr.coord_off + sr.plan.ncoords_offsets64()[l], n,
+                                 block0});
+                block0 += static_cast<uint32_t>(ceil_div(n, 256));
+            }
+        }
+        if (!tasks.empty()) {
+            m_launches.push_back({Stage::kResolve, 0, false, static_cast<uint32_t>(tasks.size()),
+                                  block0, append(blob, tasks), 0});
+        }
+    }
+    // 2. brute folds (LDS-staged segments and long segments in separate launches)
+    for (bool lds : {true, false}) {
+        std::vector<kern::FoldTask> tasks;
+        uint64_t block0 = 0;
+        for (const auto& sr : m_series) {
+            const auto& L = sr.lists;
+            if (kern::brute_fold_uses_lds(L.seglen) != lds) {
+                continue;
+            }
+            const uint32_t bps = kern::brute_fold_blocks_per_segment(L.nfreqs, L.nbins);
+            kern::FoldTask t{};
+            t.ts_e   = ts_e(static_cast<SizeType>(&sr - m_series.data()));
+            t.ts_v   = ts_v(static_cast<SizeType>(&sr - m_series.data()));
+            t.list   = L.list.as<uint32_t>();
+            t.base   = L.base.as<uint64_t>();
+            t.counts = L.counts.as<uint32_t>();
+            t.fold   = buf(sr, 0);
+            t.scale  = 1.0F;
+            t.seglen = L.seglen;
+            t.nfreqs = L.nfreqs;
+            t.nbins  = L.nbins;
+            t.bps    = bps;
+            t.block0 = static_cast<uint32_t>(block0);
+            tasks.push_back(t);
+            block0 += sr.plan.nsegments()[0] * bps;
+        }
+        LOKI_REQUIRE(block0 < (1ULL << 31), "FFABatch: brute-fold grid too large");
+        if (!tasks.empty()) {
+            m_launches.push_back({Stage::kFold, 0, lds, static_cast<uint32_t>(tasks.size()),
+                                  static_cast<uint32_t>(block0), append(blob, tasks), 0});
+        }
+    }
+    // 3. merges, one launch per level index and nbins class
+    SizeType max_levels = 0;
+    std::map<uint32_t, int> classes;
+    for (const auto& sr : m_series) {
+        max_levels = std::max(max_levels, sr.nlevels);
+        classes[static_cast<uint32_t>(sr.plan.config().get_nbins())] = 1;
+    }
+    for (SizeType k = 1; k + 1 < max_levels; ++k) {
+        for (const auto& [nb, unused] : classes) {
+            (void)unused;
+            std::vector<kern::MergeTask> tasks;
+            uint64_t block0 = 0;
+            for (const auto& sr : m_series) {
+                if (sr.plan.config().get_nbins() != nb || k + 1 >= sr.nlevels) {
+                    continue;
+                }
+                const auto& pl = sr.plan;
+                kern::MergeTask t{};
+                t.in          = buf(sr, k - 1);
+                t.out         = buf(sr, k);
+                t.coords      = nullptr;
+                t.coords_freq = coords + sr.coord_off + pl.ncoords_offsets64()[k];
+                t.ncur        = pl.ncoords()[k];
+                t.nprev       = pl.ncoords()[k - 1];
+                t.ts = kern::make_schedule(t.ncur, pl.param_counts()[k].back(), pl.nsegments()[k],
+                                           kern::merge_tile(nb), m_sched.merge_group,
+                                           m_sched.merge_run);
+                t.scale  = 1.0F;
+                t.block0 = static_cast<uint32_t>(block0);
+                tasks.push_back(t);
+                block0 += t.ts.nblocks;
+            }
+            LOKI_REQUIRE(block0 < (1ULL << 31), "FFABatch: merge grid too large");
+            if (!tasks.empty()) {
+                m_launches.push_back({Stage::kMerge, nb, false,
+                                      static_cast<uint32_t>(tasks.size()),
+                                      static_cast<uint32_t>(block0), append(blob, tasks), 0});
+            }
+        }
+    }
+    // 4. fused last levels (shift tables + merge + S/N), one launch per nbins class
+    for (const auto& [nb, unused] : classes) {
+        (void)unused;
+        std::vector<kern::FinalTask> tasks;
+        uint64_t block0 = 0;
+        uint32_t stride = 1;
+        for (const auto& sr : m_series) {
+            if (sr.plan.config().get_nbins() == nb) {
+                const auto wt = kern::make_width_table(sr.widths.data(), sr.widths.size(), nb);
+                stride        = std::max(stride, kern::final_lds_stride(nb, wt.wmax));
+            }
+        }
+        for (const auto& sr : m_series) {
+            if (sr.plan.config().get_nbins() != nb) {
+                continue;
+            }
+            const auto& pl  = sr.plan;
+            const SizeType L = sr.nlevels;
+            kern::FinalTask t{};
+            t.in     = buf(sr, L - 2);
+            t.scores = m_scores.as<float>() + sr.score_off;
+            t.nprev  = pl.ncoords()[L - 2];
+            t.lv     = resolve_level(pl, L - 1);
+            for (int q = 0; q < 3; ++q) {
+                t.fg.dv[q] = (t.lv.hi[q] - t.lv.lo[q]) / static_cast<double>(t.lv.n_cur[q]);
+            }
+            t.fg.stride = stride;
+            t.ts = kern::make_schedule(pl.ncoords().back(), t.lv.n_cur[2], 1, kern::kFinalRows,
+                                       m_sched.final_group, m_sched.final_run);
+            t.wt      = kern::make_width_table(sr.widths.data(), sr.widths.size(), nb);
+            t.unscale = 1.0F;
+            t.block0  = static_cast<uint32_t>(block0);
+            tasks.push_back(t);
+            block0 += t.ts.nblocks;
+        }
+        LOKI_REQUIRE(block0 < (1ULL << 31), "FFABatch: last-level grid too large");
+        if (!tasks.empty()) {
+            m_launches.push_back({Stage::kFinal, nb, false, static_cast<uint32_t>(tasks.size()),
+                                  static_cast<uint32_t>(block0), append(blob, tasks), stride});
+        }
+    }
+    m_tasks.reserve(std::max<SizeType>(blob.size(), 1));
+    hip_check(hipMemcpy(m_tasks.get(), blob.data(), blob.size(), hipMemcpyHostToDevice),
+              "copy batch task tables");
+}
+
+void FFABatch::enqueue(hipStream_t s) const {
+    const auto* base = static_cast<const uint8_t*>(m_tasks.get());
+    for (const auto& ln : m_launches) {
+        const void* t = base + ln.task_off;
+        switch (ln.stage) {
+        case Stage::kResolve:
+            kern::resolve_grouped(static_cast<const kern::ResolveTask*>(t), ln.ntasks, ln.nblocks,
+                                  1, s);
+            break;
+        case Stage::kFold:
+            kern::brute_fold_grouped(static_cast<const kern::FoldTask*>(t), ln.ntasks,
+                                     ln.nblocks, ln.flag, FoldStorage::kF32, s);
+            break;
+        case Stage::kMerge:
+            kern::merge_grouped(static_cast<const kern::MergeTask*>(t), ln.ntasks, ln.nblocks,
+                                ln.nbins, false, FoldStorage::kF32, s);
+            break;
+        case Stage::kFinal:
+            kern::final_grouped(static_cast<const kern::FinalTask*>(t), ln.ntasks, ln.nblocks,
+                                ln.nbins, FoldStorage::kF32, false, true, ln.stride, s);
+            break;
+        }
+        hip_check(hipGetLastError(), "batch launch");
+    }
+}
+
+void FFABatch::execute(hipStream_t s) {
+    DeviceGuard g(m_device);
+    if (!m_use_graph) {
+        enqueue(s);
+        return;
+    }
+    if (m_exec == nullptr) {
+        // capture on a stream of our own (the legacy default stream cannot be captured)
+        hipStream_t cap = nullptr;
+        hip_check(hipStreamCreate(&cap), "hipStreamCreate");
+        hip_check(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal),
+                  "hipStreamBeginCapture");
+        try {
+            enqueue(cap);
+        } catch (...) {
+            hipGraph_t tmp = nullptr;
+            (void)hipStreamEndCapture(cap, &tmp);
+            if (tmp != nullptr) {
+                (void)hipGraphDestroy(tmp);
+            }
+            (void)hipStreamDestroy(cap);
+            throw;
+        }
+        hip_check(hipStreamEndCapture(cap, &m_graph), "hipStreamEndCapture");
+        (void)hipStreamDestroy(cap);
+        hip_check(hipGraphInstantiate(&m_exec, m_graph, nullptr, nullptr, 0),
+                  "hipGraphInstantiate");
+    }
+    hip_check(hipGraphLaunch(m_exec, s), "hipGraphLaunch");
+}
+
+BatchSchedule FFABatch::autotune(hipStream_t s, int repeats) {
+    DeviceGuard g(m_device);
+    LOKI_REQUIRE(repeats >= 1, "autotune: repeats must be >= 1");
+    const bool graph = m_use_graph;
+    m_use_graph      = false;
+    std::vector<BatchSchedule> cands;
+    for (uint32_t mg : {4U, 16U, 64U}) {
+        for (uint32_t mr : {4U, 16U, 64U}) {
+            cands.push_back({mg, mr, 16, 16});
+        }
+    }
+    for (uint32_t fg : {4U, 64U}) {
+        for (uint32_t fr : {4U, 64U}) {
+            cands.push_back({16, 16, fg, fr});
+        }
+    }
+    hipEvent_t a = nullptr, b = nullptr;
+    hip_check(hipEventCreate(&a), "hipEventCreate");
+    hip_check(hipEventCreate(&b), "hipEventCreate");
+    // candidates 0..8 vary the merges (last levels at 16/16), 9..12 the last
+    // levels (merges at 16/16); the two winners are combined
+    const SizeType n_merge = 9;
+    BatchSchedule best_merge = cands[4], best_final = cands[4];
+    float best_merge_ms = -1.0F, best_final_ms = -1.0F;
+    try {
+        for (SizeType ci = 0; ci < cands.size(); ++ci) {
+            const auto& c = cands[ci];
+            set_schedule(c);
+            enqueue(s);  // warm-up
+            hip_check(hipEventRecord(a, s), "hipEventRecord");
+            for (int r = 0; r < repeats; ++r) {
+                enqueue(s);
+            }
+            hip_check(hipEventRecord(b, s), "hipEventRecord");
+            hip_check(hipEventSynchronize(b), "hipEventSynchronize");
+            float ms = 0.0F;
+            hip_check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+            if (ci < n_merge || (c.merge_group == 16 && c.merge_run == 16)) {
+                if (ci < n_merge && (best_merge_ms < 0.0F || ms < best_merge_ms)) {
+                    best_merge_ms = ms;
+                    best_merge    = c;
+                }
+            }
+            if (ci >= n_merge || (c.final_group == 16 && c.final_run == 16 &&
+                                  c.merge_group == 16 && c.merge_run == 16)) {
+                if (best_final_ms < 0.0F || ms < best_final_ms) {
+                    best_final_ms = ms;
+                    best_final    = c;
+                }
+            }
+        }
+    } catch (...) {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        m_use_graph = graph;
+        throw;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    const BatchSchedule merged{best_merge.merge_group, best_merge.merge_run,
+                               best_final.final_group, best_final.final_run};
+    set_schedule(merged);
+    m_use_graph = graph;
+    return merged;
+}
+
+}  // namespace loki_amd

@@ -10,6 +10,7 @@
 #include <string>
 #include <vector>
 
+#include "batch.hpp"
 #include "ffa.hpp"
 #include "kernels.hpp"
 
@@ -22,6 +23,10 @@ struct loki_plan_s {
 struct loki_ffa_s {
     loki_amd::FFAExecutor exec;
     loki_plan_s plan_view;
+};
+struct loki_batch_s {
+    std::unique_ptr<loki_amd::FFABatch> batch;
+    std::vector<loki_plan_s> plan_views;
 };
 
 namespace {
@@ -449,6 +454,167 @@ int loki_ffa_timings(loki_ffa_t ffa, loki_kernel_timing* out, size_t cap, size_t
             for (size_t i = 0; i < t.size(); ++i) {
                 out[i] = {static_cast<int>(t[i].kind), t[i].level, t[i].ms};
             }
+        }
+    });
+}
+
+int loki_batch_create(const loki_config_t* cfgs, size_t nseries, const size_t* widths,
+                      size_t nwidths, int device_id, loki_batch_t* out) {
+    return guarded([&] {
+        REQ(cfgs != nullptr && out != nullptr && nseries > 0, "null argument or empty batch");
+        REQ(widths != nullptr || nwidths == 0, "null widths");
+        std::vector<loki_amd::SearchConfig> v;
+        v.reserve(nseries);
+        for (size_t i = 0; i < nseries; ++i) {
+            REQ(cfgs[i] != nullptr, "null config in batch");
+            v.push_back(cfgs[i]->cfg);
+        }
+        std::vector<loki_amd::SizeType> w(widths, widths + nwidths);
+        auto h   = std::make_unique<loki_batch_s>();
+        h->batch = std::make_unique<loki_amd::FFABatch>(v, w, device_id);
+        for (size_t i = 0; i < nseries; ++i) {
+            h->plan_views.push_back(loki_plan_s{h->batch->plan(i)});
+        }
+        *out = h.release();
+    });
+}
+
+void loki_batch_destroy(loki_batch_t batch) { delete batch; }
+
+int loki_batch_info(loki_batch_t batch, size_t* nseries, size_t* total_trials,
+                    size_t* device_bytes, size_t* launches) {
+    return guarded([&] {
+        REQ(batch != nullptr, "null batch");
+        if (nseries != nullptr) {
+            *nseries = batch->batch->size();
+        }
+        if (total_trials != nullptr) {
+            *total_trials = batch->batch->total_trials();
+        }
+        if (device_bytes != nullptr) {
+            *device_bytes = batch->batch->device_bytes();
+        }
+        if (launches != nullptr) {
+            *launches = batch->batch->launches();
+        }
+    });
+}
+
+int loki_batch_plan(loki_batch_t batch, size_t i, loki_plan_t* out_plan_view) {
+    return guarded([&] {
+        REQ(batch != nullptr && out_plan_view != nullptr, "null argument");
+        REQ(i < batch->plan_views.size(), "series index out of range");
+        *out_plan_view = &batch->plan_views[i];
+    });
+}
+
+int loki_batch_series(loki_batch_t batch, size_t i, float** ts_e, float** ts_v, float** scores,
+                      size_t* scores_len, size_t* nwidths) {
+    return guarded([&] {
+        REQ(batch != nullptr, "null batch");
+        auto& b = *batch->batch;
+        REQ(i < b.size(), "series index out of range");
+        if (ts_e != nullptr) {
+            *ts_e = b.ts_e(i);
+        }
+        if (ts_v != nullptr) {
+            *ts_v = b.ts_v(i);
+        }
+        if (scores != nullptr) {
+            *scores = b.scores(i);
+        }
+        if (scores_len != nullptr) {
+            *scores_len = b.scores_len(i);
+        }
+        if (nwidths != nullptr) {
+            *nwidths = b.widths(i).size();
+        }
+    });
+}
+
+int loki_batch_set_input_host(loki_batch_t batch, size_t i, const float* ts_e,
+                              const float* ts_v, size_t nsamps) {
+    return guarded([&] {
+        REQ(batch != nullptr && ts_e != nullptr && ts_v != nullptr, "null argument");
+        auto& b = *batch->batch;
+        REQ(i < b.size(), "series index out of range");
+        REQ(nsamps == b.plan(i).config().get_nsamps(), "FFA::execute: ts_e must have size nsamps");
+        loki_amd::DeviceGuard g(b.device_id());
+        loki_amd::hip_check(hipMemcpy(b.ts_e(i), ts_e, nsamps * 4, hipMemcpyHostToDevice), "H2D");
+        loki_amd::hip_check(hipMemcpy(b.ts_v(i), ts_v, nsamps * 4, hipMemcpyHostToDevice), "H2D");
+    });
+}
+
+int loki_batch_set_input_device(loki_batch_t batch, size_t i, const float* ts_e,
+                                const float* ts_v, size_t nsamps, void* stream) {
+    return guarded([&] {
+        REQ(batch != nullptr && ts_e != nullptr && ts_v != nullptr, "null argument");
+        auto& b = *batch->batch;
+        REQ(i < b.size(), "series index out of range");
+        REQ(nsamps == b.plan(i).config().get_nsamps(), "FFA::execute: ts_e must have size nsamps");
+        loki_amd::DeviceGuard g(b.device_id());
+        const hipStream_t s = as_stream(stream);
+        loki_amd::hip_check(
+            hipMemcpyAsync(b.ts_e(i), ts_e, nsamps * 4, hipMemcpyDeviceToDevice, s), "D2D");
+        loki_amd::hip_check(
+            hipMemcpyAsync(b.ts_v(i), ts_v, nsamps * 4, hipMemcpyDeviceToDevice, s), "D2D");
+    });
+}
+
+int loki_batch_scores_host(loki_batch_t batch, size_t i, float* out, size_t len) {
+    return guarded([&] {
+        REQ(batch != nullptr && out != nullptr, "null argument");
+        auto& b = *batch->batch;
+        REQ(i < b.size(), "series index out of range");
+        REQ(len == b.scores_len(i), "scores must have size ncoords * nwidths");
+        loki_amd::DeviceGuard g(b.device_id());
+        loki_amd::hip_check(hipDeviceSynchronize(), "batch");
+        loki_amd::hip_check(hipMemcpy(out, b.scores(i), len * 4, hipMemcpyDeviceToHost), "D2H");
+    });
+}
+
+int loki_batch_widths(loki_batch_t batch, size_t i, size_t* out, size_t cap, size_t* n) {
+    return guarded([&] {
+        REQ(batch != nullptr && n != nullptr, "null argument");
+        REQ(i < batch->batch->size(), "series index out of range");
+        const auto& w = batch->batch->widths(i);
+        *n            = w.size();
+        REQ(out == nullptr || cap >= w.size(), "output buffer too small");
+        if (out != nullptr) {
+            std::copy(w.begin(), w.end(), out);
+        }
+    });
+}
+
+int loki_batch_execute(loki_batch_t batch, void* stream) {
+    return guarded([&] {
+        REQ(batch != nullptr, "null batch");
+        batch->batch->execute(as_stream(stream));
+    });
+}
+
+int loki_batch_set_graph(loki_batch_t batch, int enable) {
+    return guarded([&] {
+        REQ(batch != nullptr, "null batch");
+        batch->batch->set_graph(enable != 0);
+    });
+}
+
+int loki_batch_set_schedule(loki_batch_t batch, const loki_batch_schedule* sc) {
+    return guarded([&] {
+        REQ(batch != nullptr && sc != nullptr, "null argument");
+        batch->batch->set_schedule({sc->merge_group, sc->merge_run, sc->final_group,
+                                    sc->final_run});
+    });
+}
+
+int loki_batch_autotune(loki_batch_t batch, void* stream, int repeats,
+                        loki_batch_schedule* chosen) {
+    return guarded([&] {
+        REQ(batch != nullptr, "null batch");
+        const auto sc = batch->batch->autotune(as_stream(stream), repeats);
+        if (chosen != nullptr) {
+            *chosen = {sc.merge_group, sc.merge_run, sc.final_group, sc.final_run};
         }
     });
 }

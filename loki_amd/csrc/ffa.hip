@@ -120,9 +120,7 @@ std::vector<KernelTiming> LaunchTimer::collect() {
     return out;
 }
 
-namespace {
-
-kern::ResolveLevel make_level(const FFAPlan& plan, SizeType l) {
+kern::ResolveLevel resolve_level(const FFAPlan& plan, SizeType l) {
     const auto& cfg  = plan.config();
     const auto lims  = cfg.get_param_limits();
     const SizeType n = plan.n_params();
@@ -153,6 +151,8 @@ kern::ResolveLevel make_level(const FFAPlan& plan, SizeType l) {
     lv.np        = static_cast<uint32_t>(use);
     return lv;
 }
+
+namespace {
 
 void check_launch(const char* what) { hip_check(hipGetLastError(), what); }
 
@@ -188,7 +188,7 @@ void FFAExecutor::resolve_coordinates(void* coords_dev, hipStream_t s, SizeType 
     auto* base = static_cast<char*>(coords_dev);
     const SizeType top = std::min(last_level, m_plan.n_levels() - 1);
     for (SizeType l = 1; l <= top; ++l) {
-        const auto lv      = make_level(m_plan, l);
+        const auto lv      = resolve_level(m_plan, l);
         const SizeType off = m_plan.ncoords_offsets64()[l];
         const SizeType n   = m_plan.ncoords()[l];
         const int slot     = m_timer.start(KernelKind::kResolve, static_cast<int>(l), s);
@@ -396,7 +396,7 @@ void FFAExecutor::execute_scores(const float* ts_e, const float* ts_v,
         m_result.reserve(need);
         const void* parent =
             run_levels(ts_e, ts_v, m_result.get(), m_internal.get(), L - 2, false, s);
-        const auto lv  = make_level(m_plan, L - 1);
+        const auto lv  = resolve_level(m_plan, L - 1);
         const int slot = m_timer.start(KernelKind::kMergeScore, static_cast<int>(L - 1), s);
         kern::final_merge_score(parent, m_storage, std::ldexp(1.0F, m_k0 + static_cast<int>(L) - 2),
                                 ncoords, m_plan.ncoords()[L - 2], lv, wt, scores, find_max,

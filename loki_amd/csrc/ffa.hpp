@@ -142,6 +142,10 @@ private:
     SizeType elem_bytes() const noexcept { return m_fourier ? 2 * sizeof(float) : sizeof(float); }
 };
 
+/// Geometry of merge level `level` (>= 1) of a plan for the shift-table
+/// resolvers (the last three parameters: jerk, accel, freq).
+kern::ResolveLevel resolve_level(const FFAPlan& plan, SizeType level);
+
 /// Standalone brute fold (fold_cuda.cu:139-164, 260-277): fold has
 /// nsamps/segment_len * nfreqs * 2 * nbins floats.
 void brute_fold_time_device(const float* ts_e, const float* ts_v, SizeType nsamps,

@@ -102,15 +102,18 @@ k_bf_fill(const double* __restrict__ freqs, uint32_t seglen, uint32_t nbins, dou
 // TO: stored element type of the fold (float, or _Float16 / __bf16 with each
 // FP32 sum multiplied by `scale` before rounding).
 template <bool USE_LDS, typename TO>
-__global__ void __launch_bounds__(kBlock)
-k_bf_fold(const float* __restrict__ ts_e, const float* __restrict__ ts_v, uint32_t seglen,
-          uint32_t nfreqs, uint32_t nbins, uint32_t blocks_per_seg,
-          const uint32_t* __restrict__ list, const uint64_t* __restrict__ base_f,
-          const uint32_t* __restrict__ counts, TO* __restrict__ fold, float scale) {
+__device__ __forceinline__ void bf_fold_block(uint32_t bid, const float* __restrict__ ts_e,
+                                              const float* __restrict__ ts_v, uint32_t seglen,
+                                              uint32_t nfreqs, uint32_t nbins,
+                                              uint32_t blocks_per_seg,
+                                              const uint32_t* __restrict__ list,
+                                              const uint64_t* __restrict__ base_f,
+                                              const uint32_t* __restrict__ counts,
+                                              TO* __restrict__ fold, float scale) {
     __shared__ float s_e[USE_LDS ? kLdsSamps : 1];
     __shared__ float s_v[USE_LDS ? kLdsSamps : 1];
-    const uint64_t seg   = blockIdx.x / blocks_per_seg;
-    const uint32_t pbase = (blockIdx.x % blocks_per_seg) * kPairsBlk;
+    const uint64_t seg   = bid / blocks_per_seg;
+    const uint32_t pbase = (bid % blocks_per_seg) * kPairsBlk;
     const uint32_t npair = nfreqs * nbins;
     const float* __restrict__ ge = ts_e + seg * seglen;
     const float* __restrict__ gv = ts_v + seg * seglen;
@@ -162,6 +165,36 @@ k_bf_fold(const float* __restrict__ ts_e, const float* __restrict__ ts_v, uint32
             out[nbins + b] = narrow1<TO>(sv * scale);
         }
     }
+}
+
+template <bool USE_LDS, typename TO>
+__global__ void __launch_bounds__(kBlock)
+k_bf_fold(const float* __restrict__ ts_e, const float* __restrict__ ts_v, uint32_t seglen,
+          uint32_t nfreqs, uint32_t nbins, uint32_t blocks_per_seg,
+          const uint32_t* __restrict__ list, const uint64_t* __restrict__ base_f,
+          const uint32_t* __restrict__ counts, TO* __restrict__ fold, float scale) {
+    bf_fold_block<USE_LDS, TO>(blockIdx.x, ts_e, ts_v, seglen, nfreqs, nbins, blocks_per_seg,
+                               list, base_f, counts, fold, scale);
+}
+
+// Grouped brute folds of many series (one task each, sorted by block0).
+template <bool USE_LDS, typename TO>
+__global__ void __launch_bounds__(kBlock)
+k_bf_fold_grouped(const FoldTask* __restrict__ tasks, uint32_t ntasks) {
+    uint32_t lo = 0;
+    uint32_t hi = ntasks;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (tasks[mid].block0 <= blockIdx.x) {
+            lo = mid;
+        } else {
+            hi = mid;
+        }
+    }
+    const FoldTask& t = tasks[lo];
+    bf_fold_block<USE_LDS, TO>(blockIdx.x - t.block0, t.ts_e, t.ts_v, t.seglen, t.nfreqs,
+                               t.nbins, t.bps, t.list, t.base, t.counts,
+                               static_cast<TO*>(t.fold), t.scale);
 }
 
 }  // namespace
@@ -241,6 +274,44 @@ void brute_fold_time(const float* ts_e, const float* ts_v, uint64_t nseg,
         return launch_fold(ts_e, ts_v, blocks, bps, L, static_cast<_Float16*>(fold), scale, s);
     case FoldStorage::kBF16:
         return launch_fold(ts_e, ts_v, blocks, bps, L, static_cast<__bf16*>(fold), scale, s);
+    }
+}
+
+uint32_t brute_fold_blocks_per_segment(uint32_t nfreqs, uint32_t nbins) {
+    const uint64_t pairs = static_cast<uint64_t>(nfreqs) * nbins;
+    LOKI_REQUIRE(pairs < (1ULL << 31), "brute fold: too many (freq, bin) pairs");
+    return static_cast<uint32_t>((pairs + kPairsBlk - 1) / kPairsBlk);
+}
+
+bool brute_fold_uses_lds(uint32_t seglen) { return seglen <= static_cast<uint32_t>(kLdsSamps); }
+
+void brute_fold_grouped(const FoldTask* tasks, uint32_t ntasks, uint32_t nblocks, bool use_lds,
+                        FoldStorage so, hipStream_t s) {
+    if (ntasks == 0 || nblocks == 0) {
+        return;
+    }
+    switch (so) {
+    case FoldStorage::kF32:
+        if (use_lds) {
+            k_bf_fold_grouped<true, float><<<nblocks, kBlock, 0, s>>>(tasks, ntasks);
+        } else {
+            k_bf_fold_grouped<false, float><<<nblocks, kBlock, 0, s>>>(tasks, ntasks);
+        }
+        break;
+    case FoldStorage::kF16:
+        if (use_lds) {
+            k_bf_fold_grouped<true, _Float16><<<nblocks, kBlock, 0, s>>>(tasks, ntasks);
+        } else {
+            k_bf_fold_grouped<false, _Float16><<<nblocks, kBlock, 0, s>>>(tasks, ntasks);
+        }
+        break;
+    case FoldStorage::kBF16:
+        if (use_lds) {
+            k_bf_fold_grouped<true, __bf16><<<nblocks, kBlock, 0, s>>>(tasks, ntasks);
+        } else {
+            k_bf_fold_grouped<false, __bf16><<<nblocks, kBlock, 0, s>>>(tasks, ntasks);
+        }
+        break;
     }
 }
 

@@ -28,6 +28,7 @@ namespace loki_amd::kern {
 namespace {
 
 constexpr uint32_t kRows     = 64;
+
 constexpr uint32_t kRowGroup = 16;  // child rows scheduled back to back (parent reuse in L2)
 constexpr uint32_t kRun      = 16;  // frequency tiles per sequential output run
 
@@ -41,13 +42,44 @@ __device__ __forceinline__ float4 normalise4(const float4 e, const float4 v) {
     return p;
 }
 
-// TI: stored element type of the parent level; unless float, the merged e and
-// v are multiplied by `unscale` (2^(k0 + L - 2), exact) before normalisation.
+/// e = tail + head, v = tail + head (one FP32 add each), unscaled for
+/// reduced-precision parents, normalised and stored as 4 profile bins.
+template <bool EXACT, typename TI>
+__device__ __forceinline__ void merge_store(const float4 te, const float4 tv, const float4 he,
+                                            const float4 hv, float unscale,
+                                            float* __restrict__ dst) {
+    float4 e, v;
+    e.x = te.x + he.x;
+    e.y = te.y + he.y;
+    e.z = te.z + he.z;
+    e.w = te.w + he.w;
+    v.x = tv.x + hv.x;
+    v.y = tv.y + hv.y;
+    v.z = tv.z + hv.z;
+    v.w = tv.w + hv.w;
+    if constexpr (!std::is_same_v<TI, float>) {
+        e.x *= unscale;
+        e.y *= unscale;
+        e.z *= unscale;
+        e.w *= unscale;
+        v.x *= unscale;
+        v.y *= unscale;
+        v.z *= unscale;
+        v.w *= unscale;
+    }
+    const float4 p = normalise4<EXACT>(e, v);
+    dst[0] = p.x;
+    dst[1] = p.y;
+    dst[2] = p.z;
+    dst[3] = p.w;
+}
+
 template <int NB, int NP, bool FINDMAX, bool EXACT, typename TI>
-__global__ void __launch_bounds__(kRows)
-k_final_merge_score(const TI* __restrict__ in, uint64_t ncur, uint64_t nprev,
-                    ResolveLevel lv, FinalGrid fg, WidthTable wt, float* __restrict__ scores,
-                    TileSchedule ts, float unscale) {
+__device__ __forceinline__ void final_block(uint32_t bid, const TI* __restrict__ in,
+                                            uint64_t nprev, const ResolveLevel& lv,
+                                            const FinalGrid& fg, const WidthTable& wt,
+                                            float* __restrict__ scores, const TileSchedule& ts,
+                                            float unscale) {
     using CI = chunk4_t<TI>;
     constexpr uint32_t C4  = NB / 4;        // float4 chunks per component row
     constexpr uint32_t LPR = C4;            // lanes per row (4 bins of e and v each)
@@ -58,7 +90,7 @@ k_final_merge_score(const TI* __restrict__ in, uint64_t ncur, uint64_t nprev,
     __shared__ FFACoord s_coord[kRows];
     const uint32_t stride = fg.stride;
 
-    const Tile tl = tile_of(ts, xcd_remap(blockIdx.x, ts.nblocks), kRows);
+    const Tile tl = tile_of(ts, xcd_remap(bid, ts.nblocks), kRows);
     if (tl.count == 0) {
         return;  // idle block of a partial row group (before any barrier)
     }
@@ -106,31 +138,8 @@ k_final_merge_score(const TI* __restrict__ in, uint64_t ncur, uint64_t nprev,
             for (uint32_t u = 0; u < U; ++u) {
                 const uint32_t r = rsub + (s0 + u) * RPI;
                 if (r < nrow) {
-                    float4 e, v;
-                    e.x = te[u].x + he[u].x;
-                    e.y = te[u].y + he[u].y;
-                    e.z = te[u].z + he[u].z;
-                    e.w = te[u].w + he[u].w;
-                    v.x = tv[u].x + hv[u].x;
-                    v.y = tv[u].y + hv[u].y;
-                    v.z = tv[u].z + hv[u].z;
-                    v.w = tv[u].w + hv[u].w;
-                    if constexpr (!std::is_same_v<TI, float>) {
-                        e.x *= unscale;
-                        e.y *= unscale;
-                        e.z *= unscale;
-                        e.w *= unscale;
-                        v.x *= unscale;
-                        v.y *= unscale;
-                        v.z *= unscale;
-                        v.w *= unscale;
-                    }
-                    const float4 p = normalise4<EXACT>(e, v);
-                    float* __restrict__ dst = s_p + r * stride + 4 * i4;
-                    dst[0] = p.x;
-                    dst[1] = p.y;
-                    dst[2] = p.z;
-                    dst[3] = p.w;
+                    merge_store<EXACT, TI>(te[u], tv[u], he[u], hv[u], unscale,
+                                           s_p + r * stride + 4 * i4);
                 }
             }
         }
@@ -174,6 +183,36 @@ k_final_merge_score(const TI* __restrict__ in, uint64_t ncur, uint64_t nprev,
     }
 }
 
+template <int NB, int NP, bool FINDMAX, bool EXACT, typename TI>
+__global__ void __launch_bounds__(kRows)
+k_final_merge_score(const TI* __restrict__ in, uint64_t nprev, ResolveLevel lv, FinalGrid fg,
+                    WidthTable wt, float* __restrict__ scores, TileSchedule ts, float unscale) {
+    final_block<NB, NP, FINDMAX, EXACT, TI>(blockIdx.x, in, nprev, lv, fg, wt, scores, ts,
+                                            unscale);
+}
+
+// Grouped launch: the last levels of many independent FFAs (one task each,
+// sorted by first block); block -> task by binary search over block0.
+template <int NB, int NP, bool FINDMAX, bool EXACT, typename TI>
+__global__ void __launch_bounds__(kRows)
+k_final_grouped(const FinalTask* __restrict__ tasks, uint32_t ntasks) {
+    static_assert(kRows == kFinalRows, "grouped tiles must match the single-level kernel");
+    uint32_t lo = 0;
+    uint32_t hi = ntasks;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (tasks[mid].block0 <= blockIdx.x) {
+            lo = mid;
+        } else {
+            hi = mid;
+        }
+    }
+    const FinalTask& t = tasks[lo];
+    final_block<NB, NP, FINDMAX, EXACT, TI>(blockIdx.x - t.block0, static_cast<const TI*>(t.in),
+                                            t.nprev, t.lv, t.fg, t.wt, t.scores, t.ts,
+                                            t.unscale);
+}
+
 template <int NB, int NP, typename TI>
 void launch_nb_np(const TI* in, float unscale, uint64_t ncur, uint64_t nprev,
                   const ResolveLevel& lv, const WidthTable& wt, float* scores, bool find_max,
@@ -185,20 +224,20 @@ void launch_nb_np(const TI* in, float unscale, uint64_t ncur, uint64_t nprev,
     for (int k = 0; k < 3; ++k) {
         fg.dv[k] = (lv.hi[k] - lv.lo[k]) / static_cast<double>(lv.n_cur[k]);
     }
-    fg.stride        = (static_cast<uint32_t>(NB) + wt.wmax) | 1U;  // odd: conflict-free rows
+    fg.stride = final_lds_stride(NB, wt.wmax);
     const size_t lds = static_cast<size_t>(kRows) * fg.stride * sizeof(float);
     if (find_max && exact) {
         k_final_merge_score<NB, NP, true, true, TI>
-            <<<nb, kRows, lds, s>>>(in, ncur, nprev, lv, fg, wt, scores, ts, unscale);
+            <<<nb, kRows, lds, s>>>(in, nprev, lv, fg, wt, scores, ts, unscale);
     } else if (find_max) {
         k_final_merge_score<NB, NP, true, false, TI>
-            <<<nb, kRows, lds, s>>>(in, ncur, nprev, lv, fg, wt, scores, ts, unscale);
+            <<<nb, kRows, lds, s>>>(in, nprev, lv, fg, wt, scores, ts, unscale);
     } else if (exact) {
         k_final_merge_score<NB, NP, false, true, TI>
-            <<<nb, kRows, lds, s>>>(in, ncur, nprev, lv, fg, wt, scores, ts, unscale);
+            <<<nb, kRows, lds, s>>>(in, nprev, lv, fg, wt, scores, ts, unscale);
     } else {
         k_final_merge_score<NB, NP, false, false, TI>
-            <<<nb, kRows, lds, s>>>(in, ncur, nprev, lv, fg, wt, scores, ts, unscale);
+            <<<nb, kRows, lds, s>>>(in, nprev, lv, fg, wt, scores, ts, unscale);
     }
 }
 
@@ -254,6 +293,46 @@ void final_merge_score(const void* in, FoldStorage si, float unscale, uint64_t n
     case FoldStorage::kBF16:
         return launch_typed<__bf16>(in, unscale, ncur, nprev, lv, wt, scores, find_max, exact, s);
     }
+}
+
+namespace {
+
+template <int NB, typename TI>
+void grouped_launch(const FinalTask* tasks, uint32_t ntasks, uint32_t nblocks, bool find_max,
+                    bool exact, size_t lds, hipStream_t s) {
+    if (find_max && exact) {
+        k_final_grouped<NB, 1, true, true, TI><<<nblocks, kRows, lds, s>>>(tasks, ntasks);
+    } else if (find_max) {
+        k_final_grouped<NB, 1, true, false, TI><<<nblocks, kRows, lds, s>>>(tasks, ntasks);
+    } else if (exact) {
+        k_final_grouped<NB, 1, false, true, TI><<<nblocks, kRows, lds, s>>>(tasks, ntasks);
+    } else {
+        k_final_grouped<NB, 1, false, false, TI><<<nblocks, kRows, lds, s>>>(tasks, ntasks);
+    }
+}
+
+}  // namespace
+
+void final_grouped(const FinalTask* tasks, uint32_t ntasks, uint32_t nblocks, uint32_t nbins,
+                   FoldStorage si, bool find_max, bool exact, uint32_t lds_stride,
+                   hipStream_t s) {
+    LOKI_REQUIRE(si == FoldStorage::kF32, "grouped last level: FP32 parents only");
+    if (ntasks == 0 || nblocks == 0) {
+        return;
+    }
+    const size_t lds = static_cast<size_t>(kRows) * lds_stride * sizeof(float);
+    switch (nbins) {
+    case 8: return grouped_launch<8, float>(tasks, ntasks, nblocks, find_max, exact, lds, s);
+    case 16: return grouped_launch<16, float>(tasks, ntasks, nblocks, find_max, exact, lds, s);
+    case 32: return grouped_launch<32, float>(tasks, ntasks, nblocks, find_max, exact, lds, s);
+    case 64: return grouped_launch<64, float>(tasks, ntasks, nblocks, find_max, exact, lds, s);
+    case 128: return grouped_launch<128, float>(tasks, ntasks, nblocks, find_max, exact, lds, s);
+    default: LOKI_REQUIRE(false, "grouped last level: nbins must be 8, 16, 32, 64 or 128");
+    }
+}
+
+uint32_t final_lds_stride(uint32_t nbins, uint32_t wmax) {
+    return (nbins + wmax) | 1U;  // odd: conflict-free rows; n + wmax prefix-sum slots
 }
 
 }  // namespace loki_amd::kern

@@ -25,7 +25,7 @@
 namespace loki_amd::kern {
 namespace {
 
-constexpr int kBlock         = 256;
+constexpr int kBlock         = kMergeBlock;
 constexpr uint32_t kRowGroup = 16;  // child rows scheduled back to back (parent reuse in L2)
 constexpr uint32_t kRun      = 16;  // frequency tiles per sequential output run
 
@@ -41,85 +41,8 @@ __global__ void __launch_bounds__(kBlock)
 k_merge_v4(const TI* __restrict__ in, TO* __restrict__ out,
            const FFACoord* __restrict__ coords, const FFACoordFreq* __restrict__ cf,
            uint64_t ncur, uint64_t nprev, TileSchedule ts, float scale) {
-    using CI = chunk4_t<TI>;
-    using CO = chunk4_t<TO>;
-    constexpr uint32_t C4   = NB / 4;        // 4-element chunks per component
-    constexpr uint32_t LPR  = 2 * C4;        // chunks per row
-    constexpr uint32_t LANES = LPR < kBlock ? LPR : kBlock;
-    constexpr uint32_t ROWS = kBlock / LANES;
-    constexpr uint32_t TILE = ROWS * ITERS;
-    const Tile tl = tile_of(ts, xcd_remap(blockIdx.x, ts.nblocks), TILE);
-    if (tl.count == 0) {
-        return;
-    }
-    const uint64_t seg  = tl.seg;
-    const uint32_t rsub = threadIdx.x / LANES;
-    const uint32_t lch  = threadIdx.x % LANES;
-    const uint64_t in_seg_t = (2 * seg) * nprev;
-    const uint64_t in_seg_h = (2 * seg + 1) * nprev;
-    constexpr uint32_t CPL = LPR / LANES;  // chunks per lane per row
-    // All loads of the tile are issued before any add so several rows are in
-    // flight per lane.
-    float4 tv[ITERS][CPL], hv[ITERS][CPL];
-#pragma unroll
-    for (int it = 0; it < ITERS; ++it) {
-        const uint32_t local = static_cast<uint32_t>(it) * ROWS + rsub;
-        const uint64_t c     = tl.c0 + local;
-        if (local < tl.count) {
-            uint64_t it_row, ih_row;
-            uint32_t St, Sh;
-            if constexpr (POLY) {
-                const FFACoord cd = coords[c];
-                it_row = in_seg_t + cd.i_tail;
-                ih_row = in_seg_h + cd.i_head;
-                St     = shift_int(cd.shift_tail, NB);
-                Sh     = shift_int(cd.shift_head, NB);
-            } else {
-                const FFACoordFreq cd = cf[c];
-                it_row = in_seg_t + cd.idx;
-                ih_row = in_seg_h + cd.idx;
-                St     = 0;
-                Sh     = shift_int(cd.shift, NB);
-            }
-            const CI* __restrict__ trow = reinterpret_cast<const CI*>(in + it_row * (2 * NB));
-            const CI* __restrict__ hrow = reinterpret_cast<const CI*>(in + ih_row * (2 * NB));
-#pragma unroll
-            for (uint32_t k = 0; k < CPL; ++k) {
-                const uint32_t ch = lch + k * LANES;
-                const uint32_t q  = ch / C4;
-                const uint32_t i4 = ch % C4;
-                if constexpr (POLY) {
-                    tv[it][k] = rotated4_st<NB, TI>(trow + q * C4, i4, St);
-                } else {
-                    tv[it][k] = widen4<TI>(trow[ch]);
-                }
-                hv[it][k] = rotated4_st<NB, TI>(hrow + q * C4, i4, Sh);
-            }
-        }
-    }
-#pragma unroll
-    for (int it = 0; it < ITERS; ++it) {
-        const uint32_t local = static_cast<uint32_t>(it) * ROWS + rsub;
-        const uint64_t c     = tl.c0 + local;
-        if (local < tl.count) {
-            CO* __restrict__ orow = reinterpret_cast<CO*>(out + (seg * ncur + c) * (2 * NB));
-#pragma unroll
-            for (uint32_t k = 0; k < CPL; ++k) {
-                float4 o;
-                o.x = tv[it][k].x + hv[it][k].x;
-                o.y = tv[it][k].y + hv[it][k].y;
-                o.z = tv[it][k].z + hv[it][k].z;
-                o.w = tv[it][k].w + hv[it][k].w;
-                if constexpr (kScaled<TI, TO>) {
-                    o.x *= scale;
-                    o.y *= scale;
-                    o.z *= scale;
-                    o.w *= scale;
-                }
-                orow[lch + k * LANES] = narrow4<TO>(o);
-            }
-        }
-    }
+    merge_v4_block<NB, POLY, ITERS, TI, TO>(blockIdx.x, in, out, coords, cf, ncur, nprev, ts,
+                                            scale);
 }
 
 // Any nbins: one lane per output element pair (e and v of bin j).
@@ -166,22 +89,67 @@ k_merge_scalar(const TI* __restrict__ in, TO* __restrict__ out,
     }
 }
 
+// Grouped merges of many series (one task each, sorted by block0).
+template <int NB, bool POLY, int ITERS, typename T>
+__global__ void __launch_bounds__(kBlock)
+k_merge_grouped(const MergeTask* __restrict__ tasks, uint32_t ntasks) {
+    uint32_t lo = 0;
+    uint32_t hi = ntasks;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (tasks[mid].block0 <= blockIdx.x) {
+            lo = mid;
+        } else {
+            hi = mid;
+        }
+    }
+    const MergeTask& t = tasks[lo];
+    merge_v4_block<NB, POLY, ITERS, T, T>(blockIdx.x - t.block0, static_cast<const T*>(t.in),
+                                          static_cast<T*>(t.out), t.coords, t.coords_freq,
+                                          t.ncur, t.nprev, t.ts, t.scale);
+}
+
+/// Rows per lane and coordinates per block of the v4 merge for NB.
+template <int NB>
+struct V4Geom {
+    static constexpr uint32_t LPR     = NB / 2;
+    static constexpr uint32_t LANES   = LPR < kBlock ? LPR : kBlock;
+    static constexpr uint32_t ROWS    = kBlock / LANES;
+    static constexpr uint32_t CPL     = LPR / LANES;
+    static constexpr uint32_t IT_ROWS = ROWS >= 64 ? 1U : 64U / ROWS;
+    static constexpr uint32_t IT_REGS = 8U / CPL;
+    static constexpr int ITERS = static_cast<int>(IT_ROWS < IT_REGS ? IT_ROWS : IT_REGS);
+    static constexpr uint32_t TILE = ROWS * ITERS;
+};
+
+template <int NB, bool POLY, typename T>
+void launch_grouped(const MergeTask* tasks, uint32_t ntasks, uint32_t nblocks, hipStream_t s) {
+    k_merge_grouped<NB, POLY, V4Geom<NB>::ITERS, T><<<nblocks, kBlock, 0, s>>>(tasks, ntasks);
+}
+
+template <bool POLY, typename T>
+void grouped_nb(const MergeTask* tasks, uint32_t ntasks, uint32_t nblocks, uint32_t nbins,
+                hipStream_t s) {
+    switch (nbins) {
+    case 8: return launch_grouped<8, POLY, T>(tasks, ntasks, nblocks, s);
+    case 16: return launch_grouped<16, POLY, T>(tasks, ntasks, nblocks, s);
+    case 32: return launch_grouped<32, POLY, T>(tasks, ntasks, nblocks, s);
+    case 64: return launch_grouped<64, POLY, T>(tasks, ntasks, nblocks, s);
+    case 128: return launch_grouped<128, POLY, T>(tasks, ntasks, nblocks, s);
+    case 256: return launch_grouped<256, POLY, T>(tasks, ntasks, nblocks, s);
+    default: LOKI_REQUIRE(false, "grouped merge: nbins must be a power of two in [8, 256]");
+    }
+}
+
 template <int NB, bool POLY, typename TI, typename TO>
 void launch_v4(const TI* in, TO* out, const FFACoord* coords, const FFACoordFreq* cf,
                uint64_t ncur, uint64_t nprev, uint64_t nseg, uint64_t n_f, float scale,
                hipStream_t s) {
-    constexpr uint32_t LPR   = NB / 2;
-    constexpr uint32_t LANES = LPR < kBlock ? LPR : kBlock;
-    constexpr uint32_t ROWS  = kBlock / LANES;
-    constexpr uint32_t CPL   = LPR / LANES;
-    // rows per lane: up to 64 rows per block, at most 16 chunk loads in flight per lane
-    constexpr uint32_t IT_ROWS = ROWS >= 64 ? 1U : 64U / ROWS;
-    constexpr uint32_t IT_REGS = 8U / CPL;
-    constexpr int ITERS      = static_cast<int>(IT_ROWS < IT_REGS ? IT_ROWS : IT_REGS);
-    constexpr uint32_t TILE  = ROWS * ITERS;
-    const TileSchedule ts    = make_schedule(ncur, n_f, nseg, TILE, kRowGroup, kRun);
-    k_merge_v4<NB, POLY, ITERS, TI, TO><<<ts.nblocks, kBlock, 0, s>>>(in, out, coords, cf, ncur,
-                                                                      nprev, ts, scale);
+    // up to 64 rows per block, at most 16 chunk loads in flight per lane
+    using G               = V4Geom<NB>;
+    const TileSchedule ts = make_schedule(ncur, n_f, nseg, G::TILE, kRowGroup, kRun);
+    k_merge_v4<NB, POLY, G::ITERS, TI, TO><<<ts.nblocks, kBlock, 0, s>>>(in, out, coords, cf,
+                                                                         ncur, nprev, ts, scale);
 }
 
 template <bool POLY, typename TI, typename TO>
@@ -245,6 +213,36 @@ void merge_time(const void* in, void* out, const FFACoord* coords,
     }
     return so == FoldStorage::kBF16 ? std::apply(merge_typed<__bf16, __bf16>, args)
                                     : std::apply(merge_typed<__bf16, float>, args);
+}
+
+uint32_t merge_tile(uint32_t nbins) {
+    switch (nbins) {
+    case 8: return V4Geom<8>::TILE;
+    case 16: return V4Geom<16>::TILE;
+    case 32: return V4Geom<32>::TILE;
+    case 64: return V4Geom<64>::TILE;
+    case 128: return V4Geom<128>::TILE;
+    case 256: return V4Geom<256>::TILE;
+    default: LOKI_REQUIRE(false, "grouped merge: nbins must be a power of two in [8, 256]");
+    }
+    return 0;
+}
+
+void merge_grouped(const MergeTask* tasks, uint32_t ntasks, uint32_t nblocks, uint32_t nbins,
+                   bool poly, FoldStorage st, hipStream_t s) {
+    if (ntasks == 0 || nblocks == 0) {
+        return;
+    }
+    if (st == FoldStorage::kF32) {
+        return poly ? grouped_nb<true, float>(tasks, ntasks, nblocks, nbins, s)
+                    : grouped_nb<false, float>(tasks, ntasks, nblocks, nbins, s);
+    }
+    if (st == FoldStorage::kF16) {
+        return poly ? grouped_nb<true, _Float16>(tasks, ntasks, nblocks, nbins, s)
+                    : grouped_nb<false, _Float16>(tasks, ntasks, nblocks, nbins, s);
+    }
+    return poly ? grouped_nb<true, __bf16>(tasks, ntasks, nblocks, nbins, s)
+                : grouped_nb<false, __bf16>(tasks, ntasks, nblocks, nbins, s);
 }
 
 }  // namespace loki_amd::kern

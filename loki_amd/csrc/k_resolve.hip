@@ -33,6 +33,41 @@ k_resolve_poly(ResolveLevel lv, FFACoord* __restrict__ out, uint64_t n) {
     out[i] = c;
 }
 
+__device__ __forceinline__ uint32_t find_task_resolve(const ResolveTask* __restrict__ tasks,
+                                                      uint32_t ntasks) {
+    uint32_t lo = 0;
+    uint32_t hi = ntasks;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (tasks[mid].block0 <= blockIdx.x) {
+            lo = mid;
+        } else {
+            hi = mid;
+        }
+    }
+    return lo;
+}
+
+template <int NP>
+__global__ void __launch_bounds__(256)
+k_resolve_grouped(const ResolveTask* __restrict__ tasks, uint32_t ntasks) {
+    const ResolveTask& t = tasks[find_task_resolve(tasks, ntasks)];
+    const uint64_t i     = static_cast<uint64_t>(blockIdx.x - t.block0) * 256 + threadIdx.x;
+    if (i >= t.n) {
+        return;
+    }
+    if constexpr (NP == 1) {
+        const double f_cur = grid::param_val_at_idx(t.lv.lo[2], t.lv.hi[2], t.lv.n_cur[2],
+                                                    static_cast<uint32_t>(i));
+        FFACoordFreq c;
+        c.idx = grid::nearest_idx(f_cur, t.lv.lo[2], t.lv.hi[2], t.lv.n_prev[2]);
+        c.shift = grid::phase_idx(t.lv.tseg_prev, f_cur, t.lv.nbins, 0.0);
+        static_cast<FFACoordFreq*>(t.out)[i] = c;
+    } else {
+        static_cast<FFACoord*>(t.out)[i] = resolve_coord<NP>(t.lv, i);
+    }
+}
+
 unsigned grid_for(uint64_t n, unsigned block) {
     const uint64_t g = (n + block - 1) / block;
     LOKI_REQUIRE(g < (1ULL << 31), "resolver grid too large");
@@ -61,6 +96,19 @@ void resolve_poly(const ResolveLevel& lv, FFACoord* out, uint64_t ncoords, hipSt
         k_resolve_poly<2><<<grid_for(ncoords, 256), 256, 0, s>>>(lv, out, ncoords);
     } else {
         k_resolve_poly<3><<<grid_for(ncoords, 256), 256, 0, s>>>(lv, out, ncoords);
+    }
+}
+
+void resolve_grouped(const ResolveTask* tasks, uint32_t ntasks, uint32_t nblocks, uint32_t np,
+                     hipStream_t s) {
+    if (ntasks == 0 || nblocks == 0) {
+        return;
+    }
+    switch (np) {
+    case 1: k_resolve_grouped<1><<<nblocks, 256, 0, s>>>(tasks, ntasks); break;
+    case 2: k_resolve_grouped<2><<<nblocks, 256, 0, s>>>(tasks, ntasks); break;
+    case 3: k_resolve_grouped<3><<<nblocks, 256, 0, s>>>(tasks, ntasks); break;
+    default: LOKI_REQUIRE(false, "resolve_grouped: np must be 1, 2 or 3");
     }
 }
 

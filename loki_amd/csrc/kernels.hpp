@@ -8,6 +8,7 @@
 
 #include "common.hpp"
 #include "device.hpp"
+#include "merge_device.hpp"
 #include "plan.hpp"
 #include "storage.hpp"
 
@@ -29,7 +30,7 @@ struct ResolveLevel {
 /// Extra per-level constants of the fused last-level kernel.
 struct FinalGrid {
     double dv[3];     ///< (hi - lo) / n_cur per parameter (psr_utils.hpp:290-291)
-    uint32_t stride;  ///< LDS row stride in floats (>= nbins + wmax, odd)
+    uint32_t stride;  ///< LDS row stride in floats (>= nbins, odd)
 };
 
 /// Fills ncoords FFACoordFreq records (np == 1) of one level.
@@ -108,6 +109,74 @@ bool final_merge_score_supported(uint32_t nbins);
 void final_merge_score(const void* in, FoldStorage si, float unscale, uint64_t ncur,
                        uint64_t nprev, const ResolveLevel& lv, const WidthTable& wt,
                        float* scores, bool find_max, bool exact, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Grouped launches: many independent FFAs (a batch of series) advanced by one
+// launch per stage.  Each task owns a contiguous range of blocks starting at
+// block0; tasks are sorted by block0 and a block finds its task by binary
+// search.
+// ---------------------------------------------------------------------------
+struct ResolveTask {
+    ResolveLevel lv;
+    void* out;        ///< FFACoordFreq (np == 1) or FFACoord records
+    uint64_t n;       ///< coordinates of the level
+    uint32_t block0;  ///< first 256-lane block of the task
+};
+struct FoldTask {
+    const float* ts_e;
+    const float* ts_v;
+    const uint32_t* list;
+    const uint64_t* base;
+    const uint32_t* counts;
+    void* fold;
+    float scale;
+    uint32_t seglen, nfreqs, nbins, bps;  ///< bps: blocks per segment
+    uint32_t block0;
+};
+struct MergeTask {
+    const void* in;
+    void* out;
+    const FFACoord* coords;       ///< poly levels (or nullptr)
+    const FFACoordFreq* coords_freq;  ///< freq-only levels (or nullptr)
+    uint64_t ncur, nprev;
+    TileSchedule ts;
+    float scale;
+    uint32_t block0;
+};
+struct FinalTask {
+    const void* in;
+    float* scores;
+    uint64_t nprev;
+    ResolveLevel lv;
+    FinalGrid fg;
+    TileSchedule ts;
+    WidthTable wt;
+    float unscale;
+    uint32_t block0;
+};
+
+/// Grouped FP64 shift tables (np == 1 or poly, one kind per launch).
+void resolve_grouped(const ResolveTask* tasks, uint32_t ntasks, uint32_t nblocks, uint32_t np,
+                     hipStream_t s);
+/// Blocks per segment of the time brute fold, and whether a segment of
+/// seglen samples is staged in LDS (tasks of one grouped launch must agree).
+uint32_t brute_fold_blocks_per_segment(uint32_t nfreqs, uint32_t nbins);
+bool brute_fold_uses_lds(uint32_t seglen);
+/// Grouped time brute fold (all tasks stored as `so`).
+void brute_fold_grouped(const FoldTask* tasks, uint32_t ntasks, uint32_t nblocks, bool use_lds,
+                        FoldStorage so, hipStream_t s);
+/// Grouped merges of one storage type, nbins and coordinate kind.
+void merge_grouped(const MergeTask* tasks, uint32_t ntasks, uint32_t nblocks, uint32_t nbins,
+                   bool poly, FoldStorage st, hipStream_t s);
+/// Tile geometry of merge_grouped for a given nbins: coordinates per block.
+uint32_t merge_tile(uint32_t nbins);
+/// Grouped fused last levels (freq-only) of one nbins and parent storage.
+void final_grouped(const FinalTask* tasks, uint32_t ntasks, uint32_t nblocks, uint32_t nbins,
+                   FoldStorage si, bool find_max, bool exact, uint32_t lds_stride,
+                   hipStream_t s);
+/// Coordinates per block of the fused last level, and its LDS row stride.
+constexpr uint32_t kFinalRows = 64;
+uint32_t final_lds_stride(uint32_t nbins, uint32_t wmax);
 
 /// snr_boxcar_3d[_max]: folds [nprof, 2, nbins] -> scores [nprof, nw] or [nprof].
 void snr_boxcar_3d(const float* folds, uint64_t nprof, uint32_t nbins, const WidthTable& wt,

@@ -8,6 +8,7 @@
 #include <type_traits>
 
 #include "common.hpp"
+#include "plan.hpp"
 #include "storage.hpp"
 
 namespace loki_amd::kern {
@@ -144,6 +145,102 @@ __device__ __forceinline__ float4 rotated4_st(const chunk4_t<T>* __restrict__ ro
         const uint32_t c0 = e0 >> 2;
         const uint32_t c1 = (c0 + 1) & (C4 - 1);
         return widen4<T>(funnel4_16(row[c0], row[c1], e0 & 3U));
+    }
+}
+
+/// Lanes per block of the time-domain merge (single-level and grouped launches).
+constexpr int kMergeBlock = 256;
+
+/// One 256-lane block of the time-domain merge (k_merge.hip): block `bid` of
+/// the level's tile schedule.  TI/TO: stored types of parent and child
+/// levels; unless both are float the FP32 sum is multiplied by `scale` before
+/// it is rounded to TO.
+template <int NB, bool POLY, int ITERS, typename TI, typename TO>
+__device__ __forceinline__ void merge_v4_block(uint32_t bid, const TI* __restrict__ in,
+                                               TO* __restrict__ out,
+                                               const FFACoord* __restrict__ coords,
+                                               const FFACoordFreq* __restrict__ cf,
+                                               uint64_t ncur, uint64_t nprev,
+                                               const TileSchedule& ts, float scale) {
+    constexpr uint32_t kBlock = kMergeBlock;
+    using CI = chunk4_t<TI>;
+    using CO = chunk4_t<TO>;
+    constexpr uint32_t C4   = NB / 4;        // 4-element chunks per component
+    constexpr uint32_t LPR  = 2 * C4;        // chunks per row
+    constexpr uint32_t LANES = LPR < kBlock ? LPR : kBlock;
+    constexpr uint32_t ROWS = kBlock / LANES;
+    constexpr uint32_t TILE = ROWS * ITERS;
+    const Tile tl = tile_of(ts, xcd_remap(bid, ts.nblocks), TILE);
+    if (tl.count == 0) {
+        return;
+    }
+    const uint64_t seg  = tl.seg;
+    const uint32_t rsub = threadIdx.x / LANES;
+    const uint32_t lch  = threadIdx.x % LANES;
+    const uint64_t in_seg_t = (2 * seg) * nprev;
+    const uint64_t in_seg_h = (2 * seg + 1) * nprev;
+    constexpr uint32_t CPL = LPR / LANES;  // chunks per lane per row
+    // All loads of the tile are issued before any add so several rows are in
+    // flight per lane.
+    float4 tv[ITERS][CPL], hv[ITERS][CPL];
+#pragma unroll
+    for (int it = 0; it < ITERS; ++it) {
+        const uint32_t local = static_cast<uint32_t>(it) * ROWS + rsub;
+        const uint64_t c     = tl.c0 + local;
+        if (local < tl.count) {
+            uint64_t it_row, ih_row;
+            uint32_t St, Sh;
+            if constexpr (POLY) {
+                const FFACoord cd = coords[c];
+                it_row = in_seg_t + cd.i_tail;
+                ih_row = in_seg_h + cd.i_head;
+                St     = shift_int(cd.shift_tail, NB);
+                Sh     = shift_int(cd.shift_head, NB);
+            } else {
+                const FFACoordFreq cd = cf[c];
+                it_row = in_seg_t + cd.idx;
+                ih_row = in_seg_h + cd.idx;
+                St     = 0;
+                Sh     = shift_int(cd.shift, NB);
+            }
+            const CI* __restrict__ trow = reinterpret_cast<const CI*>(in + it_row * (2 * NB));
+            const CI* __restrict__ hrow = reinterpret_cast<const CI*>(in + ih_row * (2 * NB));
+#pragma unroll
+            for (uint32_t k = 0; k < CPL; ++k) {
+                const uint32_t ch = lch + k * LANES;
+                const uint32_t q  = ch / C4;
+                const uint32_t i4 = ch % C4;
+                if constexpr (POLY) {
+                    tv[it][k] = rotated4_st<NB, TI>(trow + q * C4, i4, St);
+                } else {
+                    tv[it][k] = widen4<TI>(trow[ch]);
+                }
+                hv[it][k] = rotated4_st<NB, TI>(hrow + q * C4, i4, Sh);
+            }
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < ITERS; ++it) {
+        const uint32_t local = static_cast<uint32_t>(it) * ROWS + rsub;
+        const uint64_t c     = tl.c0 + local;
+        if (local < tl.count) {
+            CO* __restrict__ orow = reinterpret_cast<CO*>(out + (seg * ncur + c) * (2 * NB));
+#pragma unroll
+            for (uint32_t k = 0; k < CPL; ++k) {
+                float4 o;
+                o.x = tv[it][k].x + hv[it][k].x;
+                o.y = tv[it][k].y + hv[it][k].y;
+                o.z = tv[it][k].z + hv[it][k].z;
+                o.w = tv[it][k].w + hv[it][k].w;
+                if constexpr (!(std::is_same_v<TI, float> && std::is_same_v<TO, float>)) {
+                    o.x *= scale;
+                    o.y *= scale;
+                    o.z *= scale;
+                    o.w *= scale;
+                }
+                orow[lch + k * LANES] = narrow4<TO>(o);
+            }
+        }
     }
 }
 

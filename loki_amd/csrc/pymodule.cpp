@@ -351,6 +351,109 @@ struct FFAComplex : FFA {
     FFAComplex(const Config& cfg, int device_id) : FFA(cfg, device_id, LOKI_FOLD_FOURIER) {}
 };
 
+/// libculoki.ffa.FFABatchCUDA: many frequency-only searches advanced together.
+class Batch {
+public:
+    Batch(const std::vector<Config>& cfgs, std::optional<PyArrayT<size_t>> widths, int device_id)
+        : m_cfgs(cfgs) {
+        std::vector<loki_config_t> hs;
+        for (const auto& c : cfgs) {
+            hs.push_back(c.handle());
+        }
+        std::vector<size_t> w;
+        if (widths.has_value()) {
+            w.assign(widths->data(), widths->data() + widths->size());
+        }
+        loki_batch_t h = nullptr;
+        check(loki_batch_create(hs.data(), hs.size(), w.empty() ? nullptr : w.data(), w.size(),
+                                device_id, &h));
+        m_h.reset(h, loki_batch_destroy);
+    }
+    size_t size() const { return m_cfgs.size(); }
+    PlanTime plan(size_t i) const { return PlanTime(m_cfgs.at(i)); }
+    void set_input(size_t i, const PyArrayT<float>& ts_e, const PyArrayT<float>& ts_v) {
+        if (ts_v.size() != ts_e.size()) {
+            throw std::runtime_error("FFA::execute: ts_v must have size nsamps");
+        }
+        check(loki_batch_set_input_host(m_h.get(), i, ts_e.data(), ts_v.data(),
+                                        static_cast<size_t>(ts_e.size())));
+    }
+    void set_input_device(size_t i, uintptr_t ts_e, uintptr_t ts_v, size_t nsamps,
+                          uintptr_t stream) {
+        check(loki_batch_set_input_device(m_h.get(), i, reinterpret_cast<const float*>(ts_e),
+                                          reinterpret_cast<const float*>(ts_v), nsamps,
+                                          reinterpret_cast<void*>(stream)));
+    }
+    py::tuple input_ptrs(size_t i) const {
+        float *e = nullptr, *v = nullptr;
+        check(loki_batch_series(m_h.get(), i, &e, &v, nullptr, nullptr, nullptr));
+        return py::make_tuple(reinterpret_cast<uintptr_t>(e), reinterpret_cast<uintptr_t>(v));
+    }
+    py::tuple scores_ptr(size_t i) const {
+        float* sp = nullptr;
+        size_t n = 0;
+        check(loki_batch_series(m_h.get(), i, nullptr, nullptr, &sp, &n, nullptr));
+        return py::make_tuple(reinterpret_cast<uintptr_t>(sp), n);
+    }
+    py::array_t<float> scores(size_t i) const {
+        size_t n = 0;
+        check(loki_batch_series(m_h.get(), i, nullptr, nullptr, nullptr, &n, nullptr));
+        py::array_t<float> out(static_cast<py::ssize_t>(n));
+        check(loki_batch_scores_host(m_h.get(), i, out.mutable_data(), n));
+        return out;
+    }
+    py::array_t<size_t> widths(size_t i) const {
+        size_t n = 0;
+        check(loki_batch_widths(m_h.get(), i, nullptr, 0, &n));
+        py::array_t<size_t> out(static_cast<py::ssize_t>(n));
+        check(loki_batch_widths(m_h.get(), i, out.mutable_data(), n, &n));
+        return out;
+    }
+    void execute(uintptr_t stream) {
+        check(loki_batch_execute(m_h.get(), reinterpret_cast<void*>(stream)));
+    }
+    void set_graph(bool on) { check(loki_batch_set_graph(m_h.get(), on ? 1 : 0)); }
+    void set_schedule(const std::map<std::string, uint32_t>& sc) {
+        loki_batch_schedule b{16, 16, 16, 16};
+        for (const auto& [k, v] : sc) {
+            if (k == "merge_group") {
+                b.merge_group = v;
+            } else if (k == "merge_run") {
+                b.merge_run = v;
+            } else if (k == "final_group") {
+                b.final_group = v;
+            } else if (k == "final_run") {
+                b.final_run = v;
+            } else {
+                throw std::runtime_error("unknown schedule key " + k);
+            }
+        }
+        check(loki_batch_set_schedule(m_h.get(), &b));
+    }
+    std::map<std::string, uint32_t> autotune(int repeats, uintptr_t stream) {
+        loki_batch_schedule b{};
+        check(loki_batch_autotune(m_h.get(), reinterpret_cast<void*>(stream), repeats, &b));
+        return {{"merge_group", b.merge_group},
+                {"merge_run", b.merge_run},
+                {"final_group", b.final_group},
+                {"final_run", b.final_run}};
+    }
+    py::dict info() const {
+        size_t n = 0, trials = 0, bytes = 0, launches = 0;
+        check(loki_batch_info(m_h.get(), &n, &trials, &bytes, &launches));
+        py::dict d;
+        d["nseries"]      = n;
+        d["total_trials"] = trials;
+        d["device_bytes"] = bytes;
+        d["launches"]     = launches;
+        return d;
+    }
+
+private:
+    std::vector<Config> m_cfgs;
+    std::shared_ptr<loki_batch_s> m_h;
+};
+
 void check_nsamps(const PyArrayT<float>& ts_e, const PyArrayT<float>& ts_v, const Config& cfg) {
     if (static_cast<size_t>(ts_e.size()) != cfg.params().nsamps) {
         throw std::runtime_error("FFA::execute: ts_e must have size nsamps");
@@ -672,6 +775,23 @@ PYBIND11_MODULE(_loki_amd, m) {
         },
         py::arg("ts_e"), py::arg("ts_v"), py::arg("cfg"), py::arg("device_id") = 0,
         py::arg("quiet") = false, py::arg("fold_storage") = "f32");
+    py::class_<Batch>(m_ffa, "FFABatchCUDA")
+        .def(py::init<const std::vector<Config>&, std::optional<PyArrayT<size_t>>, int>(),
+             py::arg("cfgs"), py::arg("widths") = std::nullopt, py::arg("device_id") = 0)
+        .def_property_readonly("size", &Batch::size)
+        .def("plan", &Batch::plan, py::arg("i"))
+        .def("set_input", &Batch::set_input, py::arg("i"), py::arg("ts_e"), py::arg("ts_v"))
+        .def("set_input_device", &Batch::set_input_device, py::arg("i"), py::arg("ts_e_ptr"),
+             py::arg("ts_v_ptr"), py::arg("nsamps"), py::arg("stream") = 0)
+        .def("input_ptrs", &Batch::input_ptrs, py::arg("i"))
+        .def("scores_ptr", &Batch::scores_ptr, py::arg("i"))
+        .def("scores", &Batch::scores, py::arg("i"))
+        .def("widths", &Batch::widths, py::arg("i"))
+        .def("execute", &Batch::execute, py::arg("stream") = 0)
+        .def("set_graph", &Batch::set_graph, py::arg("enable"))
+        .def("set_schedule", &Batch::set_schedule, py::arg("schedule"))
+        .def("autotune", &Batch::autotune, py::arg("repeats") = 3, py::arg("stream") = 0)
+        .def("info", &Batch::info);
     m_ffa.def(
         "compute_ffa_fourier_cuda",
         [](const PyArrayT<float>& ts_e, const PyArrayT<float>& ts_v, const Config& cfg,

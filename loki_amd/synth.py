@@ -70,3 +70,45 @@ def make_series(nsamps: int, tsamp: float, seed: int, f0: float | None = None,
         amp = snr / unit if unit > 0 else 0.0
         ts_e += (amp * shape).astype(np.float32)
     return ts_e, ts_v, amp
+
+
+def make_batch_specs(nseries: int, seed: int, log2_len=(18, 22), nbins_choices=(32, 64, 128),
+                     tsamp: float = 64e-6, period_centre=(0.3, 3.0), min_turns: float = 4.0):
+    """Specs of a batch of independent period searches of uneven length.
+
+    Series i (numpy PCG64 seeded with seed + i): nsamps = 2^round(U(log2_len)); nbins drawn
+    from nbins_choices; a period centre P drawn uniformly in period_centre (capped so that the
+    observation spans at least ``min_turns`` turns of the longest period, which keeps at least
+    one FFA merge level), searched over periods [P/2, 2P], i.e. frequencies [1/(2P), 2/P];
+    every other series carries a pulse with period uniform in [P/2, 2P], FWHM 2-10 % of the
+    period and S/N 8-30.  Returns a list of dicts (nsamps, nbins, f_lo, f_hi, seed, signal).
+    """
+    specs = []
+    for i in range(nseries):
+        rng = np.random.default_rng(seed + i)
+        nsamps = 2 ** int(round(rng.uniform(*log2_len)))
+        nbins = int(rng.choice(nbins_choices))
+        tobs = nsamps * tsamp
+        p_hi = min(period_centre[1], tobs / (2.0 * min_turns))
+        p_lo = min(period_centre[0], 0.5 * p_hi)
+        p_c = rng.uniform(p_lo, p_hi)
+        sig = None
+        if i % 2 == 0:
+            period = rng.uniform(0.5 * p_c, 2.0 * p_c)
+            sig = dict(f0=1.0 / period, fwhm=float(rng.uniform(0.02, 0.10)),
+                       snr=float(rng.uniform(8.0, 30.0)))
+        specs.append(dict(nsamps=nsamps, nbins=nbins, f_lo=1.0 / (2.0 * p_c), f_hi=2.0 / p_c,
+                          seed=seed + i, signal=sig))
+    return specs
+
+
+def make_batch_series(spec: dict, tsamp: float = 64e-6, widths=(1, 2, 3, 4, 6, 8, 12, 16, 24, 32)):
+    """(ts_e, ts_v) of one batch spec (see make_batch_specs)."""
+    sig = spec["signal"]
+    w = [x for x in widths if x < spec["nbins"]]
+    if sig is None:
+        e, v, _ = make_series(spec["nsamps"], tsamp, spec["seed"])
+    else:
+        e, v, _ = make_series(spec["nsamps"], tsamp, spec["seed"], f0=sig["f0"], fwhm=sig["fwhm"],
+                              snr=sig["snr"], nbins=spec["nbins"], widths=w)
+    return e, v
