@@ -89,6 +89,8 @@ FFABatch::FFABatch(const std::vector<SearchConfig>& cfgs, const std::vector<Size
             m_freqs.as<double>(), static_cast<uint32_t>(nfr),
             static_cast<uint32_t>(sr.plan.segment_lens()[0]), static_cast<uint32_t>(cfg.get_nbins()),
             cfg.get_tsamp(), 0.0, s);
+        // many series share each grouped launch: favour reuse of staged samples
+        kern::choose_run_slots(sr.lists, 1024);
     }
     build_tasks();
 }
@@ -175,26 +177,30 @@ void FFABatch::build_tasks() {
             }
         }
         if (!tasks.empty()) {
-            m_launches.push_back({Stage::kResolve, 0, false, static_cast<uint32_t>(tasks.size()),
+            m_launches.push_back({Stage::kResolve, 0, static_cast<uint32_t>(tasks.size()),
                                   block0, append(blob, tasks), 0});
         }
     }
-    // 2. brute folds (LDS-staged segments and long segments in separate launches)
-    for (bool lds : {true, false}) {
+    // 2. brute folds: list-path short segments, and run-path long segments by
+    //    pairs per lane (one grouped launch per kind)
+    for (uint32_t kind : {0U, 1U, 2U, 4U, 8U}) {
         std::vector<kern::FoldTask> tasks;
         uint64_t block0 = 0;
         for (const auto& sr : m_series) {
             const auto& L = sr.lists;
-            if (kern::brute_fold_uses_lds(L.seglen) != lds) {
+            if ((L.use_runs ? L.run_slots : 0U) != kind) {
                 continue;
             }
-            const uint32_t bps = kern::brute_fold_blocks_per_segment(L.nfreqs, L.nbins);
+            const uint32_t bps = kern::brute_fold_blocks_per_segment(L);
             kern::FoldTask t{};
             t.ts_e   = ts_e(static_cast<SizeType>(&sr - m_series.data()));
             t.ts_v   = ts_v(static_cast<SizeType>(&sr - m_series.data()));
             t.list   = L.list.as<uint32_t>();
             t.base   = L.base.as<uint64_t>();
             t.counts = L.counts.as<uint32_t>();
+            t.freqs  = L.freqs.as<double>();
+            t.tsamp  = L.tsamp;
+            t.t_ref  = L.t_ref;
             t.fold   = buf(sr, 0);
             t.scale  = 1.0F;
             t.seglen = L.seglen;
@@ -207,7 +213,7 @@ void FFABatch::build_tasks() {
         }
         LOKI_REQUIRE(block0 < (1ULL << 31), "FFABatch: brute-fold grid too large");
         if (!tasks.empty()) {
-            m_launches.push_back({Stage::kFold, 0, lds, static_cast<uint32_t>(tasks.size()),
+            m_launches.push_back({Stage::kFold, kind, static_cast<uint32_t>(tasks.size()),
                                   static_cast<uint32_t>(block0), append(blob, tasks), 0});
         }
     }
@@ -245,8 +251,7 @@ void FFABatch::build_tasks() {
             }
             LOKI_REQUIRE(block0 < (1ULL << 31), "FFABatch: merge grid too large");
             if (!tasks.empty()) {
-                m_launches.push_back({Stage::kMerge, nb, false,
-                                      static_cast<uint32_t>(tasks.size()),
+                m_launches.push_back({Stage::kMerge, nb, static_cast<uint32_t>(tasks.size()),
                                       static_cast<uint32_t>(block0), append(blob, tasks), 0});
             }
         }
@@ -288,7 +293,7 @@ void FFABatch::build_tasks() {
         }
         LOKI_REQUIRE(block0 < (1ULL << 31), "FFABatch: last-level grid too large");
         if (!tasks.empty()) {
-            m_launches.push_back({Stage::kFinal, nb, false, static_cast<uint32_t>(tasks.size()),
+            m_launches.push_back({Stage::kFinal, nb, static_cast<uint32_t>(tasks.size()),
                                   static_cast<uint32_t>(block0), append(blob, tasks), stride});
         }
     }
@@ -308,7 +313,7 @@ void FFABatch::enqueue(hipStream_t s) const {
             break;
         case Stage::kFold:
             kern::brute_fold_grouped(static_cast<const kern::FoldTask*>(t), ln.ntasks,
-                                     ln.nblocks, ln.flag, FoldStorage::kF32, s);
+                                     ln.nblocks, ln.nbins, FoldStorage::kF32, s);
             break;
         case Stage::kMerge:
             kern::merge_grouped(static_cast<const kern::MergeTask*>(t), ln.ntasks, ln.nblocks,

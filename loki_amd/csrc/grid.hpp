@@ -43,6 +43,23 @@ LOKI_HD inline uint32_t phase_bin(double proper_time, double freq, uint32_t nbin
     return b;
 }
 
+/// Unwrapped phase bin: floor(x) * nbins + w with x = t * freq and w the bin of
+/// phase_bin before its nbins -> 0 wrap (w in [0, nbins]), computed with the
+/// same FP64/FP32 operations.  phase_bin(t) == U mod nbins, and U is
+/// non-decreasing in t (every step is a monotone rounding), so the samples of
+/// one bin form contiguous runs, one per value of U.
+LOKI_HD inline int64_t unwrapped_bin(double proper_time, double freq, uint32_t nbins) {
+    const double total = (proper_time - 0.0) * freq;
+    const double fl    = floor(total);
+    const double norm  = total - fl;
+    const double iphase = norm * static_cast<double>(nbins);
+    uint32_t w = nbins;
+    if (iphase < static_cast<double>(nbins)) {
+        w = static_cast<uint32_t>(static_cast<float>(iphase) + 0.5F);
+    }
+    return static_cast<int64_t>(fl) * static_cast<int64_t>(nbins) + static_cast<int64_t>(w);
+}
+
 /// kernels.cpp:78-85 / kernels_cuda.cu:305-312: integer shift from a float
 /// phase shift, u32(shift + 0.5f), nbins -> 0.
 LOKI_HD inline uint32_t shift_bins(float shift, uint32_t nbins) {

@@ -98,7 +98,8 @@ k_bf_fill(const double* __restrict__ freqs, uint32_t seglen, uint32_t nbins, dou
 
 // One lane per (segment, freq, bin) pair; consecutive lanes are consecutive
 // bins, so list reads and output writes are contiguous.  USE_LDS: the
-// segment's samples are staged in LDS (seglen <= kLdsSamps).
+// segment's samples are staged in LDS (seglen <= kLdsSamps; longer segments
+// take the run path above).
 // TO: stored element type of the fold (float, or _Float16 / __bf16 with each
 // FP32 sum multiplied by `scale` before rounding).
 template <bool USE_LDS, typename TO>
@@ -167,6 +168,168 @@ __device__ __forceinline__ void bf_fold_block(uint32_t bid, const float* __restr
     }
 }
 
+// ---------------------------------------------------------------------------
+// Long segments: bucket sums over contiguous runs, no lists.
+// For one frequency the unwrapped bin U(k) (grid::unwrapped_bin) is
+// non-decreasing in the sample index k, so bin b's samples are the runs
+// [K(u), K(u + 1)) for u = b (mod nbins), K(u) = min{k : U(k) >= u}.  Each lane
+// finds its run boundaries from an analytic estimate corrected against the
+// exact U (a binary search if the estimate is far off) and sums the runs in
+// ascending k: the same elements in the same order as the bucket list.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int64_t u_at(uint32_t k, double tsamp, double t_ref, double f,
+                                        uint32_t nbins) {
+    const double t = (static_cast<double>(k) * tsamp) - t_ref;
+    return grid::unwrapped_bin(t, f, nbins);
+}
+
+/// K(u) = min{k in [0, n] : U(k) >= u} (n if none).
+__device__ __forceinline__ uint32_t run_bound(int64_t u, uint32_t n, double tsamp, double t_ref,
+                                              double f, uint32_t nbins) {
+    const double est = ((static_cast<double>(u) - 0.5) / (static_cast<double>(nbins) * f) +
+                        t_ref) / tsamp;
+    uint32_t k = 0;
+    if (est >= static_cast<double>(n)) {
+        k = n;
+    } else if (est > 0.0) {
+        k = static_cast<uint32_t>(ceil(est));
+        k = k > n ? n : k;
+    }
+    for (int it = 0; it < 4 && k > 0 && u_at(k - 1, tsamp, t_ref, f, nbins) >= u; ++it) {
+        --k;
+    }
+    for (int it = 0; it < 4 && k < n && u_at(k, tsamp, t_ref, f, nbins) < u; ++it) {
+        ++k;
+    }
+    const bool lo_ok = k == 0 || u_at(k - 1, tsamp, t_ref, f, nbins) < u;
+    const bool hi_ok = k == n || u_at(k, tsamp, t_ref, f, nbins) >= u;
+    if (lo_ok && hi_ok) {
+        return k;
+    }
+    uint32_t lo = 0, hi = n;  // invariant: answer in [lo, hi]
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (u_at(mid, tsamp, t_ref, f, nbins) >= u) {
+            hi = mid;
+        } else {
+            lo = mid + 1;
+        }
+    }
+    return lo;
+}
+
+// A block owns F * kBlock (freq, bin) pairs of one segment: lane l handles
+// pairs pbase + l + s * kBlock (s < F; F in {1, 2, 4, 8}, chosen per plan so
+// the grid still fills the GPU).  The segment streams
+// through LDS in chunks of kChunk samples that every slot of every lane reads,
+// so each sample leaves L2 once per block instead of once per frequency.  A
+// slot keeps its running sums and its current run [k0, k1) across chunks, so
+// every bucket is still summed in ascending sample order.
+template <typename TO, int kRunSlots>
+__device__ __forceinline__ void bf_runs_block(uint32_t bid, const float* __restrict__ ts_e,
+                                              const float* __restrict__ ts_v, uint32_t seglen,
+                                              uint32_t nfreqs, uint32_t nbins,
+                                              uint32_t blocks_per_seg,
+                                              const double* __restrict__ freqs, double tsamp,
+                                              double t_ref, TO* __restrict__ fold, float scale) {
+    __shared__ float s_e[kChunk];
+    __shared__ float s_v[kChunk];
+    const uint64_t seg   = bid / blocks_per_seg;
+    const uint32_t pbase = (bid % blocks_per_seg) * (kBlock * kRunSlots);
+    const uint32_t npair = nfreqs * nbins;
+    const float* __restrict__ ge = ts_e + seg * seglen;
+    const float* __restrict__ gv = ts_v + seg * seglen;
+    const int64_t nb = nbins;
+
+    float se[kRunSlots], sv[kRunSlots];
+    int64_t u[kRunSlots], u_end[kRunSlots];
+    uint32_t k0[kRunSlots], k1[kRunSlots];
+    double fq[kRunSlots];
+#pragma unroll
+    for (int sl = 0; sl < kRunSlots; ++sl) {
+        se[sl] = 0.0F;
+        sv[sl] = 0.0F;
+        u[sl]  = 1;
+        u_end[sl] = 0;  // inactive unless the pair exists
+        k0[sl] = seglen;
+        k1[sl] = seglen;
+        fq[sl] = 0.0;
+        const uint32_t p = pbase + threadIdx.x + static_cast<uint32_t>(sl) * kBlock;
+        if (p < npair) {
+            const uint32_t fi = p / nbins;
+            const uint32_t b  = p - fi * nbins;
+            const double f    = freqs[fi];
+            const int64_t u0  = u_at(0, tsamp, t_ref, f, nbins);
+            fq[sl]    = f;
+            u_end[sl] = u_at(seglen - 1, tsamp, t_ref, f, nbins);
+            u[sl]     = u0 + ((static_cast<int64_t>(b) - u0) % nb + nb) % nb;
+            if (u[sl] <= u_end[sl]) {
+                k0[sl] = run_bound(u[sl], seglen, tsamp, t_ref, f, nbins);
+                k1[sl] = run_bound(u[sl] + 1, seglen, tsamp, t_ref, f, nbins);
+            }
+        }
+    }
+    for (uint32_t c0 = 0; c0 < seglen; c0 += kChunk) {
+        const uint32_t c1 = min(c0 + static_cast<uint32_t>(kChunk), seglen);
+        __syncthreads();  // the previous chunk is no longer read
+        for (uint32_t k = c0 + threadIdx.x; k < c1; k += kBlock) {
+            s_e[k - c0] = ge[k];
+            s_v[k - c0] = gv[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int sl = 0; sl < kRunSlots; ++sl) {
+            while (u[sl] <= u_end[sl] && k0[sl] < c1) {
+                const uint32_t a = k0[sl] > c0 ? k0[sl] : c0;
+                const uint32_t z = k1[sl] < c1 ? k1[sl] : c1;
+                float ae = se[sl], av = sv[sl];
+                for (uint32_t k = a; k < z; ++k) {
+                    ae = ae + s_e[k - c0];
+                    av = av + s_v[k - c0];
+                }
+                se[sl] = ae;
+                sv[sl] = av;
+                if (k1[sl] > c1) {
+                    k0[sl] = c1;  // the run continues in the next chunk
+                    break;
+                }
+                u[sl] += nb;
+                if (u[sl] <= u_end[sl]) {
+                    k0[sl] = run_bound(u[sl], seglen, tsamp, t_ref, fq[sl], nbins);
+                    k1[sl] = run_bound(u[sl] + 1, seglen, tsamp, t_ref, fq[sl], nbins);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int sl = 0; sl < kRunSlots; ++sl) {
+        const uint32_t p = pbase + threadIdx.x + static_cast<uint32_t>(sl) * kBlock;
+        if (p >= npair) {
+            continue;
+        }
+        const uint32_t fi = p / nbins;
+        const uint32_t b  = p - fi * nbins;
+        TO* __restrict__ out = fold + (seg * nfreqs + fi) * 2 * nbins;
+        if constexpr (std::is_same_v<TO, float>) {
+            out[b]         = se[sl];
+            out[nbins + b] = sv[sl];
+        } else {
+            out[b]         = narrow1<TO>(se[sl] * scale);
+            out[nbins + b] = narrow1<TO>(sv[sl] * scale);
+        }
+    }
+}
+
+template <typename TO, int F>
+__global__ void __launch_bounds__(kBlock)
+k_bf_runs(const float* __restrict__ ts_e, const float* __restrict__ ts_v, uint32_t seglen,
+          uint32_t nfreqs, uint32_t nbins, uint32_t blocks_per_seg,
+          const double* __restrict__ freqs, double tsamp, double t_ref, TO* __restrict__ fold,
+          float scale) {
+    bf_runs_block<TO, F>(blockIdx.x, ts_e, ts_v, seglen, nfreqs, nbins, blocks_per_seg, freqs,
+                         tsamp, t_ref, fold, scale);
+}
+
 template <bool USE_LDS, typename TO>
 __global__ void __launch_bounds__(kBlock)
 k_bf_fold(const float* __restrict__ ts_e, const float* __restrict__ ts_v, uint32_t seglen,
@@ -197,6 +360,24 @@ k_bf_fold_grouped(const FoldTask* __restrict__ tasks, uint32_t ntasks) {
                                static_cast<TO*>(t.fold), t.scale);
 }
 
+template <typename TO, int F>
+__global__ void __launch_bounds__(kBlock)
+k_bf_runs_grouped(const FoldTask* __restrict__ tasks, uint32_t ntasks) {
+    uint32_t lo = 0;
+    uint32_t hi = ntasks;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (tasks[mid].block0 <= blockIdx.x) {
+            lo = mid;
+        } else {
+            hi = mid;
+        }
+    }
+    const FoldTask& t = tasks[lo];
+    bf_runs_block<TO, F>(blockIdx.x - t.block0, t.ts_e, t.ts_v, t.seglen, t.nfreqs, t.nbins,
+                         t.bps, t.freqs, t.tsamp, t.t_ref, static_cast<TO*>(t.fold), t.scale);
+}
+
 }  // namespace
 
 BruteFoldLists build_brute_fold_lists(const double* freqs_dev, uint32_t nfreqs, uint32_t seglen,
@@ -204,10 +385,23 @@ BruteFoldLists build_brute_fold_lists(const double* freqs_dev, uint32_t nfreqs, 
                                       hipStream_t s) {
     LOKI_REQUIRE(nbins >= 1 && nbins <= 1024, "brute fold supports 1 <= nbins <= 1024");
     LOKI_REQUIRE(nfreqs >= 1 && seglen >= 1, "brute fold: empty frequency or segment");
+    LOKI_REQUIRE(tsamp > 0.0, "brute fold: tsamp must be positive");
     BruteFoldLists L;
-    L.nfreqs = nfreqs;
-    L.seglen = seglen;
-    L.nbins  = nbins;
+    L.nfreqs   = nfreqs;
+    L.seglen   = seglen;
+    L.nbins    = nbins;
+    L.tsamp    = tsamp;
+    L.t_ref    = t_ref;
+    L.use_runs = seglen > static_cast<uint32_t>(kLdsSamps);
+    L.run_slots = 1;  // choose_run_slots() tunes it for the number of segments
+    L.freqs.reserve(static_cast<SizeType>(nfreqs) * sizeof(double));
+    hip_check(hipMemcpyAsync(L.freqs.get(), freqs_dev, nfreqs * sizeof(double),
+                             hipMemcpyDeviceToDevice, s),
+              "copy brute-fold frequencies");
+    if (L.use_runs) {
+        hip_check(hipStreamSynchronize(s), "brute-fold frequencies");
+        return L;
+    }
     L.counts.reserve(static_cast<SizeType>(nfreqs) * nbins * sizeof(uint32_t));
     DeviceBuffer kmax(static_cast<SizeType>(nfreqs) * sizeof(uint32_t));
     const uint32_t threads = ((nbins + 63) / 64) * 64;
@@ -243,12 +437,29 @@ namespace {
 template <typename TO>
 void launch_fold(const float* ts_e, const float* ts_v, uint64_t blocks, uint32_t bps,
                  const BruteFoldLists& L, TO* fold, float scale, hipStream_t s) {
-    if (L.seglen <= static_cast<uint32_t>(kLdsSamps)) {
-        k_bf_fold<true, TO><<<static_cast<unsigned>(blocks), kBlock, 0, s>>>(
-            ts_e, ts_v, L.seglen, L.nfreqs, L.nbins, bps, L.list.as<uint32_t>(),
-            L.base.as<uint64_t>(), L.counts.as<uint32_t>(), fold, scale);
+    if (L.use_runs) {
+        const auto g = static_cast<unsigned>(blocks);
+        const auto* fr = L.freqs.as<double>();
+        switch (L.run_slots) {
+        case 1:
+            k_bf_runs<TO, 1><<<g, kBlock, 0, s>>>(ts_e, ts_v, L.seglen, L.nfreqs, L.nbins, bps,
+                                                  fr, L.tsamp, L.t_ref, fold, scale);
+            break;
+        case 2:
+            k_bf_runs<TO, 2><<<g, kBlock, 0, s>>>(ts_e, ts_v, L.seglen, L.nfreqs, L.nbins, bps,
+                                                  fr, L.tsamp, L.t_ref, fold, scale);
+            break;
+        case 4:
+            k_bf_runs<TO, 4><<<g, kBlock, 0, s>>>(ts_e, ts_v, L.seglen, L.nfreqs, L.nbins, bps,
+                                                  fr, L.tsamp, L.t_ref, fold, scale);
+            break;
+        default:
+            k_bf_runs<TO, 8><<<g, kBlock, 0, s>>>(ts_e, ts_v, L.seglen, L.nfreqs, L.nbins, bps,
+                                                  fr, L.tsamp, L.t_ref, fold, scale);
+            break;
+        }
     } else {
-        k_bf_fold<false, TO><<<static_cast<unsigned>(blocks), kBlock, 0, s>>>(
+        k_bf_fold<true, TO><<<static_cast<unsigned>(blocks), kBlock, 0, s>>>(
             ts_e, ts_v, L.seglen, L.nfreqs, L.nbins, bps, L.list.as<uint32_t>(),
             L.base.as<uint64_t>(), L.counts.as<uint32_t>(), fold, scale);
     }
@@ -259,9 +470,7 @@ void launch_fold(const float* ts_e, const float* ts_v, uint64_t blocks, uint32_t
 void brute_fold_time(const float* ts_e, const float* ts_v, uint64_t nseg,
                      const BruteFoldLists& L, void* fold, FoldStorage so, float scale,
                      hipStream_t s) {
-    const uint64_t pairs = static_cast<uint64_t>(L.nfreqs) * L.nbins;
-    LOKI_REQUIRE(pairs < (1ULL << 31), "brute fold: too many (freq, bin) pairs");
-    const uint32_t bps    = static_cast<uint32_t>((pairs + kPairsBlk - 1) / kPairsBlk);
+    const uint32_t bps    = brute_fold_blocks_per_segment(L);
     const uint64_t blocks = nseg * bps;
     LOKI_REQUIRE(blocks < (1ULL << 31), "brute fold grid too large");
     if (blocks == 0) {
@@ -277,41 +486,49 @@ void brute_fold_time(const float* ts_e, const float* ts_v, uint64_t nseg,
     }
 }
 
-uint32_t brute_fold_blocks_per_segment(uint32_t nfreqs, uint32_t nbins) {
-    const uint64_t pairs = static_cast<uint64_t>(nfreqs) * nbins;
+uint32_t brute_fold_blocks_per_segment(const BruteFoldLists& L) {
+    const uint64_t pairs = static_cast<uint64_t>(L.nfreqs) * L.nbins;
     LOKI_REQUIRE(pairs < (1ULL << 31), "brute fold: too many (freq, bin) pairs");
-    return static_cast<uint32_t>((pairs + kPairsBlk - 1) / kPairsBlk);
+    const uint64_t per = L.use_runs ? static_cast<uint64_t>(kBlock) * L.run_slots : kPairsBlk;
+    return static_cast<uint32_t>((pairs + per - 1) / per);
 }
 
-bool brute_fold_uses_lds(uint32_t seglen) { return seglen <= static_cast<uint32_t>(kLdsSamps); }
+void choose_run_slots(BruteFoldLists& L, uint64_t nseg) {
+    L.run_slots = 1;
+    if (!L.use_runs) {
+        return;
+    }
+    const uint64_t pairs = static_cast<uint64_t>(L.nfreqs) * L.nbins;
+    for (uint32_t f : {8U, 4U, 2U}) {
+        const uint64_t per = static_cast<uint64_t>(kBlock) * f;
+        if (nseg * ((pairs + per - 1) / per) >= 1024) {
+            L.run_slots = f;
+            return;
+        }
+    }
+}
 
-void brute_fold_grouped(const FoldTask* tasks, uint32_t ntasks, uint32_t nblocks, bool use_lds,
-                        FoldStorage so, hipStream_t s) {
+template <typename TO>
+void grouped_fold(const FoldTask* tasks, uint32_t ntasks, uint32_t nblocks, uint32_t run_slots,
+                  hipStream_t s) {
+    switch (run_slots) {
+    case 0: k_bf_fold_grouped<true, TO><<<nblocks, kBlock, 0, s>>>(tasks, ntasks); break;
+    case 1: k_bf_runs_grouped<TO, 1><<<nblocks, kBlock, 0, s>>>(tasks, ntasks); break;
+    case 2: k_bf_runs_grouped<TO, 2><<<nblocks, kBlock, 0, s>>>(tasks, ntasks); break;
+    case 4: k_bf_runs_grouped<TO, 4><<<nblocks, kBlock, 0, s>>>(tasks, ntasks); break;
+    default: k_bf_runs_grouped<TO, 8><<<nblocks, kBlock, 0, s>>>(tasks, ntasks); break;
+    }
+}
+
+void brute_fold_grouped(const FoldTask* tasks, uint32_t ntasks, uint32_t nblocks,
+                        uint32_t run_slots, FoldStorage so, hipStream_t s) {
     if (ntasks == 0 || nblocks == 0) {
         return;
     }
     switch (so) {
-    case FoldStorage::kF32:
-        if (use_lds) {
-            k_bf_fold_grouped<true, float><<<nblocks, kBlock, 0, s>>>(tasks, ntasks);
-        } else {
-            k_bf_fold_grouped<false, float><<<nblocks, kBlock, 0, s>>>(tasks, ntasks);
-        }
-        break;
-    case FoldStorage::kF16:
-        if (use_lds) {
-            k_bf_fold_grouped<true, _Float16><<<nblocks, kBlock, 0, s>>>(tasks, ntasks);
-        } else {
-            k_bf_fold_grouped<false, _Float16><<<nblocks, kBlock, 0, s>>>(tasks, ntasks);
-        }
-        break;
-    case FoldStorage::kBF16:
-        if (use_lds) {
-            k_bf_fold_grouped<true, __bf16><<<nblocks, kBlock, 0, s>>>(tasks, ntasks);
-        } else {
-            k_bf_fold_grouped<false, __bf16><<<nblocks, kBlock, 0, s>>>(tasks, ntasks);
-        }
-        break;
+    case FoldStorage::kF32: return grouped_fold<float>(tasks, ntasks, nblocks, run_slots, s);
+    case FoldStorage::kF16: return grouped_fold<_Float16>(tasks, ntasks, nblocks, run_slots, s);
+    case FoldStorage::kBF16: return grouped_fold<__bf16>(tasks, ntasks, nblocks, run_slots, s);
     }
 }
 

@@ -41,10 +41,17 @@ void resolve_poly(const ResolveLevel& lv, FFACoord* out, uint64_t ncoords, hipSt
 /// Bucket lists of the time-domain brute fold, built once per plan: for
 /// each frequency f and phase bin b, the sample indices of the bin in
 /// ascending order, stored transposed: list[base[f] + k * nbins + b].
+/// Segments longer than the LDS staging limit use no lists: their buckets are
+/// walked as contiguous runs of samples (use_runs).
 struct BruteFoldLists {
     uint32_t nfreqs{0};
     uint32_t seglen{0};
     uint32_t nbins{0};
+    bool use_runs{false};
+    uint32_t run_slots{1};  ///< run path: (freq, bin) pairs per lane (1, 2, 4 or 8)
+    double tsamp{0.0};
+    double t_ref{0.0};
+    DeviceBuffer freqs;   ///< f64 [nfreqs]
     DeviceBuffer counts;  ///< u32 [nfreqs * nbins]
     DeviceBuffer base;    ///< u64 [nfreqs]
     DeviceBuffer list;    ///< u32 [sum_f kmax_f * nbins]
@@ -125,9 +132,11 @@ struct ResolveTask {
 struct FoldTask {
     const float* ts_e;
     const float* ts_v;
-    const uint32_t* list;
+    const uint32_t* list;    ///< bucket lists (list path)
     const uint64_t* base;
     const uint32_t* counts;
+    const double* freqs;     ///< frequencies (run path)
+    double tsamp, t_ref;
     void* fold;
     float scale;
     uint32_t seglen, nfreqs, nbins, bps;  ///< bps: blocks per segment
@@ -158,13 +167,16 @@ struct FinalTask {
 /// Grouped FP64 shift tables (np == 1 or poly, one kind per launch).
 void resolve_grouped(const ResolveTask* tasks, uint32_t ntasks, uint32_t nblocks, uint32_t np,
                      hipStream_t s);
-/// Blocks per segment of the time brute fold, and whether a segment of
-/// seglen samples is staged in LDS (tasks of one grouped launch must agree).
-uint32_t brute_fold_blocks_per_segment(uint32_t nfreqs, uint32_t nbins);
-bool brute_fold_uses_lds(uint32_t seglen);
-/// Grouped time brute fold (all tasks stored as `so`).
-void brute_fold_grouped(const FoldTask* tasks, uint32_t ntasks, uint32_t nblocks, bool use_lds,
-                        FoldStorage so, hipStream_t s);
+/// Blocks per segment of the time brute fold of `lists` (list or run path;
+/// tasks of one grouped launch must share the path and run_slots).
+uint32_t brute_fold_blocks_per_segment(const BruteFoldLists& lists);
+/// Sets lists.run_slots for a fold of nseg segments: the most pairs per lane
+/// (up to 8) that keeps at least 1024 blocks.
+void choose_run_slots(BruteFoldLists& lists, uint64_t nseg);
+/// Grouped time brute fold (all tasks stored as `so`); run_slots 0: list
+/// path, else the run path with that many pairs per lane.
+void brute_fold_grouped(const FoldTask* tasks, uint32_t ntasks, uint32_t nblocks,
+                        uint32_t run_slots, FoldStorage so, hipStream_t s);
 /// Grouped merges of one storage type, nbins and coordinate kind.
 void merge_grouped(const MergeTask* tasks, uint32_t ntasks, uint32_t nblocks, uint32_t nbins,
                    bool poly, FoldStorage st, hipStream_t s);

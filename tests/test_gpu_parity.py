@@ -96,11 +96,26 @@ def test_resolve_coordinates_bit_exact(name):
     [(64, 2**13, 0.0), (32, 1024, 0.0328), (100, 4096, 0.0), (128, 2**14, 0.5), (7, 2048, 0.1)],
 )
 def test_brute_fold_time_bit_exact(oracle, nbins, segment_len, t_ref):
+    _check_brute_fold(oracle, nbins, segment_len, t_ref, np.linspace(140.0, 145.0, 50))
+
+
+# Segments longer than 4096 samples take the run path (contiguous runs of one phase bin,
+# boundaries found from the exact FP64 bin function): long periods, periods shorter than nbins
+# (empty runs), less than one period per segment, and negative phases (t_ref > 0).
+@pytest.mark.parametrize(("nbins", "segment_len", "t_ref", "f_lo", "f_hi", "nf"), [
+    (64, 2**16, 0.0, 0.5, 2.0, 23), (128, 2**15, 1.1, 0.7, 1.3, 9), (32, 2**13, 0.0, 3000.0,
+    5000.0, 7), (64, 2**16, 0.0, 0.01, 0.05, 5), (16, 2**17, 0.37, 9.9, 10.1, 5),
+    (100, 8192, 0.0, 1.0, 40.0, 11),
+])
+def test_brute_fold_time_run_path_bit_exact(oracle, nbins, segment_len, t_ref, f_lo, f_hi, nf):
+    _check_brute_fold(oracle, nbins, segment_len, t_ref, np.linspace(f_lo, f_hi, nf))
+
+
+def _check_brute_fold(oracle, nbins, segment_len, t_ref, freqs):
     """Brute fold vs the ascending-order bucket sums (fold.cpp:113-142, kernels.cpp:550-580)."""
     from loki_amd import libculoki
 
     ts_e, ts_v = _series(2**17, seed=7)
-    freqs = np.linspace(140.0, 145.0, 50)
     got = libculoki.fold.compute_brute_fold_time_cuda(
         ts_e, ts_v, freqs, segment_len=segment_len, nbins=nbins, tsamp=TSAMP, t_ref=t_ref,
         device_id=0)
