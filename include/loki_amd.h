@@ -308,6 +308,31 @@ int loki_snr_boxcar_2d_max_host(const float* folds, size_t nprof, size_t nbins,
                                 const size_t* widths, size_t nwidths, float stdnoise,
                                 float* scores, int device_id);
 
+/* --- Differentiable candidate refinement -------------------------------- */
+/* No counterpart in the reference: a continuous-parameter generalisation of the Fourier brute
+ * fold (kernels_cuda.cu:780-853, one segment spanning the series) with the Taylor phase model of
+ * transforms.cpp:101-120, the inverse real DFT of fft_cuda.cu:357-448 and the fixed-width boxcar
+ * S/N of score.cpp:95-172, plus its adjoint.  theta: ncand x (f [Hz], fdot [Hz/s],
+ * fddot [Hz/s^2]) FP64 on the device, phase phi = f tau + fdot tau^2/2 + fddot tau^3/6 with
+ * tau = k tsamp - t_ref.  nbins in {16, 32, 64, 128}; width in [1, nbins); 1 <= ncand < 65536.
+ * All pointers are device pointers; calls are asynchronous on `stream`. */
+
+/* Bytes of device workspace either call needs. */
+size_t loki_refine_workspace_bytes(size_t nsamps, size_t ncand, size_t nbins);
+/* scores [ncand] float; adj_state (may be NULL) [ncand][2][nbins/2+1] complex64: the adjoint
+ * seed dS/dE_m, dS/dV_m that loki_refine_grad_device consumes. */
+int loki_refine_score_device(const float* ts_e, const float* ts_v, size_t nsamps, double tsamp,
+                             double t_ref, const double* theta, size_t ncand, size_t nbins,
+                             size_t width, float* scores, float* adj_state, void* workspace,
+                             size_t workspace_bytes, void* stream);
+/* grad [ncand][3] FP64 = dS/d(f, fdot, fddot) at theta, from the adj_state of a score call at
+ * the same theta (the fold is recomputed, not stored).  The boxcar position is held at the
+ * forward's maximum. */
+int loki_refine_grad_device(const float* ts_e, const float* ts_v, size_t nsamps, double tsamp,
+                            double t_ref, const double* theta, size_t ncand, size_t nbins,
+                            const float* adj_state, double* grad, void* workspace,
+                            size_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -709,6 +709,56 @@ int loki_rfft_batch_device(const float* real_in, float* complex_out, size_t batc
     });
 }
 
+size_t loki_refine_workspace_bytes(size_t nsamps, size_t ncand, size_t nbins) {
+    return loki_amd::kern::refine_workspace_bytes(nsamps, ncand, static_cast<uint32_t>(nbins));
+}
+
+int loki_refine_score_device(const float* ts_e, const float* ts_v, size_t nsamps, double tsamp,
+                             double t_ref, const double* theta, size_t ncand, size_t nbins,
+                             size_t width, float* scores, float* adj_state, void* workspace,
+                             size_t workspace_bytes, void* stream) {
+    return guarded([&] {
+        REQ(ncand >= 1 && ncand < 65536, "refine: 1 <= candidates < 65536");
+        REQ(ts_e != nullptr && ts_v != nullptr && theta != nullptr && scores != nullptr &&
+                workspace != nullptr,
+            "null argument");
+        REQ(nbins <= 128, "refine: nbins must be 16, 32, 64 or 128");
+        REQ(workspace_bytes >= loki_amd::kern::refine_workspace_bytes(
+                                   nsamps, ncand, static_cast<uint32_t>(nbins)),
+            "refine: workspace_bytes too small");
+        REQ(tsamp > 0.0, "refine: tsamp must be positive");
+        loki_amd::kern::refine_forward(ts_e, ts_v, nsamps, tsamp, t_ref, theta, ncand,
+                                       static_cast<uint32_t>(nbins),
+                                       static_cast<uint32_t>(width < 1024 ? width : 1024), scores,
+                                       reinterpret_cast<float2*>(adj_state), workspace,
+                                       as_stream(stream));
+        loki_amd::hip_check(hipGetLastError(), "refine score launch");
+    });
+}
+
+int loki_refine_grad_device(const float* ts_e, const float* ts_v, size_t nsamps, double tsamp,
+                            double t_ref, const double* theta, size_t ncand, size_t nbins,
+                            const float* adj_state, double* grad, void* workspace,
+                            size_t workspace_bytes, void* stream) {
+    return guarded([&] {
+        REQ(ncand >= 1 && ncand < 65536, "refine: 1 <= candidates < 65536");
+        REQ(ts_e != nullptr && ts_v != nullptr && theta != nullptr && adj_state != nullptr &&
+                grad != nullptr && workspace != nullptr,
+            "null argument");
+        REQ(nbins <= 128, "refine: nbins must be 16, 32, 64 or 128");
+        REQ(nsamps >= 1, "refine: empty series");
+        REQ(workspace_bytes >= loki_amd::kern::refine_workspace_bytes(
+                                   nsamps, ncand, static_cast<uint32_t>(nbins)),
+            "refine: workspace_bytes too small");
+        REQ(tsamp > 0.0, "refine: tsamp must be positive");
+        loki_amd::kern::refine_adjoint(ts_e, ts_v, nsamps, tsamp, t_ref, theta, ncand,
+                                       static_cast<uint32_t>(nbins),
+                                       reinterpret_cast<const float2*>(adj_state), grad,
+                                       workspace, as_stream(stream));
+        loki_amd::hip_check(hipGetLastError(), "refine grad launch");
+    });
+}
+
 int loki_snr_boxcar_3d_device(const float* folds, size_t nprof, size_t nbins,
                               const size_t* widths, size_t nwidths, float* scores, int find_max,
                               void* stream) {

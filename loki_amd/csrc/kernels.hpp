@@ -95,6 +95,21 @@ void irfft_rows(const float2* in, float* out, uint64_t nrows, uint32_t nbins, hi
 /// Forward real DFT (unnormalised) of nrows rows of nbins floats.
 void rfft_rows(const float* in, float2* out, uint64_t nrows, uint32_t nbins, hipStream_t s);
 
+/// Workspace bytes of refine_forward / refine_adjoint (either may reuse it).
+SizeType refine_workspace_bytes(uint64_t nsamps, uint64_t ncand, uint32_t nbins);
+/// Continuous Fourier-fold score of ncand candidates theta [ncand][3] =
+/// (f, fdot, fddot) (FP64, device) over the whole series: scores [ncand] and,
+/// when state != nullptr, the adjoint state [ncand][2][nbins/2+1] float2.
+/// nbins in {16, 32, 64, 128}, width in [1, nbins).
+void refine_forward(const float* ts_e, const float* ts_v, uint64_t nsamps, double tsamp,
+                    double t_ref, const double* theta, uint64_t ncand, uint32_t nbins,
+                    uint32_t width, float* scores, float2* state, void* workspace,
+                    hipStream_t s);
+/// dS/dtheta [ncand][3] (FP64) from the state of refine_forward at the same theta.
+void refine_adjoint(const float* ts_e, const float* ts_v, uint64_t nsamps, double tsamp,
+                    double t_ref, const double* theta, uint64_t ncand, uint32_t nbins,
+                    const float2* state, double* grad, void* workspace, hipStream_t s);
+
 /// Boxcar widths with the reference's template constants (score.cpp:107-116).
 struct WidthTable {
     static constexpr uint32_t kMax = 64;
