@@ -49,6 +49,12 @@ WORKLOADS = {
         param_limits=[(-299792458.0 * 1e-14 / 50.0, 299792458.0 * 1e-14 / 50.0),
                       (-299792458.0 * 1e-8 / 50.0, 299792458.0 * 1e-8 / 50.0), (50.0, 500.0)],
         signal=dict(f0=251.3, accel=0.0, jerk=0.0, fwhm=0.05, snr=15.0), seed=5150,
+        # 64 injections (f in [50, 500] Hz, |fdot| <= 1e-8 Hz/s, |fddot| <= 1e-14 Hz/s^2,
+        # S/N 10-25); the 4096 strongest grid peaks are refined by gradient ascent of the
+        # fixed-width S/N in (f, fdot, fddot) (loki_amd.refine).
+        injections=dict(n=64, seed=5151, f=(50.0, 500.0), fdot=1e-8, fddot=1e-14,
+                        snr=(10.0, 25.0), fwhm=0.05),
+        refine=dict(ncand=4096, iters=10),
         cpu_sample=dict(param_limits=[(-299792458.0 * 1e-14 / 50.0, 299792458.0 * 1e-14 / 50.0),
                                       (-299792458.0 * 1e-8 / 50.0, 299792458.0 * 1e-8 / 50.0),
                                       (496.0, 500.0)], bseg_brute=2048),
@@ -84,6 +90,8 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--no-storage-compare", action="store_true",
                     help="skip timing the FP16 / BF16 storage variants beside the main run")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cpu count)")
+    ap.add_argument("--no-refine", action="store_true",
+                    help="Fourier workload: skip the gradient-ascent refinement leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES (3 times): the "
                          "reference kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -179,6 +187,124 @@ def cpu_baseline(wl: dict, ts_e, ts_v, widths, threads: int) -> dict:
                   f"{ocfg.bseg_brute}: {nc} final trials x {len(widths)} widths in {dt:.2f} s"
                   + note,
     }
+
+
+VALU_F32_PEAK_TFLOPS = 157.3
+REFINE_FLOPS_PER_TERM = 14.0  # twiddle power step (2 mul + 2 FMA) + 4 FMAs, per harmonic
+
+
+def grid_peaks(best, ncand):
+    """Indices of the ncand strongest local maxima (>= both neighbours) of a 1-D score row."""
+    left = np.concatenate([[-np.inf], best[:-1]])
+    right = np.concatenate([best[1:], [-np.inf]])
+    idx = np.nonzero((best >= left) & (best >= right))[0]
+    idx = idx[np.argsort(-best[idx], kind="stable")][:ncand]
+    return idx
+
+
+def run_refine(wl, plan, scores, widths, d_e, d_v, injections):
+    """Refines the strongest grid peaks of a Fourier-domain FFA by gradient ascent of the
+    fixed-width S/N (each candidate keeps its grid-best width); timed with events on the
+    current stream.  Returns the bench's "refine" object."""
+    import torch
+
+    from loki_amd import refine
+
+    rcfg = wl["refine"]
+    nw = len(widths)
+    nsamps, tsamp = wl["nsamps"], wl["tsamp"]
+    t_ref = 0.5 * nsamps * tsamp
+    sc = scores.view(-1, nw).cpu().numpy()
+    best_w = sc.argmax(axis=1)
+    best = sc.max(axis=1)
+    freqs = np.asarray(plan.params_dict["freq"], np.float64)
+    nf = freqs.size
+    best_f = best.reshape(-1, nf).max(axis=0)      # collapse the (sub-cell) fdot/fddot axes
+    arg_f = best.reshape(-1, nf).argmax(axis=0)
+    idx = grid_peaks(best_f, rcfg["ncand"])
+    widx = best_w.reshape(-1, nf)[arg_f[idx], idx]
+    theta0 = np.zeros((idx.size, 3))
+    theta0[:, 0] = freqs[idx]
+    th_out = np.zeros_like(theta0)
+    s0_out = np.zeros(idx.size)
+    s1_out = np.zeros(idx.size)
+    ev = []
+    total_ms = 0.0
+    evals = 0
+    nbins = wl["nbins"]
+    for wi in np.unique(widx):
+        sel = np.nonzero(widx == wi)[0]
+        dth = torch.from_numpy(theta0[sel]).to(d_e.device)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        res = refine.refine_candidates(d_e, d_v, dth, tsamp, t_ref, nbins, int(widths[wi]),
+                                       iters=rcfg["iters"])
+        e1.record()
+        torch.cuda.synchronize()
+        total_ms += e0.elapsed_time(e1)
+        evals += res.evaluations * sel.size
+        th_out[sel] = res.theta.cpu().numpy()
+        s0_out[sel] = res.scores0.cpu().numpy()
+        s1_out[sel] = res.scores.cpu().numpy()
+        ev.append((int(widths[wi]), int(sel.size)))
+    # one forward + one adjoint over all candidates at one width, timed on their own
+    ws = torch.empty(refine.workspace_bytes(nsamps, idx.size, nbins), dtype=torch.uint8,
+                     device=d_e.device)
+    dth = torch.from_numpy(th_out).to(d_e.device)
+    w0 = int(widths[np.bincount(widx).argmax()])
+    refine.score_raw(d_e, d_v, dth, tsamp, t_ref, nbins, w0, True, ws)
+    ea, eb, ec = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    ea.record()
+    _, st = refine.score_raw(d_e, d_v, dth, tsamp, t_ref, nbins, w0, True, ws)
+    eb.record()
+    refine.grad_raw(d_e, d_v, dth, st, tsamp, t_ref, nbins, ws)
+    ec.record()
+    torch.cuda.synchronize()
+    fwd_ms, adj_ms = ea.elapsed_time(eb), eb.elapsed_time(ec)
+    terms = float(nsamps) * idx.size * (nbins // 2 + 1)
+    # injections matched to the refined candidate nearest in f (within one grid cell)
+    df = float(freqs[1] - freqs[0]) if nf > 1 else 1.0
+    rec = []
+    for sg in injections:
+        j = int(np.argmin(np.abs(th_out[:, 0] - sg["f0"])))
+        ok = abs(th_out[j, 0] - sg["f0"]) <= df
+        rec.append((ok, s0_out[j], s1_out[j], th_out[j, 0] - sg["f0"]))
+    nrec = sum(r[0] for r in rec)
+    gain = [r[2] - r[1] for r in rec if r[0]]
+    flops = REFINE_FLOPS_PER_TERM * terms
+    return {
+        "ncand": int(idx.size), "iters": rcfg["iters"], "width_groups": ev,
+        "ms_total": total_ms, "evaluations": evals,
+        "forward_ms": fwd_ms, "adjoint_ms": adj_ms,
+        "harmonic_terms_per_pass": terms,
+        "roofline": {"bound": "valu", "kernel": "refine_forward", "unit": "TFLOP/s",
+                     "achieved": flops / (fwd_ms * 1e-3) / 1e12, "peak": VALU_F32_PEAK_TFLOPS,
+                     "frac": flops / (fwd_ms * 1e-3) / 1e12 / VALU_F32_PEAK_TFLOPS,
+                     "adjoint_achieved": flops / (adj_ms * 1e-3) / 1e12,
+                     "flops_per_term": REFINE_FLOPS_PER_TERM},
+        "injections_recovered": int(nrec), "injections": len(injections),
+        "median_snr_gain": float(np.median(gain)) if gain else None,
+        "median_abs_df_hz": float(np.median([abs(r[3]) for r in rec if r[0]])) if nrec else None,
+        "snr_never_decreased": bool(np.all(s1_out >= s0_out)),
+    }
+
+
+def refine_cpu_baseline(wl, ts_e, ts_v) -> dict:
+    """FP64 numpy adjoint (oracle/refine.py) on a bounded sample: 2 candidates over the first
+    2^18 samples, in harmonic terms per second (one forward + one adjoint each)."""
+    from oracle import refine as oref
+
+    n = 1 << 18
+    nbins = wl["nbins"]
+    t_ref = 0.5 * n * wl["tsamp"]
+    t0 = time.perf_counter()
+    for f in (100.0, 300.0):
+        oref.grad(ts_e[:n], ts_v[:n], wl["tsamp"], t_ref, (f, 0.0, 0.0), nbins, 4)
+    dt = time.perf_counter() - t0
+    terms = 2.0 * n * (nbins // 2 + 1)
+    return {"value": terms / dt, "unit": "harmonic terms/s (forward + adjoint)", "cores": 1,
+            "kind": "port", "sample": f"2 candidates x 2^18 samples x {nbins // 2 + 1} harmonics "
+                                      f"in {dt:.2f} s (FP64 numpy)"}
 
 
 def run_batch(args, wl) -> int:
@@ -310,9 +436,20 @@ def main() -> int:
     nbins = wl["nbins"]
     widths = np.array([w for w in WIDTHS if w < nbins], dtype=np.uint64)
     sig = wl["signal"]
-    ts_e, ts_v, _amp = make_series(wl["nsamps"], wl["tsamp"], wl["seed"], f0=sig["f0"],
-                                   accel=sig["accel"], jerk=sig["jerk"], fwhm=sig["fwhm"],
-                                   snr=sig["snr"], nbins=nbins, widths=widths)
+    injections = None
+    if "injections" in wl:
+        from loki_amd.synth import make_injected_series, make_injections
+
+        inj = wl["injections"]
+        injections = make_injections(inj["n"], inj["seed"], inj["f"], inj["fdot"], inj["fddot"],
+                                     inj["snr"], inj["fwhm"])
+        ts_e, ts_v, _amps = make_injected_series(wl["nsamps"], wl["tsamp"], wl["seed"],
+                                                 injections, nbins=nbins, widths=widths,
+                                                 device=torch.device("cuda:0"))
+    else:
+        ts_e, ts_v, _amp = make_series(wl["nsamps"], wl["tsamp"], wl["seed"], f0=sig["f0"],
+                                       accel=sig["accel"], jerk=sig["jerk"], fwhm=sig["fwhm"],
+                                       snr=sig["snr"], nbins=nbins, widths=widths)
     fourier = bool(wl.get("fourier", False))
     cfg = libloki.configs.PulsarSearchConfig(
         nsamps=wl["nsamps"], tsamp=wl["tsamp"], nbins=nbins, eta=wl["eta"],
@@ -446,9 +583,15 @@ def main() -> int:
         "found": found,
         "injected": {"freq": sig["f0"], "accel": sig["accel"], "snr": sig["snr"]},
     }
+    if injections is not None:
+        result["injected"] = {"n": len(injections), "spec": wl["injections"]}
+        if not args.no_refine:
+            result["refine"] = run_refine(wl, plan, scores, widths, d_e, d_v, injections)
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         result["cpu_baseline"] = cpu_baseline(wl, ts_e, ts_v, widths, threads)
+        if "refine" in result:
+            result["refine"]["cpu_baseline"] = refine_cpu_baseline(wl, ts_e, ts_v)
     print(json.dumps(result), flush=True)
     return 0
 

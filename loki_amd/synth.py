@@ -72,6 +72,60 @@ def make_series(nsamps: int, tsamp: float, seed: int, f0: float | None = None,
     return ts_e, ts_v, amp
 
 
+def make_injections(n: int, seed: int, f_range=(50.0, 500.0), fdot_max: float = 1e-8,
+                    fddot_max: float = 1e-14, snr_range=(10.0, 25.0), fwhm: float = 0.05):
+    """n signals with f, fdot, fddot and S/N drawn uniformly (FP64), sorted by f."""
+    rng = np.random.default_rng(seed)
+    sig = [dict(f0=float(rng.uniform(*f_range)), fdot=float(rng.uniform(-fdot_max, fdot_max)),
+                fddot=float(rng.uniform(-fddot_max, fddot_max)),
+                snr=float(rng.uniform(*snr_range)), fwhm=fwhm) for _ in range(n)]
+    return sorted(sig, key=lambda d: d["f0"])
+
+
+def make_injected_series(nsamps: int, tsamp: float, seed: int, signals, nbins: int = 64,
+                         widths=(1, 2, 3, 4, 6, 8, 12, 16, 24, 32), device=None):
+    """N(0,1) noise plus several pulses with the continuous ephemeris
+    phi = f0 tau + fdot tau^2 / 2 + fddot tau^3 / 6, tau = t - tobs / 2; each amplitude is
+    solved for that signal's S/N as in make_series.  Returns (ts_e, ts_v, amplitudes) as numpy
+    arrays.  With a torch ``device`` the FP64 phases and pulse shapes are evaluated there (same
+    arithmetic; the noise always comes from numpy's PCG64)."""
+    rng = np.random.default_rng(seed)
+    ts_e = rng.standard_normal(nsamps, dtype=np.float32)
+    ts_v = np.ones(nsamps, dtype=np.float32)
+    amps = []
+    if device is None:
+        tau = np.arange(nsamps, dtype=np.float64) * tsamp - 0.5 * nsamps * tsamp
+        acc = np.zeros(nsamps, np.float64)
+        for sg in signals:
+            ph = sg["f0"] * tau + sg["fdot"] * tau * tau / 2.0 + sg["fddot"] * tau**3 / 6.0
+            shape = gaussian_profile(ph, sg["fwhm"])
+            bins = np.floor((ph - np.floor(ph)) * nbins).astype(np.int64) % nbins
+            prof = np.bincount(bins, weights=shape, minlength=nbins)
+            cnt = np.bincount(bins, minlength=nbins).astype(np.float64)
+            unit = boxcar_snr_noise_free(prof, np.maximum(cnt, 1.0), widths)
+            amp = sg["snr"] / unit if unit > 0 else 0.0
+            acc += (amp * shape).astype(np.float32)
+            amps.append(amp)
+        return (ts_e + acc).astype(np.float32), ts_v, amps
+    import torch
+
+    tau = torch.arange(nsamps, dtype=torch.float64, device=device) * tsamp - 0.5 * nsamps * tsamp
+    acc = torch.zeros(nsamps, dtype=torch.float64, device=device)
+    for sg in signals:
+        ph = sg["f0"] * tau + sg["fdot"] * tau * tau / 2.0 + sg["fddot"] * tau**3 / 6.0
+        fr = ph - torch.floor(ph)
+        sigma = sg["fwhm"] / (2.0 * np.sqrt(2.0 * np.log(2.0)))
+        shape = torch.exp(-0.5 * ((fr - 0.5) / sigma) ** 2)
+        bins = torch.floor(fr * nbins).to(torch.int64) % nbins
+        prof = torch.bincount(bins, weights=shape, minlength=nbins).cpu().numpy()
+        cnt = torch.bincount(bins, minlength=nbins).to(torch.float64).cpu().numpy()
+        unit = boxcar_snr_noise_free(prof, np.maximum(cnt, 1.0), widths)
+        amp = sg["snr"] / unit if unit > 0 else 0.0
+        acc += (amp * shape).to(torch.float32).to(torch.float64)
+        amps.append(amp)
+    return (ts_e + acc.cpu().numpy()).astype(np.float32), ts_v, amps
+
+
 def make_batch_specs(nseries: int, seed: int, log2_len=(18, 22), nbins_choices=(32, 64, 128),
                      tsamp: float = 64e-6, period_centre=(0.3, 3.0), min_turns: float = 4.0):
     """Specs of a batch of independent period searches of uneven length.
