@@ -457,6 +457,7 @@ def main() -> int:
     dev = torch.device("cuda:0")
     if fourier:
         ffa = libculoki.ffa.FFACOMPLEXCUDA(cfg, device_id=0)
+        ffa.set_fused(not args.unfused)
     else:
         ffa = libculoki.ffa.FFACUDA(cfg, device_id=0)
         ffa.set_exact_norm(not args.fast_norm)
@@ -574,7 +575,8 @@ def main() -> int:
             "domain": "fourier" if fourier else "time",
             "fold_storage": "c64" if fourier else args.storage,
             "score_norm": "hw_rsqrt" if args.fast_norm else "exact",
-            "final_level": "unfused" if (args.unfused or fourier) else "fused",
+            "final_level": ("unfused" if args.unfused else
+                            "fused_irfft_score" if fourier else "fused"),
         },
         "roofline": roof,
         "pipeline_alg_GBps": total_alg / (ms_per_step * 1e-3) / 1e9,

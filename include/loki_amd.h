@@ -188,8 +188,8 @@ int loki_ffa_scores_host(loki_ffa_t ffa, const float* ts_e, const float* ts_v, s
  * launch stream around each kernel).  No reference counterpart: it replaces the reference's
  * host-side ScopeTimer / get_brute_fold_timing (ffa.hpp:52, ffa.cpp:74,204-239).
  * kind: 0 shift-table resolve, 1 brute fold, 2 merge, 3 score, 4 fused last level
- * (shift tables + merge + score), 5 real FFT between time and Fourier folds; level: FFA
- * level. */
+ * (shift tables + merge + score), 5 real FFT between time and Fourier folds, 6 fused inverse
+ * real DFT + score of the last Fourier level; level: FFA level. */
 typedef struct {
     int kind;
     int level;
@@ -198,8 +198,10 @@ typedef struct {
 int loki_ffa_set_timing(loki_ffa_t ffa, int enable);
 /* Scores path: 1 (default) fuses the last level's shift tables, merge and S/N into one kernel
  * (the final fold cube is never stored); 0 runs the unfused merge + snr_boxcar_3d.  Both give
- * identical results.  No reference counterpart (the reference always stores the final fold,
- * ffa_cuda.cu:670-692). */
+ * identical results.  For Fourier plans with nbins 16, 32 or 64, 1 fuses the inverse real DFT
+ * of the last level with the S/N (the time-domain final fold is never stored); both orders
+ * stay within the Fourier S/N bound, not bitwise equal.  No reference counterpart (the
+ * reference always stores the final fold, ffa_cuda.cu:670-692). */
 int loki_ffa_set_fused(loki_ffa_t ffa, int enable);
 /* Profile normalisation in the fused scores path: 1 (default) the CPU's correctly rounded
  * e * (1/sqrt(v)), bit-identical to the reference CPU; 0 the hardware reciprocal square root

@@ -411,6 +411,15 @@ void FFAExecutor::execute_scores(const float* ts_e, const float* ts_v,
     const void* last =
         run_levels(ts_e, ts_v, m_result.get(), m_internal.get(), L - 1, true, s);
     const float* fin = static_cast<const float*>(last);
+    if (m_fourier && m_fused && kern::irfft_score_supported(static_cast<uint32_t>(nbins))) {
+        const int slot =
+            m_timer.start(KernelKind::kTransformScore, static_cast<int>(L - 1), s);
+        kern::irfft_score(static_cast<const float2*>(last), ncoords, static_cast<uint32_t>(nbins),
+                          wt, scores, find_max, m_exact_norm, s);
+        check_launch("irfft+score launch");
+        m_timer.stop(slot, s);
+        return;
+    }
     if (m_fourier) {
         // Back to the time domain into the other ping-pong buffer: it holds
         // buffer_size complex = 2*buffer_size floats >= fold_size_time.

@@ -24,6 +24,7 @@ enum class KernelKind : int {
     kScore      = 3,
     kMergeScore = 4,  ///< fused last level (shift tables + merge + S/N)
     kTransform  = 5,  ///< real FFT between time and Fourier folds
+    kTransformScore = 6,  ///< fused inverse real DFT + S/N of the last Fourier level
 };
 
 struct KernelTiming {

@@ -208,6 +208,14 @@ uint32_t final_lds_stride(uint32_t nbins, uint32_t wmax);
 /// snr_boxcar_3d[_max]: folds [nprof, 2, nbins] -> scores [nprof, nw] or [nprof].
 void snr_boxcar_3d(const float* folds, uint64_t nprof, uint32_t nbins, const WidthTable& wt,
                    float* scores, bool find_max, hipStream_t s);
+
+/// Fused inverse real DFT (matrix cores) + boxcar S/N of final Fourier fold
+/// rows [ntrials, 2, nbins/2+1] float2 -> scores [ntrials, nw] or [ntrials];
+/// the time-domain fold is never stored.  nbins in {16, 32, 64}.
+bool irfft_score_supported(uint32_t nbins);
+void irfft_score(const float2* fold, uint64_t ntrials, uint32_t nbins, const WidthTable& wt,
+                 float* scores, bool find_max, bool exact, hipStream_t s);
+
 /// snr_boxcar_2d_max: folds [nprof, nbins] (already normalised) / stdnoise.
 void snr_boxcar_2d_max(const float* folds, uint64_t nprof, uint32_t nbins, const WidthTable& wt,
                        float stdnoise, float* scores, hipStream_t s);

@@ -314,12 +314,13 @@ public:
         check(loki_ffa_timings(m_h.get(), nullptr, 0, &n));
         std::vector<loki_kernel_timing> t(n);
         check(loki_ffa_timings(m_h.get(), t.data(), n, &n));
-        static const char* kinds[] = {"resolve", "brute_fold", "merge",
-                                      "score",   "merge_score", "transform"};
+        static const char* kinds[] = {"resolve", "brute_fold",  "merge",
+                                      "score",   "merge_score", "transform",
+                                      "transform_score"};
         py::list out;
         for (const auto& x : t) {
             out.append(
-                py::make_tuple(kinds[x.kind >= 0 && x.kind <= 5 ? x.kind : 0], x.level, x.ms));
+                py::make_tuple(kinds[x.kind >= 0 && x.kind <= 6 ? x.kind : 0], x.level, x.ms));
         }
         return out;
     }
@@ -758,6 +759,8 @@ PYBIND11_MODULE(_loki_amd, m) {
         .def("scores", &FFA::scores_host, py::arg("ts_e"), py::arg("ts_v"), py::arg("widths"),
              py::arg("find_max") = false)
         .def("set_timing", &FFA::set_timing, py::arg("enable"))
+        .def("set_fused", &FFA::set_fused, py::arg("enable"))
+        .def("set_exact_norm", &FFA::set_exact_norm, py::arg("exact"))
         .def("timings", &FFA::timings);
 
     m_ffa.def(

@@ -121,6 +121,36 @@ def test_ffa_scores_fourier_within_bound(oracle, name):
     assert np.all(d <= 1e-4 + 1e-5 * np.abs(want))
 
 
+@pytest.mark.parametrize("name", ["freq16", "acc17w", "acc_eta2_bseg", "jerk17w"])
+@pytest.mark.parametrize(("fused", "exact", "find_max"), [
+    (True, True, False), (True, True, True), (True, False, False), (False, True, False),
+    (False, False, True)])
+def test_fourier_scores_fused_and_unfused(oracle, name, fused, exact, find_max):
+    """The fused inverse-DFT + S/N kernel (nbins 16/32/64, matrix cores, no time-domain final
+    fold) and the unfused IRFFT + snr_boxcar_3d both stay within the Fourier S/N bound, with
+    exact or hardware-rsqrt normalisation, per width and max over widths."""
+    from loki_amd import libculoki
+
+    cfg, ocfg = _cfgs(name, use_fourier=True)
+    ts_e, ts_v = _series(cfg.nsamps, seed=19)
+    widths = np.array([w for w in (1, 2, 3, 5, 8, 13) if w < cfg.nbins], dtype=np.uint64)
+    ffa = libculoki.ffa.FFACOMPLEXCUDA(cfg, device_id=0)
+    ffa.set_fused(fused)
+    ffa.set_exact_norm(exact)
+    ffa.set_timing(True)
+    got = ffa.scores(ts_e, ts_v, widths, find_max)
+    kinds = {k for k, _, _ in ffa.timings()}
+    assert ("transform_score" in kinds) == (fused and cfg.nbins in (16, 32, 64)), kinds
+    want, _ = oracle.compute_ffa_scores(ts_e, ts_v, ocfg, widths=widths)
+    want = want.reshape(-1, widths.size)
+    if find_max:
+        want = want.max(axis=1)
+    got = got.reshape(want.shape)
+    d = np.abs(got.astype(np.float64) - want)
+    print(f"{name} fused={fused} exact={exact} max={find_max}: max |dS/N| = {d.max():.3e}")
+    assert np.all(d <= 1e-4 + 1e-5 * np.abs(want))
+
+
 def test_ffacomplexcuda_class_overloads(oracle):
     """FFACOMPLEXCUDA.execute: complex64 fold of buffer_size, or float fold of 2*buffer_size
     returned to the time domain (loki_templates_cuda.cuh:21-51)."""
