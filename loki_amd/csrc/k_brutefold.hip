@@ -183,11 +183,44 @@ __device__ __forceinline__ int64_t u_at(uint32_t k, double tsamp, double t_ref, 
     return grid::unwrapped_bin(t, f, nbins);
 }
 
-/// K(u) = min{k in [0, n] : U(k) >= u} (n if none).
+/// Per-frequency constants of run_bound: k_est(u) = (u - 1/2) A + B with
+/// A = 1 / (nbins f tsamp) samples per bin and B = t_ref / tsamp, and the bound
+/// eta (in bins) on the rounding of U(k) against nbins (k tsamp - t_ref) f:
+/// the FP32 conversion and +0.5F of the bin ((2 nbins + 1/2) 2^-24) plus the
+/// FP64 evaluation of the phase (3 * 2^-53 (n tsamp + |t_ref|) f nbins).
+struct RunGeom {
+    double f, A, B, eta_len;
+};
+
+__device__ __forceinline__ RunGeom run_geom(double f, uint32_t n, double tsamp, double t_ref,
+                                            uint32_t nbins) {
+    RunGeom g;
+    g.f             = f;
+    const double nb = static_cast<double>(nbins);
+    g.A             = 1.0 / (nb * f * tsamp);
+    g.B             = t_ref / tsamp;
+    const double eta = ((2.0 * nb) + 0.5) * 0x1p-24 +
+                       3.0 * 0x1p-53 * ((static_cast<double>(n) * tsamp) + fabs(t_ref)) * f * nb;
+    g.eta_len       = eta * g.A;  // in samples
+    return g;
+}
+
+/// K(u) = min{k in [0, n] : U(k) >= u} (n if none).  U(k) >= u exactly when
+/// nbins (k tsamp - t_ref) f >= u - 1/2 up to eta bins, i.e. k >= k_est(u) up to
+/// eta * A samples; when k_est is farther than 4x that (plus its own FP64
+/// rounding) from an integer, K = ceil(k_est) needs no check.  Otherwise the
+/// estimate is corrected against the exact U (a binary search if it is far).
 __device__ __forceinline__ uint32_t run_bound(int64_t u, uint32_t n, double tsamp, double t_ref,
-                                              double f, uint32_t nbins) {
-    const double est = ((static_cast<double>(u) - 0.5) / (static_cast<double>(nbins) * f) +
-                        t_ref) / tsamp;
+                                              const RunGeom& g, uint32_t nbins) {
+    const double est = ((static_cast<double>(u) - 0.5) * g.A) + g.B;
+    if (est > 0.0 && est < static_cast<double>(n)) {
+        const double fr = est - floor(est);
+        const double m  = (4.0 * g.eta_len) + 16.0 * 0x1p-53 * (est + 2.0 * fabs(g.B)) + 1e-9;
+        if (fr > m && fr < 1.0 - m) {
+            return static_cast<uint32_t>(ceil(est));
+        }
+    }
+    const double f = g.f;
     uint32_t k = 0;
     if (est >= static_cast<double>(n)) {
         k = n;
@@ -218,9 +251,13 @@ __device__ __forceinline__ uint32_t run_bound(int64_t u, uint32_t n, double tsam
     return lo;
 }
 
-// A block owns F * kBlock (freq, bin) pairs of one segment: lane l handles
-// pairs pbase + l + s * kBlock (s < F; F in {1, 2, 4, 8}, chosen per plan so
-// the grid still fills the GPU).  The segment streams
+// A block owns F * kBlock (freq, bin) pairs of one segment (F in {1, 2, 4, 8}
+// dividing nbins, chosen per plan so the grid still fills the GPU).  A lane
+// owns one frequency and the F bins b0 + s * nbins / F, spread evenly around
+// the cycle: when a cycle is longer than a chunk, a chunk covers a short arc of
+// bins, and spreading each lane's bins keeps most lanes busy in every chunk
+// (with a lane's bins adjacent or its slots at one bin, only the lanes of the
+// arc would work).  The segment streams
 // through LDS in chunks of kChunk samples that every slot of every lane reads,
 // so each sample leaves L2 once per block instead of once per frequency.  A
 // slot keeps its running sums and its current run [k0, k1) across chunks, so
@@ -232,11 +269,12 @@ __device__ __forceinline__ void bf_runs_block(uint32_t bid, const float* __restr
                                               uint32_t blocks_per_seg,
                                               const double* __restrict__ freqs, double tsamp,
                                               double t_ref, TO* __restrict__ fold, float scale) {
-    __shared__ float s_e[kChunk];
-    __shared__ float s_v[kChunk];
+    __shared__ float2 s_ev[kChunk];  // (e, v) interleaved: one 8-byte LDS read per sample
     const uint64_t seg   = bid / blocks_per_seg;
-    const uint32_t pbase = (bid % blocks_per_seg) * (kBlock * kRunSlots);
-    const uint32_t npair = nfreqs * nbins;
+    const uint32_t lpf   = nbins / kRunSlots;  // lanes per frequency
+    const uint32_t lg    = (bid % blocks_per_seg) * kBlock + threadIdx.x;
+    const uint32_t fi    = lg / lpf;
+    const uint32_t b0    = lg - fi * lpf;
     const float* __restrict__ ge = ts_e + seg * seglen;
     const float* __restrict__ gv = ts_v + seg * seglen;
     const int64_t nb = nbins;
@@ -244,7 +282,8 @@ __device__ __forceinline__ void bf_runs_block(uint32_t bid, const float* __restr
     float se[kRunSlots], sv[kRunSlots];
     int64_t u[kRunSlots], u_end[kRunSlots];
     uint32_t k0[kRunSlots], k1[kRunSlots];
-    double fq[kRunSlots];
+    // every slot of a lane has the lane's frequency
+    const RunGeom rg = run_geom(fi < nfreqs ? freqs[fi] : 1.0, seglen, tsamp, t_ref, nbins);
 #pragma unroll
     for (int sl = 0; sl < kRunSlots; ++sl) {
         se[sl] = 0.0F;
@@ -253,19 +292,15 @@ __device__ __forceinline__ void bf_runs_block(uint32_t bid, const float* __restr
         u_end[sl] = 0;  // inactive unless the pair exists
         k0[sl] = seglen;
         k1[sl] = seglen;
-        fq[sl] = 0.0;
-        const uint32_t p = pbase + threadIdx.x + static_cast<uint32_t>(sl) * kBlock;
-        if (p < npair) {
-            const uint32_t fi = p / nbins;
-            const uint32_t b  = p - fi * nbins;
-            const double f    = freqs[fi];
+        if (fi < nfreqs) {
+            const uint32_t b  = b0 + static_cast<uint32_t>(sl) * lpf;
+            const double f    = rg.f;
             const int64_t u0  = u_at(0, tsamp, t_ref, f, nbins);
-            fq[sl]    = f;
             u_end[sl] = u_at(seglen - 1, tsamp, t_ref, f, nbins);
             u[sl]     = u0 + ((static_cast<int64_t>(b) - u0) % nb + nb) % nb;
             if (u[sl] <= u_end[sl]) {
-                k0[sl] = run_bound(u[sl], seglen, tsamp, t_ref, f, nbins);
-                k1[sl] = run_bound(u[sl] + 1, seglen, tsamp, t_ref, f, nbins);
+                k0[sl] = run_bound(u[sl], seglen, tsamp, t_ref, rg, nbins);
+                k1[sl] = run_bound(u[sl] + 1, seglen, tsamp, t_ref, rg, nbins);
             }
         }
     }
@@ -273,8 +308,7 @@ __device__ __forceinline__ void bf_runs_block(uint32_t bid, const float* __restr
         const uint32_t c1 = min(c0 + static_cast<uint32_t>(kChunk), seglen);
         __syncthreads();  // the previous chunk is no longer read
         for (uint32_t k = c0 + threadIdx.x; k < c1; k += kBlock) {
-            s_e[k - c0] = ge[k];
-            s_v[k - c0] = gv[k];
+            s_ev[k - c0] = make_float2(ge[k], gv[k]);
         }
         __syncthreads();
 #pragma unroll
@@ -283,9 +317,24 @@ __device__ __forceinline__ void bf_runs_block(uint32_t bid, const float* __restr
                 const uint32_t a = k0[sl] > c0 ? k0[sl] : c0;
                 const uint32_t z = k1[sl] < c1 ? k1[sl] : c1;
                 float ae = se[sl], av = sv[sl];
-                for (uint32_t k = a; k < z; ++k) {
-                    ae = ae + s_e[k - c0];
-                    av = av + s_v[k - c0];
+                // 8 reads in flight before the (strictly sequential) adds
+                uint32_t k = a;
+                for (; k + 8 <= z; k += 8) {
+                    float2 x[8];
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        x[i] = s_ev[k - c0 + i];
+                    }
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        ae = ae + x[i].x;
+                        av = av + x[i].y;
+                    }
+                }
+                for (; k < z; ++k) {
+                    const float2 x = s_ev[k - c0];
+                    ae = ae + x.x;
+                    av = av + x.y;
                 }
                 se[sl] = ae;
                 sv[sl] = av;
@@ -295,20 +344,18 @@ __device__ __forceinline__ void bf_runs_block(uint32_t bid, const float* __restr
                 }
                 u[sl] += nb;
                 if (u[sl] <= u_end[sl]) {
-                    k0[sl] = run_bound(u[sl], seglen, tsamp, t_ref, fq[sl], nbins);
-                    k1[sl] = run_bound(u[sl] + 1, seglen, tsamp, t_ref, fq[sl], nbins);
+                    k0[sl] = run_bound(u[sl], seglen, tsamp, t_ref, rg, nbins);
+                    k1[sl] = run_bound(u[sl] + 1, seglen, tsamp, t_ref, rg, nbins);
                 }
             }
         }
     }
+    if (fi >= nfreqs) {
+        return;
+    }
 #pragma unroll
     for (int sl = 0; sl < kRunSlots; ++sl) {
-        const uint32_t p = pbase + threadIdx.x + static_cast<uint32_t>(sl) * kBlock;
-        if (p >= npair) {
-            continue;
-        }
-        const uint32_t fi = p / nbins;
-        const uint32_t b  = p - fi * nbins;
+        const uint32_t b     = b0 + static_cast<uint32_t>(sl) * lpf;
         TO* __restrict__ out = fold + (seg * nfreqs + fi) * 2 * nbins;
         if constexpr (std::is_same_v<TO, float>) {
             out[b]         = se[sl];
@@ -489,8 +536,12 @@ void brute_fold_time(const float* ts_e, const float* ts_v, uint64_t nseg,
 uint32_t brute_fold_blocks_per_segment(const BruteFoldLists& L) {
     const uint64_t pairs = static_cast<uint64_t>(L.nfreqs) * L.nbins;
     LOKI_REQUIRE(pairs < (1ULL << 31), "brute fold: too many (freq, bin) pairs");
-    const uint64_t per = L.use_runs ? static_cast<uint64_t>(kBlock) * L.run_slots : kPairsBlk;
-    return static_cast<uint32_t>((pairs + per - 1) / per);
+    if (L.use_runs) {
+        // one lane per (frequency, group of run_slots bins); run_slots divides nbins
+        const uint64_t lanes = static_cast<uint64_t>(L.nfreqs) * (L.nbins / L.run_slots);
+        return static_cast<uint32_t>((lanes + kBlock - 1) / kBlock);
+    }
+    return static_cast<uint32_t>((pairs + kPairsBlk - 1) / kPairsBlk);
 }
 
 void choose_run_slots(BruteFoldLists& L, uint64_t nseg) {
@@ -500,6 +551,9 @@ void choose_run_slots(BruteFoldLists& L, uint64_t nseg) {
     }
     const uint64_t pairs = static_cast<uint64_t>(L.nfreqs) * L.nbins;
     for (uint32_t f : {8U, 4U, 2U}) {
+        if (L.nbins % f != 0) {
+            continue;
+        }
         const uint64_t per = static_cast<uint64_t>(kBlock) * f;
         if (nseg * ((pairs + per - 1) / per) >= 1024) {
             L.run_slots = f;
