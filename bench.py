@@ -59,6 +59,18 @@ WORKLOADS = {
                                       (-299792458.0 * 1e-8 / 50.0, 299792458.0 * 1e-8 / 50.0),
                                       (496.0, 500.0)], bseg_brute=2048),
     ),
+    # Jerk search with reduced-precision fold storage: levels 0..L-2 stored as f32, f16 or bf16
+    # with every add in FP32; the S/N error of each storage against an FP64 oracle fold is
+    # measured on a frequency sub-band around the injection (full accel/jerk range).
+    "jerk_ffa_fold_storage": dict(
+        nsamps=2**22, tsamp=64e-6, nbins=32, eta=1.0,
+        param_limits=[(-2.0, 2.0), (-30.0, 30.0), (50.0, 300.0)],
+        signal=dict(f0=150.0, accel=10.0, jerk=0.5, fwhm=0.05, snr=12.0), seed=4242,
+        # CPU baseline on the whole workload: ~6 s on 16 threads, ~70 GB of host memory
+        cpu_sample=dict(param_limits=[(-2.0, 2.0), (-30.0, 30.0), (50.0, 300.0)],
+                        bseg_brute=None),
+        storage_sample=dict(param_limits=[(-2.0, 2.0), (-30.0, 30.0), (149.0, 151.0)]),
+    ),
     # 256 independent period searches of uneven length (2^18..2^22 samples), nbins in
     # {32, 64, 128}, period ranges [P/2, 2P] (loki_amd.synth.make_batch_specs), run as one
     # batch: grouped launches per stage captured in a HIP graph, autotuned tile schedules.
@@ -305,6 +317,41 @@ def refine_cpu_baseline(wl, ts_e, ts_v) -> dict:
     return {"value": terms / dt, "unit": "harmonic terms/s (forward + adjoint)", "cores": 1,
             "kind": "port", "sample": f"2 candidates x 2^18 samples x {nbins // 2 + 1} harmonics "
                                       f"in {dt:.2f} s (FP64 numpy)"}
+
+
+def storage_error_vs_fp64(wl, ts_e, ts_v, widths) -> dict:
+    """|dS/N| of every fold storage against the FP64 oracle fold on the workload's storage
+    sample (same series and grid rules, a frequency sub-band around the injection)."""
+    from loki_amd import libculoki, libloki
+    from oracle import oracle as orc
+    from oracle.plan import SearchConfig
+
+    smp = wl["storage_sample"]
+    nbins = wl["nbins"]
+    ocfg = SearchConfig(wl["nsamps"], wl["tsamp"], nbins, wl["eta"], smp["param_limits"])
+    orc.lib()
+    t0 = time.perf_counter()
+    fold64, _ = orc.compute_ffa(ts_e, ts_v, ocfg, f64=True,
+                                nthreads=min(16, os.cpu_count() or 1))
+    snr64 = orc.snr_boxcar_3d_f64(fold64, widths, nbins).reshape(-1)
+    del fold64
+    cpu_s = time.perf_counter() - t0
+    cfg = libloki.configs.PulsarSearchConfig(
+        nsamps=wl["nsamps"], tsamp=wl["tsamp"], nbins=nbins, eta=wl["eta"],
+        param_limits=np.array(smp["param_limits"]), use_fourier=False)
+    ffa = libculoki.ffa.FFACUDA(cfg, device_id=0)
+    top = np.argsort(snr64)[-100:]
+    out = {"sample": f"param_limits {smp['param_limits']}: {snr64.size // len(widths)} trials "
+                     f"x {len(widths)} widths (FP64 oracle fold in {cpu_s:.1f} s)",
+           "max_fp64_snr": float(snr64.max())}
+    for st in ("f32", "f16", "bf16"):
+        ffa.set_fold_storage(st)
+        got = ffa.scores(ts_e, ts_v, widths).astype(np.float64)
+        d = np.abs(got - snr64)
+        out[st] = {"max": float(d.max()), "p99": float(np.percentile(d, 99)),
+                   "median": float(np.median(d)), "top100_max": float(d[top].max()),
+                   "frac_below_1e-3": float(np.mean(d < 1e-3))}
+    return out
 
 
 def run_batch(args, wl) -> int:
@@ -585,6 +632,8 @@ def main() -> int:
         "found": found,
         "injected": {"freq": sig["f0"], "accel": sig["accel"], "snr": sig["snr"]},
     }
+    if storage_cmp is not None and "storage_sample" in wl:
+        storage_cmp["snr_error_vs_fp64"] = storage_error_vs_fp64(wl, ts_e, ts_v, widths)
     if injections is not None:
         result["injected"] = {"n": len(injections), "spec": wl["injections"]}
         if not args.no_refine:
