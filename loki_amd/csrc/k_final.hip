@@ -34,20 +34,26 @@ constexpr uint32_t kRun      = 16;  // frequency tiles per sequential output run
 
 template <bool EXACT>
 __device__ __forceinline__ float4 normalise4(const float4 e, const float4 v) {
-    float4 p;
-    p.x = normalise<EXACT>(e.x, v.x);
-    p.y = normalise<EXACT>(e.y, v.y);
-    p.z = normalise<EXACT>(e.z, v.z);
-    p.w = normalise<EXACT>(e.w, v.w);
-    return p;
+    if constexpr (EXACT) {
+        return normalise4_exact(e, v);
+    } else {
+        float4 p;
+        p.x = normalise<false>(e.x, v.x);
+        p.y = normalise<false>(e.y, v.y);
+        p.z = normalise<false>(e.z, v.z);
+        p.w = normalise<false>(e.w, v.w);
+        return p;
+    }
 }
 
 /// e = tail + head, v = tail + head (one FP32 add each), unscaled for
-/// reduced-precision parents, normalised and stored as 4 profile bins.
-template <bool EXACT, typename TI>
+/// reduced-precision parents, normalised and stored as 4 profile bins at
+/// row[(b0 + k) mod NB], k < 4 (b0 = 4*i4 for aligned output, or rotated by
+/// the tail shift when the tail was loaded unrotated).
+template <int NB, bool EXACT, typename TI>
 __device__ __forceinline__ void merge_store(const float4 te, const float4 tv, const float4 he,
                                             const float4 hv, float unscale,
-                                            float* __restrict__ dst) {
+                                            float* __restrict__ row, uint32_t b0) {
     float4 e, v;
     e.x = te.x + he.x;
     e.y = te.y + he.y;
@@ -68,10 +74,10 @@ __device__ __forceinline__ void merge_store(const float4 te, const float4 tv, co
         v.w *= unscale;
     }
     const float4 p = normalise4<EXACT>(e, v);
-    dst[0] = p.x;
-    dst[1] = p.y;
-    dst[2] = p.z;
-    dst[3] = p.w;
+    row[(b0 + 0) & (NB - 1)] = p.x;
+    row[(b0 + 1) & (NB - 1)] = p.y;
+    row[(b0 + 2) & (NB - 1)] = p.z;
+    row[(b0 + 3) & (NB - 1)] = p.w;
 }
 
 template <int NB, int NP, bool FINDMAX, bool EXACT, typename TI>
@@ -112,6 +118,7 @@ __device__ __forceinline__ void final_block(uint32_t bid, const TI* __restrict__
 #pragma unroll
         for (uint32_t s0 = 0; s0 < NSTEP; s0 += U) {
             float4 te[U], tv[U], he[U], hv[U];
+            uint32_t b0[U];
 #pragma unroll
             for (uint32_t u = 0; u < U; ++u) {
                 const uint32_t r = rsub + (s0 + u) * RPI;
@@ -121,25 +128,25 @@ __device__ __forceinline__ void final_block(uint32_t bid, const TI* __restrict__
                         in + static_cast<uint64_t>(cd.i_tail) * (2 * NB));
                     const CI* __restrict__ hrow = reinterpret_cast<const CI*>(
                         in + (nprev + static_cast<uint64_t>(cd.i_head)) * (2 * NB));
-                    if constexpr (NP == 1) {
-                        te[u] = widen4<TI>(trow[i4]);
-                        tv[u] = widen4<TI>(trow[C4 + i4]);
-                    } else {
-                        const uint32_t St = shift_int(cd.shift_tail, NB);
-                        te[u] = rotated4_st<NB, TI>(trow, i4, St);
-                        tv[u] = rotated4_st<NB, TI>(trow + C4, i4, St);
-                    }
+                    // child[j] = tail[j - St] + head[j - Sh] (mod NB).  The tail is
+                    // read unrotated and the sums stored at j = 4*i4 + k + St, so
+                    // only the head is rotated, by Sh - St: the same two operands
+                    // of every add, half the rotations.
+                    const uint32_t St = NP == 1 ? 0U : shift_int(cd.shift_tail, NB);
                     const uint32_t Sh = shift_int(cd.shift_head, NB);
-                    he[u] = rotated4_st<NB, TI>(hrow, i4, Sh);
-                    hv[u] = rotated4_st<NB, TI>(hrow + C4, i4, Sh);
+                    te[u] = widen4<TI>(trow[i4]);
+                    tv[u] = widen4<TI>(trow[C4 + i4]);
+                    he[u] = rotated4_st<NB, TI>(hrow, i4, (Sh - St) & (NB - 1));
+                    hv[u] = rotated4_st<NB, TI>(hrow + C4, i4, (Sh - St) & (NB - 1));
+                    b0[u] = 4 * i4 + St;
                 }
             }
 #pragma unroll
             for (uint32_t u = 0; u < U; ++u) {
                 const uint32_t r = rsub + (s0 + u) * RPI;
                 if (r < nrow) {
-                    merge_store<EXACT, TI>(te[u], tv[u], he[u], hv[u], unscale,
-                                           s_p + r * stride + 4 * i4);
+                    merge_store<NB, EXACT, TI>(te[u], tv[u], he[u], hv[u], unscale,
+                                               s_p + r * stride, b0[u]);
                 }
             }
         }

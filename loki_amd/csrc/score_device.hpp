@@ -42,4 +42,55 @@ __device__ __forceinline__ float normalise(float e, float v) {
     }
 }
 
+/// Correctly rounded sqrtf(v) for 2^-96 <= v <= FLT_MAX without the scaling and
+/// special-case handling of the general sequence: v_sqrt_f32 is within 1 ulp,
+/// and the residuals v - s' * s of the two neighbours s' (exact by FMA, and
+/// never strictly between two representable values of that size) pick the
+/// nearest float.  Checked against the IEEE sequence for every float in range.
+__device__ __forceinline__ float sqrt_rn_normal(float v) {
+    const float s   = __builtin_amdgcn_sqrtf(v);
+    const float sdn = __uint_as_float(__float_as_uint(s) - 1U);
+    const float sup = __uint_as_float(__float_as_uint(s) + 1U);
+    const float rdn = fmaf(-sdn, s, v);
+    const float rup = fmaf(-sup, s, v);
+    float r         = (rdn <= 0.0F) ? sdn : s;
+    r               = (rup > 0.0F) ? sup : r;
+    return r;
+}
+
+/// Correctly rounded 1/s for 2^-48 <= s < 2^64 (v_rcp_f32 plus one FMA
+/// Newton step); checked against IEEE division for every float in range.
+__device__ __forceinline__ float rcp_rn_normal(float s) {
+    const float r = __builtin_amdgcn_rcpf(s);
+    const float e = fmaf(-s, r, 1.0F);
+    return fmaf(e, r, r);
+}
+
+/// Whether v lies in [2^-96, FLT_MAX] (excludes zero, negatives, subnormals,
+/// inf and NaN), the range of sqrt_rn_normal / rcp_rn_normal.
+__device__ __forceinline__ bool rn_normal_range(float v) {
+    return (__float_as_uint(v) - 0x0F800000U) < (0x7F800000U - 0x0F800000U);
+}
+
+/// p = e * (1/sqrtf(v)) for four bins, bit-identical to normalise<true>: the
+/// short sequences when every active lane's v is in range (wave-uniform
+/// branch), the general IEEE sequence otherwise.
+__device__ __forceinline__ float4 normalise4_exact(const float4 e, const float4 v) {
+    const bool ok = rn_normal_range(v.x) && rn_normal_range(v.y) && rn_normal_range(v.z) &&
+                    rn_normal_range(v.w);
+    float4 p;
+    if (__all(ok)) {
+        p.x = e.x * rcp_rn_normal(sqrt_rn_normal(v.x));
+        p.y = e.y * rcp_rn_normal(sqrt_rn_normal(v.y));
+        p.z = e.z * rcp_rn_normal(sqrt_rn_normal(v.z));
+        p.w = e.w * rcp_rn_normal(sqrt_rn_normal(v.w));
+    } else {
+        p.x = normalise<true>(e.x, v.x);
+        p.y = normalise<true>(e.y, v.y);
+        p.z = normalise<true>(e.z, v.z);
+        p.w = normalise<true>(e.w, v.w);
+    }
+    return p;
+}
+
 }  // namespace loki_amd::kern
