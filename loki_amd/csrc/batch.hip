@@ -183,9 +183,10 @@ void FFABatch::build_tasks() {
     }
     // 2. brute folds: list-path short segments, and run-path long segments by
     //    pairs per lane (one grouped launch per kind)
-    for (uint32_t kind : {0U, 1U, 2U, 4U, 8U}) {
+    for (uint32_t kind : {0U, 1U, 2U, 4U, 8U, kern::kRunUnits}) {
         std::vector<kern::FoldTask> tasks;
         uint64_t block0 = 0;
+        uint32_t max_nbins = 0;
         for (const auto& sr : m_series) {
             const auto& L = sr.lists;
             if ((L.use_runs ? L.run_slots : 0U) != kind) {
@@ -210,11 +211,14 @@ void FFABatch::build_tasks() {
             t.block0 = static_cast<uint32_t>(block0);
             tasks.push_back(t);
             block0 += sr.plan.nsegments()[0] * bps;
+            max_nbins = std::max(max_nbins, L.nbins);
         }
         LOKI_REQUIRE(block0 < (1ULL << 31), "FFABatch: brute-fold grid too large");
         if (!tasks.empty()) {
+            // fold launches keep the largest nbins of their tasks in `stride`
             m_launches.push_back({Stage::kFold, kind, static_cast<uint32_t>(tasks.size()),
-                                  static_cast<uint32_t>(block0), append(blob, tasks), 0});
+                                  static_cast<uint32_t>(block0), append(blob, tasks),
+                                  max_nbins});
         }
     }
     // 3. merges, one launch per level index and nbins class
@@ -313,7 +317,7 @@ void FFABatch::enqueue(hipStream_t s) const {
             break;
         case Stage::kFold:
             kern::brute_fold_grouped(static_cast<const kern::FoldTask*>(t), ln.ntasks,
-                                     ln.nblocks, ln.nbins, FoldStorage::kF32, s);
+                                     ln.nblocks, ln.nbins, ln.stride, FoldStorage::kF32, s);
             break;
         case Stage::kMerge:
             kern::merge_grouped(static_cast<const kern::MergeTask*>(t), ln.ntasks, ln.nblocks,

@@ -81,7 +81,7 @@ private:
         uint32_t nbins;   ///< merge / final: nbins; fold: 0 list path, else run slots
         uint32_t ntasks, nblocks;
         SizeType task_off;  ///< bytes into the task arena
-        uint32_t stride;    ///< final: LDS row stride
+        uint32_t stride;    ///< final: LDS row stride; fold: the largest nbins
     };
 
     int m_device;

@@ -367,6 +367,164 @@ __device__ __forceinline__ void bf_runs_block(uint32_t bid, const float* __restr
     }
 }
 
+// Long cycles: when a cycle is longer than a chunk, a chunk holds runs of only
+// a short arc of bins, so lanes bound to fixed bins mostly idle.  Here a block
+// owns NF frequencies of one segment with all their bins; their running sums
+// live in LDS, and per chunk the block enumerates the (frequency, bin) units
+// that have samples in the chunk -- for each frequency the bins of the unwrapped
+// range [U(c0), U(c1 - 1)] -- and deals them out to the lanes.  A unit adds all
+// runs of its bin inside the chunk to the bin's running sum in ascending
+// sample order, so every bucket is still the same ordered FP32 sum.
+constexpr int kUnitChunk = 2048;  // samples staged per chunk
+constexpr int kUnitMaxNF = 64;    // frequencies per block (one wave scans their unit counts)
+
+__host__ __device__ inline uint32_t unit_nf(uint32_t nbins) {
+    const uint32_t nf = 4096U / nbins;  // running sums: nf * nbins * 8 bytes <= 32 KiB
+    return nf < 1U ? 1U : (nf > static_cast<uint32_t>(kUnitMaxNF) ? kUnitMaxNF : nf);
+}
+
+__host__ __device__ inline size_t unit_lds_bytes(uint32_t nbins) {
+    const size_t nf = unit_nf(nbins);
+    return kUnitChunk * sizeof(float2) + nf * nbins * sizeof(float2) +
+           nf * (sizeof(RunGeom) + 2 * sizeof(int64_t)) + (kUnitMaxNF + 1) * sizeof(uint32_t);
+}
+
+template <typename TO>
+__device__ __forceinline__ void bf_units_block(uint32_t bid, const float* __restrict__ ts_e,
+                                               const float* __restrict__ ts_v, uint32_t seglen,
+                                               uint32_t nfreqs, uint32_t nbins,
+                                               uint32_t blocks_per_seg,
+                                               const double* __restrict__ freqs, double tsamp,
+                                               double t_ref, TO* __restrict__ fold, float scale) {
+    extern __shared__ __align__(16) unsigned char s_mem[];
+    const uint32_t NF  = unit_nf(nbins);
+    auto* s_ev         = reinterpret_cast<float2*>(s_mem);  // [kUnitChunk], 16-byte aligned
+    auto* acc          = s_ev + kUnitChunk;                   // [NF][nbins]
+    auto* s_geom       = reinterpret_cast<RunGeom*>(acc + NF * nbins);  // [NF]
+    auto* s_ulo        = reinterpret_cast<int64_t*>(s_geom + NF);        // [NF]
+    auto* s_uhi        = s_ulo + NF;                                     // [NF]
+    auto* s_pref       = reinterpret_cast<uint32_t*>(s_uhi + NF);        // [kUnitMaxNF + 1]
+    const uint64_t seg = bid / blocks_per_seg;
+    const uint32_t f0  = (bid % blocks_per_seg) * NF;
+    const uint32_t nf  = min(NF, nfreqs - f0);
+    const float* __restrict__ ge = ts_e + seg * seglen;
+    const float* __restrict__ gv = ts_v + seg * seglen;
+    const int64_t nb = nbins;
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t i = tid; i < nf * nbins; i += kBlock) {
+        acc[i] = make_float2(0.0F, 0.0F);
+    }
+    if (tid < nf) {
+        s_geom[tid] = run_geom(freqs[f0 + tid], seglen, tsamp, t_ref, nbins);
+    }
+    for (uint32_t c0 = 0; c0 < seglen; c0 += kUnitChunk) {
+        const uint32_t c1 = min(c0 + static_cast<uint32_t>(kUnitChunk), seglen);
+        __syncthreads();  // previous chunk's units are done with s_ev and s_pref
+        for (uint32_t k = c0 + tid; k < c1; k += kBlock) {
+            s_ev[k - c0] = make_float2(ge[k], gv[k]);
+        }
+        if (tid < kUnitMaxNF) {  // first wave: unit counts and their exclusive scan
+            uint32_t cnt = 0;
+            if (tid < nf) {
+                const double f   = s_geom[tid].f;
+                const int64_t lo = u_at(c0, tsamp, t_ref, f, nbins);
+                const int64_t hi = u_at(c1 - 1, tsamp, t_ref, f, nbins);
+                s_ulo[tid]       = lo;
+                s_uhi[tid]       = hi;
+                cnt = static_cast<uint32_t>(hi - lo + 1 < nb ? hi - lo + 1 : nb);
+            }
+            uint32_t incl = cnt;
+            for (uint32_t o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(incl, o);
+                incl += (tid % 64 >= o) ? y : 0U;
+            }
+            s_pref[tid + 1] = incl;
+            if (tid == 0) {
+                s_pref[0] = 0;
+            }
+        }
+        __syncthreads();
+        const uint32_t total = s_pref[nf];
+        for (uint32_t t = tid; t < total; t += kBlock) {
+            uint32_t lo = 0, hi = nf;  // largest fi with s_pref[fi] <= t
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) / 2;
+                if (s_pref[mid] <= t) {
+                    lo = mid;
+                } else {
+                    hi = mid;
+                }
+            }
+            const uint32_t fi  = lo;
+            const int64_t u0   = s_ulo[fi] + static_cast<int64_t>(t - s_pref[fi]);
+            const int64_t uhi  = s_uhi[fi];
+            const RunGeom g    = s_geom[fi];
+            const uint32_t b   = static_cast<uint32_t>(((u0 % nb) + nb) % nb);
+            float2* slot       = acc + fi * nbins + b;
+            float ae = slot->x, av = slot->y;
+            for (int64_t u = u0; u <= uhi; u += nb) {
+                uint32_t a = run_bound(u, seglen, tsamp, t_ref, g, nbins);
+                uint32_t z = run_bound(u + 1, seglen, tsamp, t_ref, g, nbins);
+                a = a > c0 ? a : c0;
+                z = z < c1 ? z : c1;
+                uint32_t k = a;
+                if ((k & 1U) != 0 && k < z) {  // c0 is even: align to a sample pair
+                    const float2 x = s_ev[k - c0];
+                    ae = ae + x.x;
+                    av = av + x.y;
+                    ++k;
+                }
+                // 16-byte reads of two (e, v) pairs: fewer LDS instructions, and 8 lanes per
+                // LDS pass instead of 16 (fewer bank collisions between the lanes' runs)
+                const auto* s4 = reinterpret_cast<const float4*>(s_ev);
+                for (; k + 8 <= z; k += 8) {
+                    float4 y[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        y[i] = s4[(k - c0) / 2 + i];
+                    }
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        ae = ae + y[i].x;
+                        av = av + y[i].y;
+                        ae = ae + y[i].z;
+                        av = av + y[i].w;
+                    }
+                }
+                for (; k < z; ++k) {
+                    const float2 x = s_ev[k - c0];
+                    ae = ae + x.x;
+                    av = av + x.y;
+                }
+            }
+            *slot = make_float2(ae, av);
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < nf * nbins; i += kBlock) {
+        const uint32_t fi    = i / nbins;
+        const uint32_t b     = i - fi * nbins;
+        TO* __restrict__ out = fold + (seg * nfreqs + f0 + fi) * 2 * nbins;
+        if constexpr (std::is_same_v<TO, float>) {
+            out[b]         = acc[i].x;
+            out[nbins + b] = acc[i].y;
+        } else {
+            out[b]         = narrow1<TO>(acc[i].x * scale);
+            out[nbins + b] = narrow1<TO>(acc[i].y * scale);
+        }
+    }
+}
+
+template <typename TO>
+__global__ void __launch_bounds__(kBlock)
+k_bf_units(const float* __restrict__ ts_e, const float* __restrict__ ts_v, uint32_t seglen,
+           uint32_t nfreqs, uint32_t nbins, uint32_t blocks_per_seg,
+           const double* __restrict__ freqs, double tsamp, double t_ref, TO* __restrict__ fold,
+           float scale) {
+    bf_units_block<TO>(blockIdx.x, ts_e, ts_v, seglen, nfreqs, nbins, blocks_per_seg, freqs,
+                       tsamp, t_ref, fold, scale);
+}
+
 template <typename TO, int F>
 __global__ void __launch_bounds__(kBlock)
 k_bf_runs(const float* __restrict__ ts_e, const float* __restrict__ ts_v, uint32_t seglen,
@@ -423,6 +581,24 @@ k_bf_runs_grouped(const FoldTask* __restrict__ tasks, uint32_t ntasks) {
     const FoldTask& t = tasks[lo];
     bf_runs_block<TO, F>(blockIdx.x - t.block0, t.ts_e, t.ts_v, t.seglen, t.nfreqs, t.nbins,
                          t.bps, t.freqs, t.tsamp, t.t_ref, static_cast<TO*>(t.fold), t.scale);
+}
+
+template <typename TO>
+__global__ void __launch_bounds__(kBlock)
+k_bf_units_grouped(const FoldTask* __restrict__ tasks, uint32_t ntasks) {
+    uint32_t lo = 0;
+    uint32_t hi = ntasks;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (tasks[mid].block0 <= blockIdx.x) {
+            lo = mid;
+        } else {
+            hi = mid;
+        }
+    }
+    const FoldTask& t = tasks[lo];
+    bf_units_block<TO>(blockIdx.x - t.block0, t.ts_e, t.ts_v, t.seglen, t.nfreqs, t.nbins, t.bps,
+                       t.freqs, t.tsamp, t.t_ref, static_cast<TO*>(t.fold), t.scale);
 }
 
 }  // namespace
@@ -488,6 +664,10 @@ void launch_fold(const float* ts_e, const float* ts_v, uint64_t blocks, uint32_t
         const auto g = static_cast<unsigned>(blocks);
         const auto* fr = L.freqs.as<double>();
         switch (L.run_slots) {
+        case kRunUnits:
+            k_bf_units<TO><<<g, kBlock, unit_lds_bytes(L.nbins), s>>>(
+                ts_e, ts_v, L.seglen, L.nfreqs, L.nbins, bps, fr, L.tsamp, L.t_ref, fold, scale);
+            break;
         case 1:
             k_bf_runs<TO, 1><<<g, kBlock, 0, s>>>(ts_e, ts_v, L.seglen, L.nfreqs, L.nbins, bps,
                                                   fr, L.tsamp, L.t_ref, fold, scale);
@@ -536,6 +716,10 @@ void brute_fold_time(const float* ts_e, const float* ts_v, uint64_t nseg,
 uint32_t brute_fold_blocks_per_segment(const BruteFoldLists& L) {
     const uint64_t pairs = static_cast<uint64_t>(L.nfreqs) * L.nbins;
     LOKI_REQUIRE(pairs < (1ULL << 31), "brute fold: too many (freq, bin) pairs");
+    if (L.use_runs && L.run_slots == kRunUnits) {
+        const uint32_t nf = unit_nf(L.nbins);
+        return (L.nfreqs + nf - 1) / nf;
+    }
     if (L.use_runs) {
         // one lane per (frequency, group of run_slots bins); run_slots divides nbins
         const uint64_t lanes = static_cast<uint64_t>(L.nfreqs) * (L.nbins / L.run_slots);
@@ -549,6 +733,8 @@ void choose_run_slots(BruteFoldLists& L, uint64_t nseg) {
     if (!L.use_runs) {
         return;
     }
+    L.run_slots = kRunUnits;
+    return;
     const uint64_t pairs = static_cast<uint64_t>(L.nfreqs) * L.nbins;
     for (uint32_t f : {8U, 4U, 2U}) {
         if (L.nbins % f != 0) {
@@ -564,25 +750,36 @@ void choose_run_slots(BruteFoldLists& L, uint64_t nseg) {
 
 template <typename TO>
 void grouped_fold(const FoldTask* tasks, uint32_t ntasks, uint32_t nblocks, uint32_t run_slots,
-                  hipStream_t s) {
+                  uint32_t max_nbins, hipStream_t s) {
     switch (run_slots) {
     case 0: k_bf_fold_grouped<true, TO><<<nblocks, kBlock, 0, s>>>(tasks, ntasks); break;
     case 1: k_bf_runs_grouped<TO, 1><<<nblocks, kBlock, 0, s>>>(tasks, ntasks); break;
     case 2: k_bf_runs_grouped<TO, 2><<<nblocks, kBlock, 0, s>>>(tasks, ntasks); break;
     case 4: k_bf_runs_grouped<TO, 4><<<nblocks, kBlock, 0, s>>>(tasks, ntasks); break;
+    case kRunUnits: {
+        size_t lds = 0;  // enough for every nbins up to the launch's largest
+        for (uint32_t nb = 1; nb <= max_nbins; ++nb) {
+            lds = std::max(lds, unit_lds_bytes(nb));
+        }
+        k_bf_units_grouped<TO><<<nblocks, kBlock, lds, s>>>(tasks, ntasks);
+        break;
+    }
     default: k_bf_runs_grouped<TO, 8><<<nblocks, kBlock, 0, s>>>(tasks, ntasks); break;
     }
 }
 
 void brute_fold_grouped(const FoldTask* tasks, uint32_t ntasks, uint32_t nblocks,
-                        uint32_t run_slots, FoldStorage so, hipStream_t s) {
+                        uint32_t run_slots, uint32_t max_nbins, FoldStorage so, hipStream_t s) {
     if (ntasks == 0 || nblocks == 0) {
         return;
     }
     switch (so) {
-    case FoldStorage::kF32: return grouped_fold<float>(tasks, ntasks, nblocks, run_slots, s);
-    case FoldStorage::kF16: return grouped_fold<_Float16>(tasks, ntasks, nblocks, run_slots, s);
-    case FoldStorage::kBF16: return grouped_fold<__bf16>(tasks, ntasks, nblocks, run_slots, s);
+    case FoldStorage::kF32:
+        return grouped_fold<float>(tasks, ntasks, nblocks, run_slots, max_nbins, s);
+    case FoldStorage::kF16:
+        return grouped_fold<_Float16>(tasks, ntasks, nblocks, run_slots, max_nbins, s);
+    case FoldStorage::kBF16:
+        return grouped_fold<__bf16>(tasks, ntasks, nblocks, run_slots, max_nbins, s);
     }
 }
 
