@@ -89,8 +89,6 @@ FFABatch::FFABatch(const std::vector<SearchConfig>& cfgs, const std::vector<Size
             m_freqs.as<double>(), static_cast<uint32_t>(nfr),
             static_cast<uint32_t>(sr.plan.segment_lens()[0]), static_cast<uint32_t>(cfg.get_nbins()),
             cfg.get_tsamp(), 0.0, s);
-        // many series share each grouped launch: favour reuse of staged samples
-        kern::choose_run_slots(sr.lists, 1024);
     }
     build_tasks();
 }
@@ -183,13 +181,13 @@ void FFABatch::build_tasks() {
     }
     // 2. brute folds: list-path short segments, and run-path long segments by
     //    pairs per lane (one grouped launch per kind)
-    for (uint32_t kind : {0U, 1U, 2U, 4U, 8U, kern::kRunUnits}) {
+    for (uint32_t kind : {0U, 1U}) {  // 0: list path, 1: run path
         std::vector<kern::FoldTask> tasks;
         uint64_t block0 = 0;
         uint32_t max_nbins = 0;
         for (const auto& sr : m_series) {
             const auto& L = sr.lists;
-            if ((L.use_runs ? L.run_slots : 0U) != kind) {
+            if ((L.use_runs ? 1U : 0U) != kind) {
                 continue;
             }
             const uint32_t bps = kern::brute_fold_blocks_per_segment(L);
@@ -317,7 +315,7 @@ void FFABatch::enqueue(hipStream_t s) const {
             break;
         case Stage::kFold:
             kern::brute_fold_grouped(static_cast<const kern::FoldTask*>(t), ln.ntasks,
-                                     ln.nblocks, ln.nbins, ln.stride, FoldStorage::kF32, s);
+                                     ln.nblocks, ln.nbins != 0, ln.stride, FoldStorage::kF32, s);
             break;
         case Stage::kMerge:
             kern::merge_grouped(static_cast<const kern::MergeTask*>(t), ln.ntasks, ln.nblocks,

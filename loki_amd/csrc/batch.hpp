@@ -78,7 +78,7 @@ private:
     enum class Stage : int { kResolve, kFold, kMerge, kFinal };
     struct Launch {
         Stage stage;
-        uint32_t nbins;   ///< merge / final: nbins; fold: 0 list path, else run slots
+        uint32_t nbins;   ///< merge / final: nbins; fold: 0 list path, 1 run path
         uint32_t ntasks, nblocks;
         SizeType task_off;  ///< bytes into the task arena
         uint32_t stride;    ///< final: LDS row stride; fold: the largest nbins

@@ -221,7 +221,6 @@ void FFAExecutor::prepare_brute_fold(hipStream_t s) {
         m_bf = kern::build_brute_fold_lists(
             m_bf_freqs.as<double>(), static_cast<uint32_t>(nfreqs), static_cast<uint32_t>(seglen),
             static_cast<uint32_t>(nbins), m_plan.config().get_tsamp(), t_ref, s);
-        kern::choose_run_slots(m_bf, m_plan.nsegments()[0]);
     } else {
         hip_check(hipStreamSynchronize(s), "copy brute-fold frequencies");
     }
@@ -453,7 +452,6 @@ void brute_fold_time_device(const float* ts_e, const float* ts_v, SizeType nsamp
     auto lists = kern::build_brute_fold_lists(
         d_freqs.as<double>(), static_cast<uint32_t>(freqs.size()),
         static_cast<uint32_t>(segment_len), static_cast<uint32_t>(nbins), tsamp, t_ref, s);
-    kern::choose_run_slots(lists, nsamps / segment_len);
     kern::brute_fold_time(ts_e, ts_v, nsamps / segment_len, lists, fold, FoldStorage::kF32, 1.0F,
                           s);
     check_launch("brute fold launch");

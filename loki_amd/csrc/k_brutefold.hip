@@ -21,7 +21,7 @@
 namespace loki_amd::kern {
 namespace {
 
-constexpr int kChunk    = 4096;  // samples whose bins are staged in LDS at once
+constexpr int kChunk    = 4096;  // samples whose bins are counted in LDS at once
 constexpr int kBlock    = 256;
 constexpr int kPairsBlk = 2048;  // (freq, bin) pairs per fold block
 constexpr int kLdsSamps = 4096;  // segments up to this length are staged in LDS
@@ -251,122 +251,6 @@ __device__ __forceinline__ uint32_t run_bound(int64_t u, uint32_t n, double tsam
     return lo;
 }
 
-// A block owns F * kBlock (freq, bin) pairs of one segment (F in {1, 2, 4, 8}
-// dividing nbins, chosen per plan so the grid still fills the GPU).  A lane
-// owns one frequency and the F bins b0 + s * nbins / F, spread evenly around
-// the cycle: when a cycle is longer than a chunk, a chunk covers a short arc of
-// bins, and spreading each lane's bins keeps most lanes busy in every chunk
-// (with a lane's bins adjacent or its slots at one bin, only the lanes of the
-// arc would work).  The segment streams
-// through LDS in chunks of kChunk samples that every slot of every lane reads,
-// so each sample leaves L2 once per block instead of once per frequency.  A
-// slot keeps its running sums and its current run [k0, k1) across chunks, so
-// every bucket is still summed in ascending sample order.
-template <typename TO, int kRunSlots>
-__device__ __forceinline__ void bf_runs_block(uint32_t bid, const float* __restrict__ ts_e,
-                                              const float* __restrict__ ts_v, uint32_t seglen,
-                                              uint32_t nfreqs, uint32_t nbins,
-                                              uint32_t blocks_per_seg,
-                                              const double* __restrict__ freqs, double tsamp,
-                                              double t_ref, TO* __restrict__ fold, float scale) {
-    __shared__ float2 s_ev[kChunk];  // (e, v) interleaved: one 8-byte LDS read per sample
-    const uint64_t seg   = bid / blocks_per_seg;
-    const uint32_t lpf   = nbins / kRunSlots;  // lanes per frequency
-    const uint32_t lg    = (bid % blocks_per_seg) * kBlock + threadIdx.x;
-    const uint32_t fi    = lg / lpf;
-    const uint32_t b0    = lg - fi * lpf;
-    const float* __restrict__ ge = ts_e + seg * seglen;
-    const float* __restrict__ gv = ts_v + seg * seglen;
-    const int64_t nb = nbins;
-
-    float se[kRunSlots], sv[kRunSlots];
-    int64_t u[kRunSlots], u_end[kRunSlots];
-    uint32_t k0[kRunSlots], k1[kRunSlots];
-    // every slot of a lane has the lane's frequency
-    const RunGeom rg = run_geom(fi < nfreqs ? freqs[fi] : 1.0, seglen, tsamp, t_ref, nbins);
-#pragma unroll
-    for (int sl = 0; sl < kRunSlots; ++sl) {
-        se[sl] = 0.0F;
-        sv[sl] = 0.0F;
-        u[sl]  = 1;
-        u_end[sl] = 0;  // inactive unless the pair exists
-        k0[sl] = seglen;
-        k1[sl] = seglen;
-        if (fi < nfreqs) {
-            const uint32_t b  = b0 + static_cast<uint32_t>(sl) * lpf;
-            const double f    = rg.f;
-            const int64_t u0  = u_at(0, tsamp, t_ref, f, nbins);
-            u_end[sl] = u_at(seglen - 1, tsamp, t_ref, f, nbins);
-            u[sl]     = u0 + ((static_cast<int64_t>(b) - u0) % nb + nb) % nb;
-            if (u[sl] <= u_end[sl]) {
-                k0[sl] = run_bound(u[sl], seglen, tsamp, t_ref, rg, nbins);
-                k1[sl] = run_bound(u[sl] + 1, seglen, tsamp, t_ref, rg, nbins);
-            }
-        }
-    }
-    for (uint32_t c0 = 0; c0 < seglen; c0 += kChunk) {
-        const uint32_t c1 = min(c0 + static_cast<uint32_t>(kChunk), seglen);
-        __syncthreads();  // the previous chunk is no longer read
-        for (uint32_t k = c0 + threadIdx.x; k < c1; k += kBlock) {
-            s_ev[k - c0] = make_float2(ge[k], gv[k]);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int sl = 0; sl < kRunSlots; ++sl) {
-            while (u[sl] <= u_end[sl] && k0[sl] < c1) {
-                const uint32_t a = k0[sl] > c0 ? k0[sl] : c0;
-                const uint32_t z = k1[sl] < c1 ? k1[sl] : c1;
-                float ae = se[sl], av = sv[sl];
-                // 8 reads in flight before the (strictly sequential) adds
-                uint32_t k = a;
-                for (; k + 8 <= z; k += 8) {
-                    float2 x[8];
-#pragma unroll
-                    for (int i = 0; i < 8; ++i) {
-                        x[i] = s_ev[k - c0 + i];
-                    }
-#pragma unroll
-                    for (int i = 0; i < 8; ++i) {
-                        ae = ae + x[i].x;
-                        av = av + x[i].y;
-                    }
-                }
-                for (; k < z; ++k) {
-                    const float2 x = s_ev[k - c0];
-                    ae = ae + x.x;
-                    av = av + x.y;
-                }
-                se[sl] = ae;
-                sv[sl] = av;
-                if (k1[sl] > c1) {
-                    k0[sl] = c1;  // the run continues in the next chunk
-                    break;
-                }
-                u[sl] += nb;
-                if (u[sl] <= u_end[sl]) {
-                    k0[sl] = run_bound(u[sl], seglen, tsamp, t_ref, rg, nbins);
-                    k1[sl] = run_bound(u[sl] + 1, seglen, tsamp, t_ref, rg, nbins);
-                }
-            }
-        }
-    }
-    if (fi >= nfreqs) {
-        return;
-    }
-#pragma unroll
-    for (int sl = 0; sl < kRunSlots; ++sl) {
-        const uint32_t b     = b0 + static_cast<uint32_t>(sl) * lpf;
-        TO* __restrict__ out = fold + (seg * nfreqs + fi) * 2 * nbins;
-        if constexpr (std::is_same_v<TO, float>) {
-            out[b]         = se[sl];
-            out[nbins + b] = sv[sl];
-        } else {
-            out[b]         = narrow1<TO>(se[sl] * scale);
-            out[nbins + b] = narrow1<TO>(sv[sl] * scale);
-        }
-    }
-}
-
 // Long cycles: when a cycle is longer than a chunk, a chunk holds runs of only
 // a short arc of bins, so lanes bound to fixed bins mostly idle.  Here a block
 // owns NF frequencies of one segment with all their bins; their running sums
@@ -525,16 +409,6 @@ k_bf_units(const float* __restrict__ ts_e, const float* __restrict__ ts_v, uint3
                        tsamp, t_ref, fold, scale);
 }
 
-template <typename TO, int F>
-__global__ void __launch_bounds__(kBlock)
-k_bf_runs(const float* __restrict__ ts_e, const float* __restrict__ ts_v, uint32_t seglen,
-          uint32_t nfreqs, uint32_t nbins, uint32_t blocks_per_seg,
-          const double* __restrict__ freqs, double tsamp, double t_ref, TO* __restrict__ fold,
-          float scale) {
-    bf_runs_block<TO, F>(blockIdx.x, ts_e, ts_v, seglen, nfreqs, nbins, blocks_per_seg, freqs,
-                         tsamp, t_ref, fold, scale);
-}
-
 template <bool USE_LDS, typename TO>
 __global__ void __launch_bounds__(kBlock)
 k_bf_fold(const float* __restrict__ ts_e, const float* __restrict__ ts_v, uint32_t seglen,
@@ -563,24 +437,6 @@ k_bf_fold_grouped(const FoldTask* __restrict__ tasks, uint32_t ntasks) {
     bf_fold_block<USE_LDS, TO>(blockIdx.x - t.block0, t.ts_e, t.ts_v, t.seglen, t.nfreqs,
                                t.nbins, t.bps, t.list, t.base, t.counts,
                                static_cast<TO*>(t.fold), t.scale);
-}
-
-template <typename TO, int F>
-__global__ void __launch_bounds__(kBlock)
-k_bf_runs_grouped(const FoldTask* __restrict__ tasks, uint32_t ntasks) {
-    uint32_t lo = 0;
-    uint32_t hi = ntasks;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) / 2;
-        if (tasks[mid].block0 <= blockIdx.x) {
-            lo = mid;
-        } else {
-            hi = mid;
-        }
-    }
-    const FoldTask& t = tasks[lo];
-    bf_runs_block<TO, F>(blockIdx.x - t.block0, t.ts_e, t.ts_v, t.seglen, t.nfreqs, t.nbins,
-                         t.bps, t.freqs, t.tsamp, t.t_ref, static_cast<TO*>(t.fold), t.scale);
 }
 
 template <typename TO>
@@ -616,7 +472,6 @@ BruteFoldLists build_brute_fold_lists(const double* freqs_dev, uint32_t nfreqs, 
     L.tsamp    = tsamp;
     L.t_ref    = t_ref;
     L.use_runs = seglen > static_cast<uint32_t>(kLdsSamps);
-    L.run_slots = 1;  // choose_run_slots() tunes it for the number of segments
     L.freqs.reserve(static_cast<SizeType>(nfreqs) * sizeof(double));
     hip_check(hipMemcpyAsync(L.freqs.get(), freqs_dev, nfreqs * sizeof(double),
                              hipMemcpyDeviceToDevice, s),
@@ -663,28 +518,8 @@ void launch_fold(const float* ts_e, const float* ts_v, uint64_t blocks, uint32_t
     if (L.use_runs) {
         const auto g = static_cast<unsigned>(blocks);
         const auto* fr = L.freqs.as<double>();
-        switch (L.run_slots) {
-        case kRunUnits:
-            k_bf_units<TO><<<g, kBlock, unit_lds_bytes(L.nbins), s>>>(
-                ts_e, ts_v, L.seglen, L.nfreqs, L.nbins, bps, fr, L.tsamp, L.t_ref, fold, scale);
-            break;
-        case 1:
-            k_bf_runs<TO, 1><<<g, kBlock, 0, s>>>(ts_e, ts_v, L.seglen, L.nfreqs, L.nbins, bps,
-                                                  fr, L.tsamp, L.t_ref, fold, scale);
-            break;
-        case 2:
-            k_bf_runs<TO, 2><<<g, kBlock, 0, s>>>(ts_e, ts_v, L.seglen, L.nfreqs, L.nbins, bps,
-                                                  fr, L.tsamp, L.t_ref, fold, scale);
-            break;
-        case 4:
-            k_bf_runs<TO, 4><<<g, kBlock, 0, s>>>(ts_e, ts_v, L.seglen, L.nfreqs, L.nbins, bps,
-                                                  fr, L.tsamp, L.t_ref, fold, scale);
-            break;
-        default:
-            k_bf_runs<TO, 8><<<g, kBlock, 0, s>>>(ts_e, ts_v, L.seglen, L.nfreqs, L.nbins, bps,
-                                                  fr, L.tsamp, L.t_ref, fold, scale);
-            break;
-        }
+        k_bf_units<TO><<<g, kBlock, unit_lds_bytes(L.nbins), s>>>(
+            ts_e, ts_v, L.seglen, L.nfreqs, L.nbins, bps, fr, L.tsamp, L.t_ref, fold, scale);
     } else {
         k_bf_fold<true, TO><<<static_cast<unsigned>(blocks), kBlock, 0, s>>>(
             ts_e, ts_v, L.seglen, L.nfreqs, L.nbins, bps, L.list.as<uint32_t>(),
@@ -714,72 +549,41 @@ void brute_fold_time(const float* ts_e, const float* ts_v, uint64_t nseg,
 }
 
 uint32_t brute_fold_blocks_per_segment(const BruteFoldLists& L) {
-    const uint64_t pairs = static_cast<uint64_t>(L.nfreqs) * L.nbins;
-    LOKI_REQUIRE(pairs < (1ULL << 31), "brute fold: too many (freq, bin) pairs");
-    if (L.use_runs && L.run_slots == kRunUnits) {
+    if (L.use_runs) {
         const uint32_t nf = unit_nf(L.nbins);
         return (L.nfreqs + nf - 1) / nf;
     }
-    if (L.use_runs) {
-        // one lane per (frequency, group of run_slots bins); run_slots divides nbins
-        const uint64_t lanes = static_cast<uint64_t>(L.nfreqs) * (L.nbins / L.run_slots);
-        return static_cast<uint32_t>((lanes + kBlock - 1) / kBlock);
-    }
+    const uint64_t pairs = static_cast<uint64_t>(L.nfreqs) * L.nbins;
+    LOKI_REQUIRE(pairs < (1ULL << 31), "brute fold: too many (freq, bin) pairs");
     return static_cast<uint32_t>((pairs + kPairsBlk - 1) / kPairsBlk);
 }
 
-void choose_run_slots(BruteFoldLists& L, uint64_t nseg) {
-    L.run_slots = 1;
-    if (!L.use_runs) {
+template <typename TO>
+void grouped_fold(const FoldTask* tasks, uint32_t ntasks, uint32_t nblocks, bool runs,
+                  uint32_t max_nbins, hipStream_t s) {
+    if (!runs) {
+        k_bf_fold_grouped<true, TO><<<nblocks, kBlock, 0, s>>>(tasks, ntasks);
         return;
     }
-    L.run_slots = kRunUnits;
-    return;
-    const uint64_t pairs = static_cast<uint64_t>(L.nfreqs) * L.nbins;
-    for (uint32_t f : {8U, 4U, 2U}) {
-        if (L.nbins % f != 0) {
-            continue;
-        }
-        const uint64_t per = static_cast<uint64_t>(kBlock) * f;
-        if (nseg * ((pairs + per - 1) / per) >= 1024) {
-            L.run_slots = f;
-            return;
-        }
+    size_t lds = 0;  // enough for every nbins up to the launch's largest
+    for (uint32_t nb = 1; nb <= max_nbins; ++nb) {
+        lds = std::max(lds, unit_lds_bytes(nb));
     }
+    k_bf_units_grouped<TO><<<nblocks, kBlock, lds, s>>>(tasks, ntasks);
 }
 
-template <typename TO>
-void grouped_fold(const FoldTask* tasks, uint32_t ntasks, uint32_t nblocks, uint32_t run_slots,
-                  uint32_t max_nbins, hipStream_t s) {
-    switch (run_slots) {
-    case 0: k_bf_fold_grouped<true, TO><<<nblocks, kBlock, 0, s>>>(tasks, ntasks); break;
-    case 1: k_bf_runs_grouped<TO, 1><<<nblocks, kBlock, 0, s>>>(tasks, ntasks); break;
-    case 2: k_bf_runs_grouped<TO, 2><<<nblocks, kBlock, 0, s>>>(tasks, ntasks); break;
-    case 4: k_bf_runs_grouped<TO, 4><<<nblocks, kBlock, 0, s>>>(tasks, ntasks); break;
-    case kRunUnits: {
-        size_t lds = 0;  // enough for every nbins up to the launch's largest
-        for (uint32_t nb = 1; nb <= max_nbins; ++nb) {
-            lds = std::max(lds, unit_lds_bytes(nb));
-        }
-        k_bf_units_grouped<TO><<<nblocks, kBlock, lds, s>>>(tasks, ntasks);
-        break;
-    }
-    default: k_bf_runs_grouped<TO, 8><<<nblocks, kBlock, 0, s>>>(tasks, ntasks); break;
-    }
-}
-
-void brute_fold_grouped(const FoldTask* tasks, uint32_t ntasks, uint32_t nblocks,
-                        uint32_t run_slots, uint32_t max_nbins, FoldStorage so, hipStream_t s) {
+void brute_fold_grouped(const FoldTask* tasks, uint32_t ntasks, uint32_t nblocks, bool runs,
+                        uint32_t max_nbins, FoldStorage so, hipStream_t s) {
     if (ntasks == 0 || nblocks == 0) {
         return;
     }
     switch (so) {
     case FoldStorage::kF32:
-        return grouped_fold<float>(tasks, ntasks, nblocks, run_slots, max_nbins, s);
+        return grouped_fold<float>(tasks, ntasks, nblocks, runs, max_nbins, s);
     case FoldStorage::kF16:
-        return grouped_fold<_Float16>(tasks, ntasks, nblocks, run_slots, max_nbins, s);
+        return grouped_fold<_Float16>(tasks, ntasks, nblocks, runs, max_nbins, s);
     case FoldStorage::kBF16:
-        return grouped_fold<__bf16>(tasks, ntasks, nblocks, run_slots, max_nbins, s);
+        return grouped_fold<__bf16>(tasks, ntasks, nblocks, runs, max_nbins, s);
     }
 }
 

@@ -43,17 +43,11 @@ void resolve_poly(const ResolveLevel& lv, FFACoord* out, uint64_t ncoords, hipSt
 /// ascending order, stored transposed: list[base[f] + k * nbins + b].
 /// Segments longer than the LDS staging limit use no lists: their buckets are
 /// walked as contiguous runs of samples (use_runs).
-/// BruteFoldLists::run_slots value of the unit-dispatch run path (a block owns
-/// a few frequencies with all their bins; per chunk the (frequency, bin) pairs
-/// with samples in it are dealt out to the lanes).
-constexpr uint32_t kRunUnits = 16;
-
 struct BruteFoldLists {
     uint32_t nfreqs{0};
     uint32_t seglen{0};
     uint32_t nbins{0};
     bool use_runs{false};
-    uint32_t run_slots{1};  ///< run path: (freq, bin) pairs per lane (1, 2, 4, 8) or kRunUnits
     double tsamp{0.0};
     double t_ref{0.0};
     DeviceBuffer freqs;   ///< f64 [nfreqs]
@@ -188,17 +182,13 @@ struct FinalTask {
 void resolve_grouped(const ResolveTask* tasks, uint32_t ntasks, uint32_t nblocks, uint32_t np,
                      hipStream_t s);
 /// Blocks per segment of the time brute fold of `lists` (list or run path;
-/// tasks of one grouped launch must share the path and run_slots).
+/// tasks of one grouped launch must share the path).
 uint32_t brute_fold_blocks_per_segment(const BruteFoldLists& lists);
-/// Sets lists.run_slots for a fold of nseg segments: the most pairs per lane
-/// (up to 8) that keeps at least 1024 blocks.
-void choose_run_slots(BruteFoldLists& lists, uint64_t nseg);
-/// Grouped time brute fold (all tasks stored as `so`); run_slots 0: list
-/// path, kRunUnits: the unit-dispatch run path (max_nbins: the largest nbins
-/// of the tasks, which sizes its LDS), else the run path with that many pairs
-/// per lane.
-void brute_fold_grouped(const FoldTask* tasks, uint32_t ntasks, uint32_t nblocks,
-                        uint32_t run_slots, uint32_t max_nbins, FoldStorage so, hipStream_t s);
+/// Grouped time brute fold (all tasks stored as `so`): the list path, or with
+/// `runs` the long-segment run path (max_nbins: the largest nbins of the
+/// tasks, which sizes its LDS).
+void brute_fold_grouped(const FoldTask* tasks, uint32_t ntasks, uint32_t nblocks, bool runs,
+                        uint32_t max_nbins, FoldStorage so, hipStream_t s);
 /// Grouped merges of one storage type, nbins and coordinate kind.
 void merge_grouped(const MergeTask* tasks, uint32_t ntasks, uint32_t nblocks, uint32_t nbins,
                    bool poly, FoldStorage st, hipStream_t s);
