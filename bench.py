@@ -202,7 +202,7 @@ def cpu_baseline(wl: dict, ts_e, ts_v, widths, threads: int) -> dict:
 
 
 VALU_F32_PEAK_TFLOPS = 157.3
-REFINE_FLOPS_PER_TERM = 14.0  # twiddle power step (2 mul + 2 FMA) + 4 FMAs, per harmonic
+REFINE_FLOPS_PER_TERM = 12.0  # three-term power recurrence (2 FMA) + 4 accumulating FMAs
 
 
 def grid_peaks(best, ncand):

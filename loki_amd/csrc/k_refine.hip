@@ -15,8 +15,8 @@
 // the fold is recomputed from the series for the adjoint pass.
 //
 // Phases are FP64; w_mk comes from an accurate sincospi of the FP64-reduced
-// phase and an FP32 power recurrence; per-lane partial sums are FP32, block
-// and grid reductions FP64.
+// phase and an FP32 three-term power recurrence; per-lane partial sums are
+// FP32, block and grid reductions FP64.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -77,13 +77,27 @@ __device__ __forceinline__ float wave_sum(float x) {
     return x;
 }
 
-/// One step of the power recurrence w <- w z.
-__device__ __forceinline__ void twiddle_step(float& wr, float& wi, float2 z) {
-    const float nr = fmaf(wr, z.x, -(wi * z.y));
-    const float ni = fmaf(wr, z.y, wi * z.x);
-    wr             = nr;
-    wi             = ni;
-}
+/// Powers w^m of a unit twiddle w = (cos t, -sin t) by the three-term
+/// (Chebyshev) recurrence w^{m+1} = 2 cos(t) w^m - w^{m-1}, applied to the real
+/// and imaginary parts separately: two FMAs per power instead of a complex
+/// product.  Its rounding error grows at most like m^2/2 ulps (a perturbation
+/// at step j reaches step m scaled by |sin((m-j)t)/sin(t)| <= m - j), ~3e-5 at
+/// m = 32, against ~m ulps for repeated complex products.
+struct Powers {
+    float pr, pi;  // w^m (the power in use)
+    float cr, ci;  // w^{m+1}
+    float c2;      // 2 cos t
+    __device__ __forceinline__ explicit Powers(float2 z)
+        : pr(1.0F), pi(0.0F), cr(z.x), ci(z.y), c2(2.0F * z.x) {}
+    __device__ __forceinline__ void step() {
+        const float nr = fmaf(c2, cr, -pr);
+        const float ni = fmaf(c2, ci, -pi);
+        pr = cr;
+        pi = ci;
+        cr = nr;
+        ci = ni;
+    }
+};
 
 // Forward partials: block (chunk q, candidate c) -> part[c][q][NH][4] =
 // (Re E, Im E, Re V, Im V) of its samples.  Each lane takes two samples per
@@ -112,15 +126,15 @@ k_refine_fold(const float* __restrict__ ts_e, const float* __restrict__ ts_v, ui
         const float2 zb = turns_twiddle(phase_at(has_b ? kb : k, tsamp, t_ref, f, fd, fdd, &tau));
         const float xa = ts_e[k], ya = ts_v[k];
         const float xb = has_b ? ts_e[kb] : 0.0F, yb = has_b ? ts_v[kb] : 0.0F;
-        float ar = 1.0F, ai = 0.0F, br = 1.0F, bi = 0.0F;
+        Powers wa(za), wb(zb);
 #pragma unroll
         for (int m = 0; m < NH; ++m) {
-            er[m] = fmaf(xb, br, fmaf(xa, ar, er[m]));
-            ei[m] = fmaf(xb, bi, fmaf(xa, ai, ei[m]));
-            vr[m] = fmaf(yb, br, fmaf(ya, ar, vr[m]));
-            vi[m] = fmaf(yb, bi, fmaf(ya, ai, vi[m]));
-            twiddle_step(ar, ai, za);
-            twiddle_step(br, bi, zb);
+            er[m] = fmaf(xb, wb.pr, fmaf(xa, wa.pr, er[m]));
+            ei[m] = fmaf(xb, wb.pi, fmaf(xa, wa.pi, ei[m]));
+            vr[m] = fmaf(yb, wb.pr, fmaf(ya, wa.pr, vr[m]));
+            vi[m] = fmaf(yb, wb.pi, fmaf(ya, wa.pi, vi[m]));
+            wa.step();
+            wb.step();
         }
     }
     const uint32_t lane = threadIdx.x & 63U, wave = threadIdx.x >> 6;
@@ -273,16 +287,16 @@ k_refine_adjoint(const float* __restrict__ ts_e, const float* __restrict__ ts_v,
         double ta, tb;
         const float2 za = turns_twiddle(phase_at(k, tsamp, t_ref, f, fd, fdd, &ta));
         const float2 zb = turns_twiddle(phase_at(has_b ? kb : k, tsamp, t_ref, f, fd, fdd, &tb));
-        float ar = 1.0F, ai = 0.0F, br = 1.0F, bi = 0.0F;
+        Powers wa(za), wb(zb);
         float dea = 0.0F, dva = 0.0F, deb = 0.0F, dvb = 0.0F;
 #pragma unroll
         for (int m = 0; m < NH; ++m) {
-            dea = fmaf(-gi_e[m], ar, fmaf(ge[m], ai, dea));
-            dva = fmaf(-gi_v[m], ar, fmaf(gv[m], ai, dva));
-            deb = fmaf(-gi_e[m], br, fmaf(ge[m], bi, deb));
-            dvb = fmaf(-gi_v[m], br, fmaf(gv[m], bi, dvb));
-            twiddle_step(ar, ai, za);
-            twiddle_step(br, bi, zb);
+            dea = fmaf(-gi_e[m], wa.pr, fmaf(ge[m], wa.pi, dea));
+            dva = fmaf(-gi_v[m], wa.pr, fmaf(gv[m], wa.pi, dva));
+            deb = fmaf(-gi_e[m], wb.pr, fmaf(ge[m], wb.pi, deb));
+            dvb = fmaf(-gi_v[m], wb.pr, fmaf(gv[m], wb.pi, dvb));
+            wa.step();
+            wb.step();
         }
         const double pa = static_cast<double>(dea) * ts_e[k] + static_cast<double>(dva) * ts_v[k];
         const double pb = has_b ? static_cast<double>(deb) * ts_e[kb] +
