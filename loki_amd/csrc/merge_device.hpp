@@ -238,7 +238,7 @@ __device__ __forceinline__ void merge_v4_block(uint32_t bid, const TI* __restric
                     o.z *= scale;
                     o.w *= scale;
                 }
-                orow[lch + k * LANES] = narrow4<TO>(o);
+                store_stream(&orow[lch + k * LANES], narrow4<TO>(o));
             }
         }
     }

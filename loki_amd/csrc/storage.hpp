@@ -26,6 +26,18 @@ template <> struct Chunk4<_Float16> { using type = uint2; };
 template <> struct Chunk4<__bf16> { using type = uint2; };
 template <typename T> using chunk4_t = typename Chunk4<T>::type;
 
+/// Streaming (non-temporal) store of one chunk: fold levels are far larger
+/// than the caches and are next read by another launch, so their writes should
+/// not evict the parent rows that the current launch is still re-reading.
+__device__ __forceinline__ void store_stream(float4* p, const float4 v) {
+    using v4 = float __attribute__((ext_vector_type(4)));
+    __builtin_nontemporal_store(v4{v.x, v.y, v.z, v.w}, reinterpret_cast<v4*>(p));
+}
+__device__ __forceinline__ void store_stream(uint2* p, const uint2 v) {
+    using u2 = unsigned int __attribute__((ext_vector_type(2)));
+    __builtin_nontemporal_store(u2{v.x, v.y}, reinterpret_cast<u2*>(p));
+}
+
 __device__ __forceinline__ float f16_lo(uint32_t w) {
     return static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(w & 0xFFFFU)));
 }
