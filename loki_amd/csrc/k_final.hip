@@ -91,7 +91,7 @@ __device__ __forceinline__ void final_block(uint32_t bid, const TI* __restrict__
     constexpr uint32_t LPR = C4;            // lanes per row (4 bins of e and v each)
     constexpr uint32_t RPI = kRows / LPR;   // rows per merge step
     constexpr uint32_t NSTEP = kRows / RPI; // merge steps for a full tile
-    constexpr uint32_t U   = NSTEP >= 4 ? 4 : NSTEP;  // rows with loads in flight
+    constexpr uint32_t U   = NSTEP >= 8 ? 8 : NSTEP;  // row steps with loads in flight
     extern __shared__ float s_p[];          // [kRows][stride]
     __shared__ FFACoord s_coord[kRows];
     const uint32_t stride = fg.stride;
