@@ -102,6 +102,10 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--no-storage-compare", action="store_true",
                     help="skip timing the FP16 / BF16 storage variants beside the main run")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cpu count)")
+    ap.add_argument("--no-score-filter", action="store_true",
+                    help="skip the score-and-filter timing")
+    ap.add_argument("--filter-threshold", type=float, default=6.0,
+                    help="S/N threshold of the score-and-filter timing")
     ap.add_argument("--no-refine", action="store_true",
                     help="Fourier workload: skip the gradient-ascent refinement leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -570,6 +574,33 @@ def main() -> int:
                     "FP64 is stated by tests/test_gpu_parity.py::test_storage_snr_error_vs_fp64",
         }
 
+    # Score-and-filter (the FFAFreqSweep step, ffa_freq_sweep_cuda.cu:219-260): the same FFA with
+    # only the (trial, width) pairs above the threshold leaving the last-level kernel.
+    score_filter = None
+    if not args.no_score_filter:
+        cap = 1 << 24
+        f_idx = torch.empty(cap, dtype=torch.int32, device=dev)
+        f_snr = torch.empty(cap, dtype=torch.float32, device=dev)
+        f_cnt = torch.zeros(1, dtype=torch.int32, device=dev)
+
+        def fstep():
+            ffa.scores_filter_device(d_e.data_ptr(), d_v.data_ptr(), widths,
+                                     args.filter_threshold, False, f_idx.data_ptr(),
+                                     f_snr.data_ptr(), cap, f_cnt.data_ptr(), stream.cuda_stream)
+
+        with torch.cuda.stream(stream):
+            fstep()
+        torch.cuda.synchronize()
+        tf0 = time.perf_counter()
+        with torch.cuda.stream(stream):
+            for _ in range(args.steps):
+                fstep()
+        torch.cuda.synchronize()
+        f_ms = (time.perf_counter() - tf0) * 1e3 / args.steps
+        score_filter = {"threshold": args.filter_threshold, "ms_per_step": f_ms,
+                        "param_trials_per_s": nc / (f_ms * 1e-3),
+                        "candidates": int(f_cnt.item()), "capacity": cap}
+
     # signal recovery check (outside the timed region)
     best = int(torch.argmax(scores).item())
     shape = [int(x) for x in plan.fold_shapes[-1][1:-2]]
@@ -629,6 +660,7 @@ def main() -> int:
         "pipeline_alg_GBps": total_alg / (ms_per_step * 1e-3) / 1e9,
         "kernel_ms": kern_ms,
         "storage_compare": storage_cmp,
+        "score_filter": score_filter,
         "found": found,
         "injected": {"freq": sig["f0"], "accel": sig["accel"], "snr": sig["snr"]},
     }

@@ -222,6 +222,25 @@ int loki_ffa_set_fold_storage(loki_ffa_t ffa, int storage);
 /* Waits for the recorded events; *n receives the number of timed launches. */
 int loki_ffa_timings(loki_ffa_t ffa, loki_kernel_timing* out, size_t cap, size_t* n);
 
+/* Score and filter (score_cuda.cu:382-443 score_and_filter_cuda_d / _max_cuda_d, used by the
+ * FFAFreqSweepCUDA pipeline, ffa_freq_sweep_cuda.cu:240-257): the FFA's final-level S/N, kept
+ * only where it exceeds `threshold`.  Per width (find_max 0): every (trial, width) with
+ * snr > threshold, index = trial * nwidths + width; find_max 1: every trial whose best S/N
+ * exceeds it, index = trial.  The final fold and the full score array are never stored (the
+ * last level is merged, scored and compacted in one kernel where nbins allows).  Device form:
+ * indices/scores/count are device buffers, `count` (uint32) is zeroed on `stream` and receives
+ * the number that passed, which may exceed `capacity` (only `capacity` entries are written,
+ * in no fixed order).  Host form: synchronous, results sorted by index, *count as above.
+ * trial * nwidths must fit in 32 bits (the reference's uint32 indices). */
+int loki_ffa_scores_filter_device(loki_ffa_t ffa, const float* ts_e, const float* ts_v,
+                                  const size_t* widths, size_t nwidths, float threshold,
+                                  int find_max, uint32_t* indices, float* scores, size_t capacity,
+                                  uint32_t* count, void* stream);
+int loki_ffa_scores_filter_host(loki_ffa_t ffa, const float* ts_e, const float* ts_v,
+                                size_t nsamps, const size_t* widths, size_t nwidths,
+                                float threshold, int find_max, uint32_t* indices, float* scores,
+                                size_t capacity, size_t* count);
+
 /* --- Batched multi-series FFA ------------------------------------------
  * No reference counterpart: the reference searches series one FFACUDA object at a time, with
  * host synchronisations per level (ffa_freq_sweep_cuda.cu:122-136, kernels_cuda.cu:1081-1082).
@@ -305,6 +324,16 @@ int loki_snr_boxcar_3d_device(const float* folds, size_t nprof, size_t nbins,
 int loki_snr_boxcar_3d_host(const float* folds, size_t nprof, size_t nbins,
                             const size_t* widths, size_t nwidths, float* scores, int find_max,
                             int device_id);
+/* score_and_filter_cuda_d / score_and_filter_max_cuda_d (score_cuda.cu:382-443) on folds
+ * [nprof, 2, nbins]: the filtered outputs of loki_ffa_scores_filter_* above. */
+int loki_snr_boxcar_3d_filter_device(const float* folds, size_t nprof, size_t nbins,
+                                     const size_t* widths, size_t nwidths, float threshold,
+                                     int find_max, uint32_t* indices, float* scores,
+                                     size_t capacity, uint32_t* count, void* stream);
+int loki_snr_boxcar_3d_filter_host(const float* folds, size_t nprof, size_t nbins,
+                                   const size_t* widths, size_t nwidths, float threshold,
+                                   int find_max, uint32_t* indices, float* scores,
+                                   size_t capacity, size_t* count, int device_id);
 /* folds [nprof, nbins] -> [nprof] max over widths of snr / stdnoise. */
 int loki_snr_boxcar_2d_max_host(const float* folds, size_t nprof, size_t nbins,
                                 const size_t* widths, size_t nwidths, float stdnoise,

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -412,6 +413,100 @@ int loki_ffa_scores_host(loki_ffa_t ffa, const float* ts_e, const float* ts_v, s
     });
 }
 
+namespace {
+
+/// Device buffers of one host-side filtered call, copied back sorted by index
+/// (the device order depends on scheduling).
+struct FilterBuffers {
+    loki_amd::DeviceBuffer idx, snr, cnt;
+    loki_amd::kern::FilterOut fo{};
+    FilterBuffers(float threshold, size_t capacity)
+        : idx(std::max<size_t>(capacity, 1) * 4), snr(std::max<size_t>(capacity, 1) * 4),
+          cnt(4) {
+        fo.threshold = threshold;
+        fo.capacity  = static_cast<uint32_t>(capacity);
+        fo.count     = cnt.as<uint32_t>();
+        fo.indices   = idx.as<uint32_t>();
+        fo.snr       = snr.as<float>();
+    }
+    size_t copy_out(uint32_t* indices, float* scores) const {
+        uint32_t n = 0;
+        loki_amd::hip_check(hipMemcpy(&n, cnt.get(), 4, hipMemcpyDeviceToHost), "D2H count");
+        const size_t k = std::min<size_t>(n, fo.capacity);
+        std::vector<uint32_t> vi(k);
+        std::vector<float> vs(k);
+        if (k > 0) {
+            loki_amd::hip_check(hipMemcpy(vi.data(), idx.get(), k * 4, hipMemcpyDeviceToHost),
+                                "D2H indices");
+            loki_amd::hip_check(hipMemcpy(vs.data(), snr.get(), k * 4, hipMemcpyDeviceToHost),
+                                "D2H scores");
+        }
+        std::vector<size_t> order(k);
+        for (size_t i = 0; i < k; ++i) {
+            order[i] = i;
+        }
+        std::sort(order.begin(), order.end(),
+                  [&](size_t a, size_t b) { return vi[a] < vi[b]; });
+        for (size_t i = 0; i < k; ++i) {
+            indices[i] = vi[order[i]];
+            scores[i]  = vs[order[i]];
+        }
+        return n;
+    }
+};
+
+loki_amd::kern::FilterOut filter_out(float threshold, uint32_t* indices, float* scores,
+                                     size_t capacity, uint32_t* count) {
+    REQ(count != nullptr, "count is null");
+    REQ(capacity == 0 || (indices != nullptr && scores != nullptr), "null output buffer");
+    REQ(capacity <= 0xFFFFFFFFULL, "capacity exceeds 32 bits");
+    loki_amd::kern::FilterOut fo{};
+    fo.threshold = threshold;
+    fo.capacity  = static_cast<uint32_t>(capacity);
+    fo.count     = count;
+    fo.indices   = indices;
+    fo.snr       = scores;
+    return fo;
+}
+
+}  // namespace
+
+int loki_ffa_scores_filter_device(loki_ffa_t ffa, const float* ts_e, const float* ts_v,
+                                  const size_t* widths, size_t nwidths, float threshold,
+                                  int find_max, uint32_t* indices, float* scores, size_t capacity,
+                                  uint32_t* count, void* stream) {
+    return guarded([&] {
+        REQ(ffa != nullptr && widths != nullptr, "null argument");
+        const auto fo = filter_out(threshold, indices, scores, capacity, count);
+        ffa->exec.execute_scores(ts_e, ts_v, std::span<const size_t>(widths, nwidths), nullptr,
+                                 0, find_max != 0, as_stream(stream), &fo);
+    });
+}
+
+int loki_ffa_scores_filter_host(loki_ffa_t ffa, const float* ts_e, const float* ts_v,
+                                size_t nsamps, const size_t* widths, size_t nwidths,
+                                float threshold, int find_max, uint32_t* indices, float* scores,
+                                size_t capacity, size_t* count) {
+    return guarded([&] {
+        REQ(ffa != nullptr && ts_e != nullptr && ts_v != nullptr && widths != nullptr &&
+                count != nullptr,
+            "null argument");
+        REQ(capacity == 0 || (indices != nullptr && scores != nullptr), "null output buffer");
+        REQ(capacity <= 0xFFFFFFFFULL, "capacity exceeds 32 bits");
+        const auto& pl = ffa->exec.plan();
+        REQ(nsamps == pl.config().get_nsamps(), "FFA::execute: ts_e must have size nsamps");
+        loki_amd::DeviceGuard g(ffa->exec.device_id());
+        loki_amd::DeviceBuffer de(nsamps * 4), dv(nsamps * 4);
+        loki_amd::hip_check(hipMemcpy(de.get(), ts_e, nsamps * 4, hipMemcpyHostToDevice), "H2D");
+        loki_amd::hip_check(hipMemcpy(dv.get(), ts_v, nsamps * 4, hipMemcpyHostToDevice), "H2D");
+        FilterBuffers fb(threshold, capacity);
+        ffa->exec.execute_scores(de.as<float>(), dv.as<float>(),
+                                 std::span<const size_t>(widths, nwidths), nullptr, 0,
+                                 find_max != 0, nullptr, &fb.fo);
+        *count = fb.copy_out(indices, scores);
+    });
+}
+
 int loki_ffa_set_timing(loki_ffa_t ffa, int enable) {
     return guarded([&] {
         REQ(ffa != nullptr, "null ffa");
@@ -787,6 +882,46 @@ int loki_snr_boxcar_3d_host(const float* folds, size_t nprof, size_t nbins,
         loki_amd::hip_check(hipGetLastError(), "score launch");
         loki_amd::hip_check(hipMemcpy(scores, dout.get(), out_len * 4, hipMemcpyDeviceToHost),
                             "D2H");
+    });
+}
+
+int loki_snr_boxcar_3d_filter_device(const float* folds, size_t nprof, size_t nbins,
+                                     const size_t* widths, size_t nwidths, float threshold,
+                                     int find_max, uint32_t* indices, float* scores,
+                                     size_t capacity, uint32_t* count, void* stream) {
+    return guarded([&] {
+        REQ(folds != nullptr && widths != nullptr, "null argument");
+        const auto wt = loki_amd::kern::make_width_table(widths, nwidths, nbins);
+        const auto fo = filter_out(threshold, indices, scores, capacity, count);
+        loki_amd::kern::zero_counter(count, as_stream(stream));
+        loki_amd::kern::snr_boxcar_3d(folds, nprof, static_cast<uint32_t>(nbins), wt, nullptr,
+                                      find_max != 0, as_stream(stream), &fo);
+        loki_amd::hip_check(hipGetLastError(), "score filter launch");
+    });
+}
+
+int loki_snr_boxcar_3d_filter_host(const float* folds, size_t nprof, size_t nbins,
+                                   const size_t* widths, size_t nwidths, float threshold,
+                                   int find_max, uint32_t* indices, float* scores,
+                                   size_t capacity, size_t* count, int device_id) {
+    return guarded([&] {
+        REQ(folds != nullptr && widths != nullptr && count != nullptr, "null argument");
+        REQ(capacity == 0 || (indices != nullptr && scores != nullptr), "null output buffer");
+        REQ(capacity <= 0xFFFFFFFFULL, "capacity exceeds 32 bits");
+        const auto wt = loki_amd::kern::make_width_table(widths, nwidths, nbins);
+        loki_amd::DeviceGuard g(device_id);
+        const size_t in_len = nprof * 2 * nbins;
+        loki_amd::DeviceBuffer di(std::max<size_t>(in_len, 1) * 4);
+        if (in_len > 0) {
+            loki_amd::hip_check(hipMemcpy(di.get(), folds, in_len * 4, hipMemcpyHostToDevice),
+                                "H2D");
+        }
+        FilterBuffers fb(threshold, capacity);
+        loki_amd::kern::zero_counter(fb.fo.count, nullptr);
+        loki_amd::kern::snr_boxcar_3d(di.as<float>(), nprof, static_cast<uint32_t>(nbins), wt,
+                                      nullptr, find_max != 0, nullptr, &fb.fo);
+        loki_amd::hip_check(hipGetLastError(), "score filter launch");
+        *count = fb.copy_out(indices, scores);
     });
 }
 

@@ -376,7 +376,8 @@ bool FFAExecutor::fused_scores_supported() const {
 
 void FFAExecutor::execute_scores(const float* ts_e, const float* ts_v,
                                  std::span<const SizeType> widths, float* scores,
-                                 SizeType scores_len, bool find_max, hipStream_t s) {
+                                 SizeType scores_len, bool find_max, hipStream_t s,
+                                 const kern::FilterOut* fo) {
     DeviceGuard g(m_device);
     LOKI_REQUIRE(m_plan.nsegments().back() == 1,
                  "compute_ffa_scores: nsegments must be 1 for scores");
@@ -385,9 +386,16 @@ void FFAExecutor::execute_scores(const float* ts_e, const float* ts_v,
     const SizeType nbins   = m_plan.config().get_nbins();
     const SizeType ncoords = m_plan.ncoords().back();
     const auto wt          = kern::make_width_table(widths.data(), widths.size(), nbins);
-    LOKI_REQUIRE(scores != nullptr, "scores buffer is null");
-    LOKI_REQUIRE(scores_len == (find_max ? ncoords : ncoords * widths.size()),
-                 "scores must have size ncoords * nwidths (or ncoords for the max)");
+    if (fo == nullptr) {
+        LOKI_REQUIRE(scores != nullptr, "scores buffer is null");
+        LOKI_REQUIRE(scores_len == (find_max ? ncoords : ncoords * widths.size()),
+                     "scores must have size ncoords * nwidths (or ncoords for the max)");
+    } else {
+        LOKI_REQUIRE(fo->count != nullptr &&
+                         (fo->capacity == 0 || (fo->indices != nullptr && fo->snr != nullptr)),
+                     "score filter: null output buffer");
+        kern::zero_counter(fo->count, s);
+    }
     if (m_fused && fused_scores_supported()) {
         // Levels 0..L-2 through HBM, the last level fused with its shift
         // tables and the S/N: the final fold cube is never stored.
@@ -400,7 +408,7 @@ void FFAExecutor::execute_scores(const float* ts_e, const float* ts_v,
         const int slot = m_timer.start(KernelKind::kMergeScore, static_cast<int>(L - 1), s);
         kern::final_merge_score(parent, m_storage, std::ldexp(1.0F, m_k0 + static_cast<int>(L) - 2),
                                 ncoords, m_plan.ncoords()[L - 2], lv, wt, scores, find_max,
-                                m_exact_norm, s);
+                                m_exact_norm, s, fo);
         check_launch("final-level merge+score launch");
         m_timer.stop(slot, s);
         return;
@@ -414,7 +422,7 @@ void FFAExecutor::execute_scores(const float* ts_e, const float* ts_v,
         const int slot =
             m_timer.start(KernelKind::kTransformScore, static_cast<int>(L - 1), s);
         kern::irfft_score(static_cast<const float2*>(last), ncoords, static_cast<uint32_t>(nbins),
-                          wt, scores, find_max, m_exact_norm, s);
+                          wt, scores, find_max, m_exact_norm, s, fo);
         check_launch("irfft+score launch");
         m_timer.stop(slot, s);
         return;
@@ -432,7 +440,7 @@ void FFAExecutor::execute_scores(const float* ts_e, const float* ts_v,
         fin = tf;
     }
     const int slot = m_timer.start(KernelKind::kScore, static_cast<int>(L - 1), s);
-    kern::snr_boxcar_3d(fin, ncoords, static_cast<uint32_t>(nbins), wt, scores, find_max, s);
+    kern::snr_boxcar_3d(fin, ncoords, static_cast<uint32_t>(nbins), wt, scores, find_max, s, fo);
     check_launch("score launch");
     m_timer.stop(slot, s);
 }

@@ -89,8 +89,13 @@ public:
     /// FFA + boxcar S/N of the final level (nsegments must be 1): scores
     /// [ncoords_final, nw] (or [ncoords_final] when find_max).  A Fourier plan
     /// returns its final level to the time domain first (ffa_cuda.cu:670-692).
+    /// With `fo` the scores are filtered instead (score_cuda.cu:382-443): every
+    /// (trial, width) -- or trial, with find_max -- whose S/N exceeds
+    /// fo->threshold goes to the compacted list; `scores` is unused and
+    /// fo->count is zeroed here.
     void execute_scores(const float* ts_e, const float* ts_v, std::span<const SizeType> widths,
-                        float* scores, SizeType scores_len, bool find_max, hipStream_t s);
+                        float* scores, SizeType scores_len, bool find_max, hipStream_t s,
+                        const kern::FilterOut* fo = nullptr);
 
     /// Resolve the shift tables of levels 1..last_level into device memory
     /// (at their ncoords_offsets positions).

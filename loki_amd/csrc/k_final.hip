@@ -85,7 +85,7 @@ __device__ __forceinline__ void final_block(uint32_t bid, const TI* __restrict__
                                             uint64_t nprev, const ResolveLevel& lv,
                                             const FinalGrid& fg, const WidthTable& wt,
                                             float* __restrict__ scores, const TileSchedule& ts,
-                                            float unscale) {
+                                            float unscale, const FilterOut& fo) {
     using CI = chunk4_t<TI>;
     constexpr uint32_t C4  = NB / 4;        // float4 chunks per component row
     constexpr uint32_t LPR = C4;            // lanes per row (4 bins of e and v each)
@@ -181,21 +181,28 @@ __device__ __forceinline__ void final_block(uint32_t bid, const TI* __restrict__
         const float snr  = ((wt.h[iw] + wt.b[iw]) * dmax) - (wt.b[iw] * sum);
         if constexpr (FINDMAX) {
             best = (best < snr) ? snr : best;
+        } else if (fo.count != nullptr) {
+            filter_emit(fo, snr > fo.threshold, static_cast<uint32_t>(c * wt.n + iw), snr);
         } else {
             scores[c * wt.n + iw] = snr;
         }
     }
     if constexpr (FINDMAX) {
-        scores[c] = best;
+        if (fo.count != nullptr) {
+            filter_emit(fo, best > fo.threshold, static_cast<uint32_t>(c), best);
+        } else {
+            scores[c] = best;
+        }
     }
 }
 
 template <int NB, int NP, bool FINDMAX, bool EXACT, typename TI>
 __global__ void __launch_bounds__(kRows)
 k_final_merge_score(const TI* __restrict__ in, uint64_t nprev, ResolveLevel lv, FinalGrid fg,
-                    WidthTable wt, float* __restrict__ scores, TileSchedule ts, float unscale) {
+                    WidthTable wt, float* __restrict__ scores, TileSchedule ts, float unscale,
+                    FilterOut fo) {
     final_block<NB, NP, FINDMAX, EXACT, TI>(blockIdx.x, in, nprev, lv, fg, wt, scores, ts,
-                                            unscale);
+                                            unscale, fo);
 }
 
 // Grouped launch: the last levels of many independent FFAs (one task each,
@@ -217,13 +224,13 @@ k_final_grouped(const FinalTask* __restrict__ tasks, uint32_t ntasks) {
     const FinalTask& t = tasks[lo];
     final_block<NB, NP, FINDMAX, EXACT, TI>(blockIdx.x - t.block0, static_cast<const TI*>(t.in),
                                             t.nprev, t.lv, t.fg, t.wt, t.scores, t.ts,
-                                            t.unscale);
+                                            t.unscale, FilterOut{});
 }
 
 template <int NB, int NP, typename TI>
 void launch_nb_np(const TI* in, float unscale, uint64_t ncur, uint64_t nprev,
                   const ResolveLevel& lv, const WidthTable& wt, float* scores, bool find_max,
-                  bool exact, hipStream_t s) {
+                  bool exact, const FilterOut& fo, hipStream_t s) {
     // nseg == 1 at the last level; child rows sharing parents run back to back
     const TileSchedule ts = make_schedule(ncur, lv.n_cur[2], 1, kRows, kRowGroup, kRun);
     const uint32_t nb     = ts.nblocks;
@@ -235,30 +242,33 @@ void launch_nb_np(const TI* in, float unscale, uint64_t ncur, uint64_t nprev,
     const size_t lds = static_cast<size_t>(kRows) * fg.stride * sizeof(float);
     if (find_max && exact) {
         k_final_merge_score<NB, NP, true, true, TI>
-            <<<nb, kRows, lds, s>>>(in, nprev, lv, fg, wt, scores, ts, unscale);
+            <<<nb, kRows, lds, s>>>(in, nprev, lv, fg, wt, scores, ts, unscale, fo);
     } else if (find_max) {
         k_final_merge_score<NB, NP, true, false, TI>
-            <<<nb, kRows, lds, s>>>(in, nprev, lv, fg, wt, scores, ts, unscale);
+            <<<nb, kRows, lds, s>>>(in, nprev, lv, fg, wt, scores, ts, unscale, fo);
     } else if (exact) {
         k_final_merge_score<NB, NP, false, true, TI>
-            <<<nb, kRows, lds, s>>>(in, nprev, lv, fg, wt, scores, ts, unscale);
+            <<<nb, kRows, lds, s>>>(in, nprev, lv, fg, wt, scores, ts, unscale, fo);
     } else {
         k_final_merge_score<NB, NP, false, false, TI>
-            <<<nb, kRows, lds, s>>>(in, nprev, lv, fg, wt, scores, ts, unscale);
+            <<<nb, kRows, lds, s>>>(in, nprev, lv, fg, wt, scores, ts, unscale, fo);
     }
 }
 
 template <int NB, typename TI>
 void launch_nb(const TI* in, float unscale, uint64_t ncur, uint64_t nprev,
                const ResolveLevel& lv, const WidthTable& wt, float* scores, bool find_max,
-               bool exact, hipStream_t s) {
+               bool exact, const FilterOut& fo, hipStream_t s) {
     switch (lv.np) {
     case 1:
-        return launch_nb_np<NB, 1>(in, unscale, ncur, nprev, lv, wt, scores, find_max, exact, s);
+        return launch_nb_np<NB, 1>(in, unscale, ncur, nprev, lv, wt, scores, find_max, exact, fo,
+                                   s);
     case 2:
-        return launch_nb_np<NB, 2>(in, unscale, ncur, nprev, lv, wt, scores, find_max, exact, s);
+        return launch_nb_np<NB, 2>(in, unscale, ncur, nprev, lv, wt, scores, find_max, exact, fo,
+                                   s);
     case 3:
-        return launch_nb_np<NB, 3>(in, unscale, ncur, nprev, lv, wt, scores, find_max, exact, s);
+        return launch_nb_np<NB, 3>(in, unscale, ncur, nprev, lv, wt, scores, find_max, exact, fo,
+                                   s);
     default: LOKI_REQUIRE(false, "final-level kernel: np must be 1, 2 or 3");
     }
 }
@@ -266,13 +276,16 @@ void launch_nb(const TI* in, float unscale, uint64_t ncur, uint64_t nprev,
 template <typename TI>
 void launch_typed(const void* in, float unscale, uint64_t ncur, uint64_t nprev,
                   const ResolveLevel& lv, const WidthTable& wt, float* scores, bool find_max,
-                  bool exact, hipStream_t s) {
+                  bool exact, const FilterOut& fo, hipStream_t s) {
     const auto* p = static_cast<const TI*>(in);
     switch (lv.nbins) {
-    case 8: return launch_nb<8>(p, unscale, ncur, nprev, lv, wt, scores, find_max, exact, s);
-    case 16: return launch_nb<16>(p, unscale, ncur, nprev, lv, wt, scores, find_max, exact, s);
-    case 32: return launch_nb<32>(p, unscale, ncur, nprev, lv, wt, scores, find_max, exact, s);
-    case 64: return launch_nb<64>(p, unscale, ncur, nprev, lv, wt, scores, find_max, exact, s);
+    case 8: return launch_nb<8>(p, unscale, ncur, nprev, lv, wt, scores, find_max, exact, fo, s);
+    case 16:
+        return launch_nb<16>(p, unscale, ncur, nprev, lv, wt, scores, find_max, exact, fo, s);
+    case 32:
+        return launch_nb<32>(p, unscale, ncur, nprev, lv, wt, scores, find_max, exact, fo, s);
+    case 64:
+        return launch_nb<64>(p, unscale, ncur, nprev, lv, wt, scores, find_max, exact, fo, s);
     default: break;
     }
 }
@@ -285,20 +298,29 @@ bool final_merge_score_supported(uint32_t nbins) {
 
 void final_merge_score(const void* in, FoldStorage si, float unscale, uint64_t ncur,
                        uint64_t nprev, const ResolveLevel& lv, const WidthTable& wt,
-                       float* scores, bool find_max, bool exact, hipStream_t s) {
+                       float* scores, bool find_max, bool exact, hipStream_t s,
+                       const FilterOut* fo) {
     LOKI_REQUIRE(final_merge_score_supported(lv.nbins), "final-level kernel: unsupported nbins");
     LOKI_REQUIRE(wt.wmax < lv.nbins, "boxcar widths must be < nbins");
     if (ncur == 0) {
         return;
     }
+    const FilterOut f = fo != nullptr ? *fo : FilterOut{};
+    if (fo != nullptr) {
+        LOKI_REQUIRE(fo->count != nullptr, "score filter: null counter");
+        LOKI_REQUIRE(ncur * (find_max ? 1U : wt.n) <= 0xFFFFFFFFULL,
+                     "score filter: trial index exceeds 32 bits");
+    }
     switch (si) {
     case FoldStorage::kF32:
-        return launch_typed<float>(in, unscale, ncur, nprev, lv, wt, scores, find_max, exact, s);
+        return launch_typed<float>(in, unscale, ncur, nprev, lv, wt, scores, find_max, exact, f,
+                                   s);
     case FoldStorage::kF16:
         return launch_typed<_Float16>(in, unscale, ncur, nprev, lv, wt, scores, find_max, exact,
-                                      s);
+                                      f, s);
     case FoldStorage::kBF16:
-        return launch_typed<__bf16>(in, unscale, ncur, nprev, lv, wt, scores, find_max, exact, s);
+        return launch_typed<__bf16>(in, unscale, ncur, nprev, lv, wt, scores, find_max, exact, f,
+                                    s);
     }
 }
 

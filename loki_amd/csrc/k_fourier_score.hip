@@ -59,7 +59,7 @@ __device__ __forceinline__ float irdft_basis(const float2* tw, int k, int n, int
 template <int NB, bool FINDMAX, bool EXACT>
 __global__ void __launch_bounds__(64)
 k_irfft_score(const float* __restrict__ fold, uint64_t ntrials, WidthTable wt, uint32_t stride,
-              float* __restrict__ scores) {
+              float* __restrict__ scores, FilterOut fo) {
     using G = IrsGeom<NB>;
     extern __shared__ float lds[];
     float* __restrict__ stage = lds;                  // [kTile]
@@ -175,12 +175,19 @@ k_irfft_score(const float* __restrict__ fold, uint64_t ntrials, WidthTable wt, u
                 const float snr = ((wt.h[iw] + wt.b[iw]) * dmax) - (wt.b[iw] * sum);
                 if constexpr (FINDMAX) {
                     best = (best < snr) ? snr : best;
+                } else if (fo.count != nullptr) {
+                    filter_emit(fo, snr > fo.threshold, static_cast<uint32_t>(c * wt.n + iw),
+                                snr);
                 } else {
                     scores[c * wt.n + iw] = snr;
                 }
             }
             if constexpr (FINDMAX) {
-                scores[c] = best;
+                if (fo.count != nullptr) {
+                    filter_emit(fo, best > fo.threshold, static_cast<uint32_t>(c), best);
+                } else {
+                    scores[c] = best;
+                }
             }
         }
         wave_sync();  // rows are rewritten by the next group
@@ -189,7 +196,7 @@ k_irfft_score(const float* __restrict__ fold, uint64_t ntrials, WidthTable wt, u
 
 template <int NB, bool FINDMAX, bool EXACT>
 void launch_irs(const float2* fold, uint64_t ntrials, const WidthTable& wt, float* scores,
-                hipStream_t s) {
+                const FilterOut& fo, hipStream_t s) {
     using G = IrsGeom<NB>;
     const uint32_t stride = NB + 1;  // odd: conflict-free per-lane rows
     const size_t lds = (static_cast<size_t>(G::kTile) + kWaveTrials * stride + 2 * NB) *
@@ -198,18 +205,18 @@ void launch_irs(const float2* fold, uint64_t ntrials, const WidthTable& wt, floa
     const uint64_t ngroups = (ntrials + kWaveTrials - 1) / kWaveTrials;
     const uint64_t nblocks = ngroups < 8192 ? ngroups : 8192;
     k_irfft_score<NB, FINDMAX, EXACT><<<static_cast<uint32_t>(nblocks), 64, lds, s>>>(
-        reinterpret_cast<const float*>(fold), ntrials, wt, stride, scores);
+        reinterpret_cast<const float*>(fold), ntrials, wt, stride, scores, fo);
 }
 
 template <int NB>
 void dispatch_irs(const float2* fold, uint64_t ntrials, const WidthTable& wt, float* scores,
-                  bool find_max, bool exact, hipStream_t s) {
+                  bool find_max, bool exact, const FilterOut& fo, hipStream_t s) {
     if (find_max) {
-        return exact ? launch_irs<NB, true, true>(fold, ntrials, wt, scores, s)
-                     : launch_irs<NB, true, false>(fold, ntrials, wt, scores, s);
+        return exact ? launch_irs<NB, true, true>(fold, ntrials, wt, scores, fo, s)
+                     : launch_irs<NB, true, false>(fold, ntrials, wt, scores, fo, s);
     }
-    return exact ? launch_irs<NB, false, true>(fold, ntrials, wt, scores, s)
-                 : launch_irs<NB, false, false>(fold, ntrials, wt, scores, s);
+    return exact ? launch_irs<NB, false, true>(fold, ntrials, wt, scores, fo, s)
+                 : launch_irs<NB, false, false>(fold, ntrials, wt, scores, fo, s);
 }
 
 }  // namespace
@@ -217,15 +224,21 @@ void dispatch_irs(const float2* fold, uint64_t ntrials, const WidthTable& wt, fl
 bool irfft_score_supported(uint32_t nbins) { return nbins == 16 || nbins == 32 || nbins == 64; }
 
 void irfft_score(const float2* fold, uint64_t ntrials, uint32_t nbins, const WidthTable& wt,
-                 float* scores, bool find_max, bool exact, hipStream_t s) {
+                 float* scores, bool find_max, bool exact, hipStream_t s, const FilterOut* fo) {
     LOKI_REQUIRE(irfft_score_supported(nbins), "irfft+score: nbins must be 16, 32 or 64");
     if (ntrials == 0) {
         return;
     }
+    const FilterOut f = fo != nullptr ? *fo : FilterOut{};
+    if (fo != nullptr) {
+        LOKI_REQUIRE(fo->count != nullptr, "score filter: null counter");
+        LOKI_REQUIRE(ntrials * (find_max ? 1U : wt.n) <= 0xFFFFFFFFULL,
+                     "score filter: trial index exceeds 32 bits");
+    }
     switch (nbins) {
-    case 16: return dispatch_irs<16>(fold, ntrials, wt, scores, find_max, exact, s);
-    case 32: return dispatch_irs<32>(fold, ntrials, wt, scores, find_max, exact, s);
-    default: return dispatch_irs<64>(fold, ntrials, wt, scores, find_max, exact, s);
+    case 16: return dispatch_irs<16>(fold, ntrials, wt, scores, find_max, exact, f, s);
+    case 32: return dispatch_irs<32>(fold, ntrials, wt, scores, find_max, exact, f, s);
+    default: return dispatch_irs<64>(fold, ntrials, wt, scores, find_max, exact, f, s);
     }
 }
 

@@ -26,7 +26,7 @@ __host__ __device__ inline uint32_t psum_stride(uint32_t nbins, uint32_t wmax) {
 template <int NB, bool IS3D, bool FINDMAX>
 __global__ void __launch_bounds__(kLanes)
 k_snr_reg(const float* __restrict__ folds, uint64_t nprof, WidthTable wt, float inv_stdnoise,
-          float* __restrict__ scores) {
+          float* __restrict__ scores, FilterOut fo) {
     extern __shared__ float s_ps[];
     const uint64_t p = static_cast<uint64_t>(blockIdx.x) * kLanes + threadIdx.x;
     if (p >= nprof) {
@@ -85,12 +85,18 @@ k_snr_reg(const float* __restrict__ folds, uint64_t nprof, WidthTable wt, float 
         const float snr = (((wt.h[iw] + wt.b[iw]) * dmax) - (wt.b[iw] * sum)) * inv_stdnoise;
         if constexpr (FINDMAX) {
             best = (best < snr) ? snr : best;
+        } else if (fo.count != nullptr) {
+            filter_emit(fo, snr > fo.threshold, static_cast<uint32_t>(p * wt.n + iw), snr);
         } else {
             scores[p * wt.n + iw] = snr;
         }
     }
     if constexpr (FINDMAX) {
-        scores[p] = best;
+        if (fo.count != nullptr) {
+            filter_emit(fo, best > fo.threshold, static_cast<uint32_t>(p), best);
+        } else {
+            scores[p] = best;
+        }
     }
 }
 
@@ -98,7 +104,7 @@ k_snr_reg(const float* __restrict__ folds, uint64_t nprof, WidthTable wt, float 
 template <bool IS3D, bool FINDMAX>
 __global__ void __launch_bounds__(kLanes)
 k_snr_lds(const float* __restrict__ folds, uint64_t nprof, uint32_t nbins, WidthTable wt,
-          float inv_stdnoise, float* __restrict__ scores) {
+          float inv_stdnoise, float* __restrict__ scores, FilterOut fo) {
     extern __shared__ float s_ps[];
     const uint64_t p = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (p >= nprof) {
@@ -132,36 +138,42 @@ k_snr_lds(const float* __restrict__ folds, uint64_t nprof, uint32_t nbins, Width
         const float snr = (((wt.h[iw] + wt.b[iw]) * dmax) - (wt.b[iw] * sum)) * inv_stdnoise;
         if constexpr (FINDMAX) {
             best = (best < snr) ? snr : best;
+        } else if (fo.count != nullptr) {
+            filter_emit(fo, snr > fo.threshold, static_cast<uint32_t>(p * wt.n + iw), snr);
         } else {
             scores[p * wt.n + iw] = snr;
         }
     }
     if constexpr (FINDMAX) {
-        scores[p] = best;
+        if (fo.count != nullptr) {
+            filter_emit(fo, best > fo.threshold, static_cast<uint32_t>(p), best);
+        } else {
+            scores[p] = best;
+        }
     }
 }
 
 template <int NB, bool IS3D, bool FINDMAX>
 void launch_reg(const float* folds, uint64_t nprof, const WidthTable& wt, float inv,
-                float* scores, hipStream_t s) {
+                float* scores, const FilterOut& fo, hipStream_t s) {
     const uint64_t blocks = (nprof + kLanes - 1) / kLanes;
     LOKI_REQUIRE(blocks < (1ULL << 31), "score grid too large");
     const size_t lds = static_cast<size_t>(kLanes) * psum_stride(NB, wt.wmax) * sizeof(float);
     k_snr_reg<NB, IS3D, FINDMAX><<<static_cast<unsigned>(blocks), kLanes, lds, s>>>(
-        folds, nprof, wt, inv, scores);
+        folds, nprof, wt, inv, scores, fo);
 }
 
 template <bool IS3D, bool FINDMAX>
 void launch(const float* folds, uint64_t nprof, uint32_t nbins, const WidthTable& wt, float inv,
-            float* scores, hipStream_t s) {
+            float* scores, const FilterOut& fo, hipStream_t s) {
     const bool reg_fits =
         static_cast<size_t>(kLanes) * psum_stride(nbins, wt.wmax) * sizeof(float) <= 65536;
     switch (reg_fits ? nbins : 0U) {
-    case 8: return launch_reg<8, IS3D, FINDMAX>(folds, nprof, wt, inv, scores, s);
-    case 16: return launch_reg<16, IS3D, FINDMAX>(folds, nprof, wt, inv, scores, s);
-    case 32: return launch_reg<32, IS3D, FINDMAX>(folds, nprof, wt, inv, scores, s);
-    case 64: return launch_reg<64, IS3D, FINDMAX>(folds, nprof, wt, inv, scores, s);
-    case 128: return launch_reg<128, IS3D, FINDMAX>(folds, nprof, wt, inv, scores, s);
+    case 8: return launch_reg<8, IS3D, FINDMAX>(folds, nprof, wt, inv, scores, fo, s);
+    case 16: return launch_reg<16, IS3D, FINDMAX>(folds, nprof, wt, inv, scores, fo, s);
+    case 32: return launch_reg<32, IS3D, FINDMAX>(folds, nprof, wt, inv, scores, fo, s);
+    case 64: return launch_reg<64, IS3D, FINDMAX>(folds, nprof, wt, inv, scores, fo, s);
+    case 128: return launch_reg<128, IS3D, FINDMAX>(folds, nprof, wt, inv, scores, fo, s);
     default: break;
     }
     const uint32_t stride = psum_stride(nbins, wt.wmax);
@@ -174,7 +186,7 @@ void launch(const float* folds, uint64_t nprof, uint32_t nbins, const WidthTable
     const uint64_t blocks = (nprof + lanes - 1) / lanes;
     LOKI_REQUIRE(blocks < (1ULL << 31), "score grid too large");
     k_snr_lds<IS3D, FINDMAX><<<static_cast<unsigned>(blocks), lanes, lds, s>>>(
-        folds, nprof, nbins, wt, inv, scores);
+        folds, nprof, nbins, wt, inv, scores, fo);
 }
 
 }  // namespace
@@ -199,14 +211,22 @@ WidthTable make_width_table(const SizeType* widths, SizeType nw, SizeType nbins)
 }
 
 void snr_boxcar_3d(const float* folds, uint64_t nprof, uint32_t nbins, const WidthTable& wt,
-                   float* scores, bool find_max, hipStream_t s) {
+                   float* scores, bool find_max, hipStream_t s, const FilterOut* fo) {
     if (nprof == 0) {
         return;
     }
+    const FilterOut f = fo != nullptr ? *fo : FilterOut{};
+    if (fo != nullptr) {
+        LOKI_REQUIRE(fo->count != nullptr && (fo->capacity == 0 ||
+                                              (fo->indices != nullptr && fo->snr != nullptr)),
+                     "score filter: null output");
+        LOKI_REQUIRE(nprof * (find_max ? 1U : wt.n) <= 0xFFFFFFFFULL,
+                     "score filter: trial index exceeds 32 bits");
+    }
     if (find_max) {
-        launch<true, true>(folds, nprof, nbins, wt, 1.0F, scores, s);
+        launch<true, true>(folds, nprof, nbins, wt, 1.0F, scores, f, s);
     } else {
-        launch<true, false>(folds, nprof, nbins, wt, 1.0F, scores, s);
+        launch<true, false>(folds, nprof, nbins, wt, 1.0F, scores, f, s);
     }
 }
 
@@ -215,7 +235,16 @@ void snr_boxcar_2d_max(const float* folds, uint64_t nprof, uint32_t nbins, const
     if (nprof == 0) {
         return;
     }
-    launch<false, true>(folds, nprof, nbins, wt, 1.0F / stdnoise, scores, s);
+    launch<false, true>(folds, nprof, nbins, wt, 1.0F / stdnoise, scores, FilterOut{}, s);
+}
+
+namespace {
+__global__ void k_zero_u32(uint32_t* p) { p[threadIdx.x] = 0U; }
+}  // namespace
+
+void zero_counter(uint32_t* count, hipStream_t s) {
+    LOKI_REQUIRE(count != nullptr, "counter is null");
+    k_zero_u32<<<1, 1, 0, s>>>(count);
 }
 
 }  // namespace loki_amd::kern

@@ -109,6 +109,21 @@ void refine_adjoint(const float* ts_e, const float* ts_v, uint64_t nsamps, doubl
                     double t_ref, const double* theta, uint64_t ncand, uint32_t nbins,
                     const float2* state, double* grad, void* workspace, hipStream_t s);
 
+/// Compacted output of the score-and-filter modes: (index, snr) of every
+/// (trial, width) with snr > threshold (index = trial * nwidths + width), or
+/// of every trial whose best S/N passes (index = trial).  `count` (device,
+/// zeroed before the launch) receives the number that passed, which may exceed
+/// `capacity`; only the first `capacity` slots are written, in no fixed order.
+struct FilterOut {
+    float threshold;
+    uint32_t capacity;
+    uint32_t* count;
+    uint32_t* indices;
+    float* snr;
+};
+/// Zeroes a device counter with a kernel (capturable in a graph).
+void zero_counter(uint32_t* count, hipStream_t s);
+
 /// Boxcar widths with the reference's template constants (score.cpp:107-116).
 struct WidthTable {
     static constexpr uint32_t kMax = 64;
@@ -127,9 +142,11 @@ bool final_merge_score_supported(uint32_t nbins);
 /// otherwise p = e * rsq(v) with the hardware reciprocal square root.
 /// si: storage of the parent level; unless FP32 the merged e and v are
 /// multiplied by `unscale` (exact power of two) before normalisation.
+/// With `fo`, the scores go to the compacted candidate list instead.
 void final_merge_score(const void* in, FoldStorage si, float unscale, uint64_t ncur,
                        uint64_t nprev, const ResolveLevel& lv, const WidthTable& wt,
-                       float* scores, bool find_max, bool exact, hipStream_t s);
+                       float* scores, bool find_max, bool exact, hipStream_t s,
+                       const FilterOut* fo = nullptr);
 
 // ---------------------------------------------------------------------------
 // Grouped launches: many independent FFAs (a batch of series) advanced by one
@@ -203,15 +220,18 @@ constexpr uint32_t kFinalRows = 64;
 uint32_t final_lds_stride(uint32_t nbins, uint32_t wmax);
 
 /// snr_boxcar_3d[_max]: folds [nprof, 2, nbins] -> scores [nprof, nw] or [nprof].
+/// With `fo`, the scores go to the compacted candidate list instead (`scores`
+/// unused; fo->count zeroed by the caller).
 void snr_boxcar_3d(const float* folds, uint64_t nprof, uint32_t nbins, const WidthTable& wt,
-                   float* scores, bool find_max, hipStream_t s);
+                   float* scores, bool find_max, hipStream_t s, const FilterOut* fo = nullptr);
 
 /// Fused inverse real DFT (matrix cores) + boxcar S/N of final Fourier fold
 /// rows [ntrials, 2, nbins/2+1] float2 -> scores [ntrials, nw] or [ntrials];
 /// the time-domain fold is never stored.  nbins in {16, 32, 64}.
 bool irfft_score_supported(uint32_t nbins);
 void irfft_score(const float2* fold, uint64_t ntrials, uint32_t nbins, const WidthTable& wt,
-                 float* scores, bool find_max, bool exact, hipStream_t s);
+                 float* scores, bool find_max, bool exact, hipStream_t s,
+                 const FilterOut* fo = nullptr);
 
 /// snr_boxcar_2d_max: folds [nprof, nbins] (already normalised) / stdnoise.
 void snr_boxcar_2d_max(const float* folds, uint64_t nprof, uint32_t nbins, const WidthTable& wt,

@@ -293,6 +293,17 @@ public:
                                      reinterpret_cast<float*>(scores), scores_len,
                                      find_max ? 1 : 0, reinterpret_cast<void*>(stream)));
     }
+    void scores_filter_device(uintptr_t ts_e, uintptr_t ts_v, const PyArrayT<size_t>& widths,
+                              double threshold, bool find_max, uintptr_t indices,
+                              uintptr_t scores, size_t capacity, uintptr_t count,
+                              uintptr_t stream) {
+        check(loki_ffa_scores_filter_device(
+            m_h.get(), reinterpret_cast<const float*>(ts_e), reinterpret_cast<const float*>(ts_v),
+            widths.data(), static_cast<size_t>(widths.size()), static_cast<float>(threshold),
+            find_max ? 1 : 0, reinterpret_cast<uint32_t*>(indices),
+            reinterpret_cast<float*>(scores), capacity, reinterpret_cast<uint32_t*>(count),
+            reinterpret_cast<void*>(stream)));
+    }
     void set_timing(bool on) { check(loki_ffa_set_timing(m_h.get(), on ? 1 : 0)); }
     void set_fused(bool on) { check(loki_ffa_set_fused(m_h.get(), on ? 1 : 0)); }
     void set_exact_norm(bool on) { check(loki_ffa_set_exact_norm(m_h.get(), on ? 1 : 0)); }
@@ -335,6 +346,36 @@ public:
                                    find_max ? 1 : 0));
         return out;
     }
+
+    /// Score-and-filter (score_cuda.cu:382-443): (indices, snr) sorted by index
+    /// of every (trial, width) -- or trial, with find_max -- whose S/N exceeds
+    /// the threshold.  The output is sized by a first pass; a second pass runs
+    /// only when more candidates pass than that first guess holds.
+    py::tuple scores_filter_host(const PyArrayT<float>& ts_e, const PyArrayT<float>& ts_v,
+                                 const PyArrayT<size_t>& widths, double threshold,
+                                 bool find_max) {
+        const size_t nc  = m_plan.info().fold_size_time / (2 * m_cfg.params().nbins);
+        const size_t all = find_max ? nc : nc * static_cast<size_t>(widths.size());
+        size_t cap = std::min<size_t>(all, size_t{1} << 22);
+        for (int pass = 0; pass < 2; ++pass) {
+            std::vector<uint32_t> idx(std::max<size_t>(cap, 1));
+            std::vector<float> snr(std::max<size_t>(cap, 1));
+            size_t n = 0;
+            check(loki_ffa_scores_filter_host(m_h.get(), ts_e.data(), ts_v.data(),
+                                              static_cast<size_t>(ts_e.size()), widths.data(),
+                                              static_cast<size_t>(widths.size()),
+                                              static_cast<float>(threshold), find_max ? 1 : 0,
+                                              idx.data(), snr.data(), cap, &n));
+            if (n <= cap) {
+                idx.resize(n);
+                snr.resize(n);
+                return py::make_tuple(to_array(idx), to_array(snr));
+            }
+            cap = n;
+        }
+        throw std::runtime_error("score filter: candidate count changed between passes");
+    }
+
 
     int fold_type() const { return m_fold_type; }
 
@@ -494,6 +535,8 @@ PYBIND11_MODULE(_loki_amd, m) {
         .def_property_readonly("eta", [](const Config& c) { return c.params().eta; })
         .def_property_readonly("param_limits", &Config::param_limits)
         .def_property_readonly("ducy_max", [](const Config& c) { return c.params().ducy_max; })
+        // not exposed by the reference's bindings: the threshold of the score-and-filter step
+        .def_property_readonly("snr_min", [](const Config& c) { return c.params().snr_min; })
         .def_property_readonly("wtsp", [](const Config& c) { return c.params().wtsp; })
         .def_property_readonly("prune_poly_order",
                                [](const Config& c) { return c.params().prune_poly_order; })
@@ -633,6 +676,33 @@ PYBIND11_MODULE(_loki_amd, m) {
         },
         py::arg("arr"), py::arg("widths"), py::arg("device_id") = 0);
     m_scores.def(
+        "snr_boxcar_3d_filter",
+        [](const PyArrayT<float>& arr, const PyArrayT<size_t>& widths, double threshold,
+           bool find_max, int device_id) {
+            if (arr.ndim() != 3 || widths.ndim() != 1) {
+                throw std::runtime_error(
+                    "Input array must be 3-dimensional, widths must be 1-dimensional");
+            }
+            if (arr.shape(0) == 0 || arr.shape(1) == 0 || widths.size() == 0) {
+                throw std::runtime_error("Input arrays cannot be empty");
+            }
+            const auto nprof = static_cast<size_t>(arr.shape(0));
+            const auto nbins = static_cast<size_t>(arr.shape(2));
+            const size_t cap = find_max ? nprof : nprof * static_cast<size_t>(widths.size());
+            std::vector<uint32_t> idx(std::max<size_t>(cap, 1));
+            std::vector<float> snr(std::max<size_t>(cap, 1));
+            size_t n = 0;
+            check(loki_snr_boxcar_3d_filter_host(arr.data(), nprof, nbins, widths.data(),
+                                                 static_cast<size_t>(widths.size()),
+                                                 static_cast<float>(threshold), find_max ? 1 : 0,
+                                                 idx.data(), snr.data(), cap, &n, device_id));
+            idx.resize(n);
+            snr.resize(n);
+            return py::make_tuple(to_array(idx), to_array(snr));
+        },
+        py::arg("arr"), py::arg("widths"), py::arg("threshold"), py::arg("find_max") = false,
+        py::arg("device_id") = 0);
+    m_scores.def(
         "snr_boxcar_3d_max",
         [](const PyArrayT<float>& arr, const PyArrayT<size_t>& widths, int device_id) {
             if (arr.ndim() != 3 || widths.ndim() != 1) {
@@ -742,6 +812,12 @@ PYBIND11_MODULE(_loki_amd, m) {
         .def("set_fused", &FFA::set_fused, py::arg("enable"))
         .def("set_exact_norm", &FFA::set_exact_norm, py::arg("exact"))
         .def("set_fold_storage", &FFA::set_fold_storage, py::arg("storage"))
+        .def("scores_filter", &FFA::scores_filter_host, py::arg("ts_e"), py::arg("ts_v"),
+             py::arg("widths"), py::arg("threshold"), py::arg("find_max") = false)
+        .def("scores_filter_device", &FFA::scores_filter_device, py::arg("ts_e_ptr"),
+             py::arg("ts_v_ptr"), py::arg("widths"), py::arg("threshold"), py::arg("find_max"),
+             py::arg("indices_ptr"), py::arg("scores_ptr"), py::arg("capacity"),
+             py::arg("count_ptr"), py::arg("stream") = 0)
         .def("timings", &FFA::timings);
     py::class_<FFAComplex>(m_ffa, "FFACOMPLEXCUDA")
         .def(py::init<const Config&, int>(), py::arg("cfg"), py::arg("device_id") = 0)
@@ -761,6 +837,12 @@ PYBIND11_MODULE(_loki_amd, m) {
         .def("set_timing", &FFA::set_timing, py::arg("enable"))
         .def("set_fused", &FFA::set_fused, py::arg("enable"))
         .def("set_exact_norm", &FFA::set_exact_norm, py::arg("exact"))
+        .def("scores_filter", &FFA::scores_filter_host, py::arg("ts_e"), py::arg("ts_v"),
+             py::arg("widths"), py::arg("threshold"), py::arg("find_max") = false)
+        .def("scores_filter_device", &FFA::scores_filter_device, py::arg("ts_e_ptr"),
+             py::arg("ts_v_ptr"), py::arg("widths"), py::arg("threshold"), py::arg("find_max"),
+             py::arg("indices_ptr"), py::arg("scores_ptr"), py::arg("capacity"),
+             py::arg("count_ptr"), py::arg("stream") = 0)
         .def("timings", &FFA::timings);
 
     m_ffa.def(
@@ -840,6 +922,24 @@ PYBIND11_MODULE(_loki_amd, m) {
         py::arg("ts_e"), py::arg("ts_v"), py::arg("cfg"), py::arg("device_id") = 0,
         py::arg("quiet") = false, py::arg("widths") = std::nullopt,
         py::arg("fold_storage") = "f32");
+
+    m_ffa.def(
+        "compute_ffa_scores_filter_cuda",
+        [](const PyArrayT<float>& ts_e, const PyArrayT<float>& ts_v, const Config& cfg,
+           std::optional<double> threshold, std::optional<PyArrayT<size_t>> widths,
+           bool find_max, int device_id) {
+            // the FFAFreqSweepCUDA region step (ffa_freq_sweep_cuda.cu:219-260): FFA, then
+            // score_and_filter_cuda_d at cfg.snr_min
+            check_nsamps(ts_e, ts_v, cfg);
+            FFA ffa(cfg, device_id,
+                    cfg.params().use_fourier != 0 ? LOKI_FOLD_FOURIER : LOKI_FOLD_TIME);
+            PyArrayT<size_t> w = widths.has_value() ? *widths : PyArrayT<size_t>(cfg.score_widths());
+            auto res = ffa.scores_filter_host(ts_e, ts_v, w,
+                                              threshold.value_or(cfg.params().snr_min), find_max);
+            return py::make_tuple(res[0], res[1], PlanTime(cfg));
+        },
+        py::arg("ts_e"), py::arg("ts_v"), py::arg("cfg"), py::arg("threshold") = std::nullopt,
+        py::arg("widths") = std::nullopt, py::arg("find_max") = false, py::arg("device_id") = 0);
 
     m.def("device_count", [] {
         int n = 0;

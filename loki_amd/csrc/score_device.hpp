@@ -5,6 +5,8 @@
 
 #include <cfloat>
 
+#include "kernels.hpp"
+
 namespace loki_amd::kern {
 
 /// max_j (hi[j] - ps[j]) over j < NB.  fmaxf pairs lower to v_max3_f32.  For
@@ -39,6 +41,34 @@ __device__ __forceinline__ float normalise(float e, float v) {
         return e * (1.0F / sqrtf(v));
     } else {
         return e * __builtin_amdgcn_rsqf(v);
+    }
+}
+
+/// Appends (index, snr) of every active lane whose `pass` is set to a
+/// compacted candidate list (score_cuda.cu:130-168 kPerWidthAndFilter /
+/// kMaxAndFilter): one atomic add per wave reserves the slots, lanes fill them
+/// in lane order.  Slots at or beyond fo.capacity are counted but not written.
+/// Must be reached by all active lanes of the wave together.
+__device__ __forceinline__ void filter_emit(const FilterOut& fo, bool pass, uint32_t index,
+                                            float snr) {
+    const uint64_t mask = __ballot(pass ? 1 : 0);
+    if (mask == 0) {
+        return;
+    }
+    const uint32_t lane   = __lane_id();
+    const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(mask))) - 1U;
+    uint32_t base         = 0;
+    if (lane == leader) {
+        base = atomicAdd(fo.count, static_cast<uint32_t>(__popcll(mask)));
+    }
+    base = __shfl(base, static_cast<int>(leader));
+    if (pass) {
+        const uint64_t below = lane == 0 ? 0ULL : (mask & ((1ULL << lane) - 1ULL));
+        const uint32_t slot  = base + static_cast<uint32_t>(__popcll(below));
+        if (slot < fo.capacity) {
+            fo.indices[slot] = index;
+            fo.snr[slot]     = snr;
+        }
     }
 }
 

@@ -17,6 +17,8 @@ scores = SimpleNamespace(
     snr_boxcar_3d=_n.scores.snr_boxcar_3d,
     snr_boxcar_3d_max=_n.scores.snr_boxcar_3d_max,
     snr_boxcar_2d_max=_n.scores.snr_boxcar_2d_max,
+    # score_and_filter[_max]_cuda_d (score_cuda.cu:382-443): (indices, snr) above a threshold
+    snr_boxcar_3d_filter=_n.scores.snr_boxcar_3d_filter,
 )
 
 __all__ = ["ffa", "fold", "scores"]
