@@ -284,6 +284,80 @@ int loki_batch_set_schedule(loki_batch_t batch, const loki_batch_schedule* sched
 int loki_batch_autotune(loki_batch_t batch, void* stream, int repeats,
                         loki_batch_schedule* chosen);
 
+/* --- Frequency sweep: region planner and driver -------------------------- */
+/* regions.hpp:40-46 generate_ffa_regions (coord.hpp FFARegion): octave bands of the period
+ * range [p_min, p_max] with nbins growing so that one bin keeps its width (p_min / nbins_min)
+ * and eta growing with nbins; nbins capped at nbins_max (0: no cap).  out may be NULL to query
+ * *n. */
+typedef struct {
+    double f_start;
+    double f_end;
+    size_t nbins;
+    double eta;
+} loki_ffa_region;
+int loki_generate_ffa_regions(double p_min, double p_max, double tsamp, size_t nbins_min,
+                              double eta_min, double octave_scale, size_t nbins_max,
+                              loki_ffa_region* out, size_t cap, size_t* n);
+/* regions.hpp FFARegionStats: maxima over the planned chunks and the memory they imply (GiB). */
+typedef struct {
+    size_t max_buffer_size;
+    size_t max_coord_size;
+    size_t max_ncoords;
+    size_t max_ffa_levels;
+    size_t max_buffer_size_time;
+    size_t max_scores_size;
+    size_t write_param_sets_size;
+    float buffer_memory_usage;
+    float coord_memory_usage;
+    float extra_memory_usage;
+    float freq_sweep_memory_usage;
+} loki_region_stats;
+int loki_region_stats_compute(size_t max_buffer_size, size_t max_coord_size, size_t max_ncoords,
+                              size_t max_ffa_levels, size_t n_widths, size_t n_params,
+                              size_t n_samps, size_t max_passing_candidates, int use_fourier,
+                              int use_gpu, loki_region_stats* out);
+/* coord.hpp FFAChunkStats. */
+typedef struct {
+    double nominal_f_start;
+    double nominal_f_end;
+    double actual_f_start;
+    double actual_f_end;
+    double nominal_width;
+    double actual_width;
+    double total_memory_gb;
+    double overlap_fraction;
+} loki_ffa_chunk_stats;
+/* regions.cpp:163-420 FFARegionPlanner<T>(cfg): each band split into frequency chunks whose FFA
+ * (with the drift of accel/jerk widening every chunk) fits cfg.max_process_memory_gb. */
+typedef struct loki_region_planner_s* loki_region_planner_t;
+int loki_region_planner_create(loki_config_t cfg, int fold_type, int use_gpu,
+                               loki_region_planner_t* out);
+void loki_region_planner_destroy(loki_region_planner_t planner);
+int loki_region_planner_nregions(loki_region_planner_t planner, size_t* n);
+/* A new config handle of chunk i (destroy with loki_config_destroy). */
+int loki_region_planner_config(loki_region_planner_t planner, size_t i, loki_config_t* out);
+int loki_region_planner_stats(loki_region_planner_t planner, loki_region_stats* out);
+int loki_region_planner_chunk(loki_region_planner_t planner, size_t i,
+                              loki_ffa_chunk_stats* out);
+/* FFAFreqSweepCUDA (ffa_freq_sweep_cuda.cu:21-342): every planned chunk as FFA + score-and-filter
+ * at cfg.snr_min on device `device_id`; the candidates (S/N and rows of the final-grid parameter
+ * values plus the boxcar width, nparams + 1 doubles each) are kept for loki_freq_sweep_results,
+ * chunk by chunk, in index order within a chunk.  The file output of the reference
+ * (FFAResultWriter, HDF5) is left to the caller. */
+typedef struct loki_freq_sweep_s* loki_freq_sweep_t;
+int loki_freq_sweep_create(loki_config_t cfg, int device_id, loki_freq_sweep_t* out);
+void loki_freq_sweep_destroy(loki_freq_sweep_t sweep);
+int loki_freq_sweep_execute(loki_freq_sweep_t sweep, const float* ts_e, const float* ts_v,
+                            size_t nsamps, size_t* ncandidates);
+/* Candidates of the last execute (0 before the first). */
+int loki_freq_sweep_count(loki_freq_sweep_t sweep, size_t* n);
+int loki_freq_sweep_results(loki_freq_sweep_t sweep, float* snr, double* param_sets, size_t cap);
+/* Candidates of each chunk in the last execute (ffa_freq_sweep_cuda.cu:128
+ * m_n_passing_scores_per_region); `cap` >= the planner's nregions. */
+int loki_freq_sweep_per_region(loki_freq_sweep_t sweep, size_t* counts, size_t cap);
+/* A new planner handle with the sweep's chunks (destroy with loki_region_planner_destroy). */
+int loki_freq_sweep_planner(loki_freq_sweep_t sweep, loki_region_planner_t* out);
+
 /* --- Brute fold (fold.hpp:78-133 BruteFoldCUDA) ------------------------ */
 /* compute_brute_fold_cuda<float> (fold_cuda.cu:260-277): fold has
  * (nsamps/segment_len) * nfreqs * 2 * nbins floats. */

@@ -26,8 +26,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_SOURCES = ["k_resolve.hip", "k_brutefold.hip", "k_merge.hip", "k_score.hip", "k_final.hip",
                "k_fourier.hip", "k_fourier_score.hip", "k_refine.hip",
-               "ffa.hip", "batch.hip", "capi.hip"]
-HOST_SOURCES = ["config.cpp", "plan.cpp"]
+               "ffa.hip", "batch.hip", "sweep.hip", "capi.hip"]
+HOST_SOURCES = ["config.cpp", "plan.cpp", "regions.cpp"]
 
 # -ffp-contract=off: no FMA contraction anywhere (bit-exact FP64 shift tables
 # and FP32 score arithmetic).  Packed FP32 instructions are disabled.

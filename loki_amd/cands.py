@@ -25,17 +25,26 @@ def param_sets(plan, indices, widths) -> np.ndarray:
     widths = np.asarray(widths)
     nw = widths.size
     idx = np.asarray(indices, dtype=np.int64)
-    grids = list(plan.params_dict.values())
-    counts = [len(g) for g in grids]
+    counts = [int(c) for c in plan.param_counts[-1]]
+    limits = np.asarray(plan.config.param_limits, dtype=np.float64)
     coord = idx // nw
     if coord.size and coord.max() >= int(np.prod(counts)):
         raise ValueError("score index beyond the final grid")
     cols = np.unravel_index(coord, counts)
-    out = np.empty((idx.size, len(grids) + 1), dtype=np.float64)
-    for j, (g, c) in enumerate(zip(grids, cols)):
-        out[:, j] = np.asarray(g, dtype=np.float64)[c]
+    out = np.empty((idx.size, len(counts) + 1), dtype=np.float64)
+    for j, (c, (lo, hi), col) in enumerate(zip(counts, limits, cols)):
+        out[:, j] = _cell_centre(float(lo), float(hi), c, col)
     out[:, -1] = widths[idx % nw].astype(np.float64)
     return out
+
+
+def _cell_centre(vmin: float, vmax: float, count: int, i) -> np.ndarray:
+    """psr_utils.hpp:283-293 get_param_val_at_idx: vmin + (vmax - vmin) / count * (i + 0.5),
+    the midpoint of the range for a single cell."""
+    i = np.asarray(i, dtype=np.float64)
+    if count <= 1:
+        return np.full(i.shape, (vmax + vmin) / 2.0)
+    return vmin + ((vmax - vmin) / float(count)) * (i + 0.5)
 
 
 def write_results(path, param_names, scoring_widths, snr, param_sets_arr) -> Path:

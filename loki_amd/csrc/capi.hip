@@ -14,6 +14,8 @@
 #include "batch.hpp"
 #include "ffa.hpp"
 #include "kernels.hpp"
+#include "regions.hpp"
+#include "sweep.hpp"
 
 struct loki_config_s {
     loki_amd::SearchConfig cfg;
@@ -24,6 +26,12 @@ struct loki_plan_s {
 struct loki_ffa_s {
     loki_amd::FFAExecutor exec;
     loki_plan_s plan_view;
+};
+struct loki_region_planner_s {
+    loki_amd::FFARegionPlanner planner;
+};
+struct loki_freq_sweep_s {
+    loki_amd::FFAFreqSweep sweep;
 };
 struct loki_batch_s {
     std::unique_ptr<loki_amd::FFABatch> batch;
@@ -711,6 +719,148 @@ int loki_batch_autotune(loki_batch_t batch, void* stream, int repeats,
         if (chosen != nullptr) {
             *chosen = {sc.merge_group, sc.merge_run, sc.final_group, sc.final_run};
         }
+    });
+}
+
+int loki_generate_ffa_regions(double p_min, double p_max, double tsamp, size_t nbins_min,
+                              double eta_min, double octave_scale, size_t nbins_max,
+                              loki_ffa_region* out, size_t cap, size_t* n) {
+    return guarded([&] {
+        REQ(n != nullptr, "null argument");
+        const auto r = loki_amd::generate_ffa_regions(p_min, p_max, tsamp, nbins_min, eta_min,
+                                                      octave_scale, nbins_max);
+        *n = r.size();
+        REQ(out == nullptr || cap >= r.size(), "region output too small");
+        for (size_t i = 0; out != nullptr && i < r.size(); ++i) {
+            out[i] = {r[i].f_start, r[i].f_end, r[i].nbins, r[i].eta};
+        }
+    });
+}
+
+namespace {
+void fill_stats(const loki_amd::FFARegionStats& st, loki_region_stats* out) {
+    out->max_buffer_size         = st.max_buffer_size();
+    out->max_coord_size          = st.max_coord_size();
+    out->max_ncoords             = st.max_ncoords();
+    out->max_ffa_levels          = st.max_ffa_levels();
+    out->max_buffer_size_time    = st.max_buffer_size_time();
+    out->max_scores_size         = st.max_scores_size();
+    out->write_param_sets_size   = st.write_param_sets_size();
+    out->buffer_memory_usage     = st.buffer_memory_usage();
+    out->coord_memory_usage      = st.coord_memory_usage();
+    out->extra_memory_usage      = st.extra_memory_usage();
+    out->freq_sweep_memory_usage = st.freq_sweep_memory_usage();
+}
+}  // namespace
+
+int loki_region_stats_compute(size_t max_buffer_size, size_t max_coord_size, size_t max_ncoords,
+                              size_t max_ffa_levels, size_t n_widths, size_t n_params,
+                              size_t n_samps, size_t max_passing_candidates, int use_fourier,
+                              int use_gpu, loki_region_stats* out) {
+    return guarded([&] {
+        REQ(out != nullptr, "null argument");
+        fill_stats(loki_amd::FFARegionStats(max_buffer_size, max_coord_size, max_ncoords,
+                                            max_ffa_levels, n_widths, n_params, n_samps,
+                                            max_passing_candidates, use_fourier != 0,
+                                            use_gpu != 0),
+                   out);
+    });
+}
+
+int loki_region_planner_create(loki_config_t cfg, int fold_type, int use_gpu,
+                               loki_region_planner_t* out) {
+    return guarded([&] {
+        REQ(cfg != nullptr && out != nullptr, "null argument");
+        *out = new loki_region_planner_s{
+            loki_amd::FFARegionPlanner(cfg->cfg, fold_kind(fold_type), use_gpu != 0)};
+    });
+}
+
+void loki_region_planner_destroy(loki_region_planner_t p) { delete p; }
+
+int loki_region_planner_nregions(loki_region_planner_t p, size_t* n) {
+    return guarded([&] {
+        REQ(p != nullptr && n != nullptr, "null argument");
+        *n = p->planner.nregions();
+    });
+}
+
+int loki_region_planner_config(loki_region_planner_t p, size_t i, loki_config_t* out) {
+    return guarded([&] {
+        REQ(p != nullptr && out != nullptr, "null argument");
+        REQ(i < p->planner.nregions(), "region index out of range");
+        *out = new loki_config_s{p->planner.cfgs()[i]};
+    });
+}
+
+int loki_region_planner_stats(loki_region_planner_t p, loki_region_stats* out) {
+    return guarded([&] {
+        REQ(p != nullptr && out != nullptr, "null argument");
+        fill_stats(p->planner.stats(), out);
+    });
+}
+
+int loki_region_planner_chunk(loki_region_planner_t p, size_t i, loki_ffa_chunk_stats* out) {
+    return guarded([&] {
+        REQ(p != nullptr && out != nullptr, "null argument");
+        REQ(i < p->planner.chunk_stats().size(), "chunk index out of range");
+        const auto& c = p->planner.chunk_stats()[i];
+        *out = {c.nominal_f_start, c.nominal_f_end, c.actual_f_start, c.actual_f_end,
+                c.nominal_width,   c.actual_width,  c.total_memory_gb, c.overlap_fraction};
+    });
+}
+
+int loki_freq_sweep_create(loki_config_t cfg, int device_id, loki_freq_sweep_t* out) {
+    return guarded([&] {
+        REQ(cfg != nullptr && out != nullptr, "null argument");
+        *out = new loki_freq_sweep_s{loki_amd::FFAFreqSweep(cfg->cfg, device_id)};
+    });
+}
+
+void loki_freq_sweep_destroy(loki_freq_sweep_t s) { delete s; }
+
+int loki_freq_sweep_execute(loki_freq_sweep_t s, const float* ts_e, const float* ts_v,
+                            size_t nsamps, size_t* ncandidates) {
+    return guarded([&] {
+        REQ(s != nullptr && ncandidates != nullptr, "null argument");
+        *ncandidates = s->sweep.execute(ts_e, ts_v, nsamps);
+    });
+}
+
+int loki_freq_sweep_count(loki_freq_sweep_t s, size_t* n) {
+    return guarded([&] {
+        REQ(s != nullptr && n != nullptr, "null argument");
+        *n = s->sweep.snr().size();
+    });
+}
+
+int loki_freq_sweep_per_region(loki_freq_sweep_t s, size_t* counts, size_t cap) {
+    return guarded([&] {
+        REQ(s != nullptr && counts != nullptr, "null argument");
+        const auto& v = s->sweep.per_region();
+        REQ(cap >= s->sweep.planner().nregions(), "per-region output too small");
+        for (size_t i = 0; i < s->sweep.planner().nregions(); ++i) {
+            counts[i] = i < v.size() ? v[i] : 0;
+        }
+    });
+}
+
+int loki_freq_sweep_results(loki_freq_sweep_t s, float* snr, double* param_sets, size_t cap) {
+    return guarded([&] {
+        REQ(s != nullptr, "null argument");
+        const auto& v = s->sweep.snr();
+        const auto& r = s->sweep.rows();
+        REQ(cap >= v.size(), "candidate output too small");
+        REQ(v.empty() || (snr != nullptr && param_sets != nullptr), "null output buffer");
+        std::copy(v.begin(), v.end(), snr);
+        std::copy(r.begin(), r.end(), param_sets);
+    });
+}
+
+int loki_freq_sweep_planner(loki_freq_sweep_t s, loki_region_planner_t* out) {
+    return guarded([&] {
+        REQ(s != nullptr && out != nullptr, "null argument");
+        *out = new loki_region_planner_s{s->sweep.planner()};
     });
 }
 

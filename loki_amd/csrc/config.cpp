@@ -173,4 +173,13 @@ std::vector<double> SearchConfig::get_dparams_actual(double tseg_cur) const {
     return out;
 }
 
+SearchConfig SearchConfig::updated(SizeType nbins, double eta, double f_min,
+                                   double f_max) const {
+    SearchConfigArgs a  = m_args;
+    a.nbins             = nbins;
+    a.eta               = eta;
+    a.param_limits.back() = ParamLimit{f_min, f_max};
+    return SearchConfig(std::move(a));
+}
+
 }  // namespace loki_amd

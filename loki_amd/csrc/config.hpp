@@ -91,6 +91,18 @@ public:
     std::vector<double> get_dparams_actual(double tseg_cur) const;
     std::vector<SizeType> get_param_grid_count(double tseg_cur) const;
 
+    double get_max_process_memory_gb() const noexcept { return m_args.max_process_memory_gb; }
+    double get_octave_scale() const noexcept { return m_args.octave_scale; }
+    SizeType get_nbins_max() const noexcept { return m_args.nbins_max; }
+    double get_snr_min() const noexcept { return m_args.snr_min; }
+    SizeType get_max_passing_candidates() const noexcept {
+        return m_args.max_passing_candidates;
+    }
+    /// configs.cpp:187-223 get_updated_config: the same search with another
+    /// nbins, eta and frequency range (explicit bseg values carried over, the
+    /// defaults re-derived).
+    SearchConfig updated(SizeType nbins, double eta, double f_min, double f_max) const;
+
 private:
     SearchConfigArgs m_args;
     SizeType m_bseg_brute{};

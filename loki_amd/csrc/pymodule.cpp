@@ -1,10 +1,12 @@
 // Python bindings over the C-ABI (include/loki_amd.h), mirroring the names,
 // argument names/defaults and return types of loki's pybind modules:
 //   libloki:   configs.PulsarSearchConfig, plans.FFAPlan{Base,Time,Fourier},
+//              plans.generate_ffa_regions / FFARegionStats / FFARegionPlanner{Time,Fourier},
 //              scores.generate_box_width_trials   (src/loki_python.cpp:71-520)
 //   libculoki: ffa.FFACUDA / FFACOMPLEXCUDA / compute_ffa_time_cuda /
 //              compute_ffa_fourier_cuda / compute_ffa_fourier_return_to_time_cuda /
-//              compute_ffa_scores_cuda, fold.compute_brute_fold_{time,fourier}_cuda,
+//              compute_ffa_scores_cuda, FFAFreqSweepCUDA,
+//              fold.compute_brute_fold_{time,fourier}_cuda,
 //              scores.snr_boxcar_*
 //              (src/loki_python_cuda.cpp:18-203, loki_templates_cuda.cuh:21-51)
 // C-ABI failures raise RuntimeError with the library's message, the same
@@ -15,6 +17,7 @@
 
 #include <complex>
 #include <cstdint>
+#include <cstring>
 #include <map>
 #include <memory>
 #include <optional>
@@ -84,6 +87,12 @@ public:
         p.use_conservative_tile  = use_conservative_tile ? 1 : 0;
         loki_config_t h          = nullptr;
         check(loki_config_create(&p, &h));
+        m_h.reset(h, loki_config_destroy);
+        check(loki_config_info_get(h, &m_info));
+        check(loki_config_params_get(h, &m_p));
+    }
+    /// Takes ownership of a handle made by the library (e.g. a planned chunk).
+    explicit Config(loki_config_t h) {
         m_h.reset(h, loki_config_destroy);
         check(loki_config_info_get(h, &m_info));
         check(loki_config_params_get(h, &m_p));
@@ -507,6 +516,122 @@ void check_nsamps(const PyArrayT<float>& ts_e, const PyArrayT<float>& ts_v, cons
 
 }  // namespace
 
+namespace {
+
+// regions.hpp FFARegionStats (bound at loki_python.cpp:482-510).
+struct RegionStats {
+    loki_region_stats s{};
+};
+
+// regions.cpp:163-420 FFARegionPlanner<T> (bound at loki_templates.hpp:76-84).
+class RegionPlanner {
+public:
+    RegionPlanner(const Config& cfg, int fold_type) {
+        loki_region_planner_t h = nullptr;
+        check(loki_region_planner_create(cfg.handle(), fold_type, 1, &h));
+        adopt(h);
+    }
+    explicit RegionPlanner(loki_region_planner_t h) { adopt(h); }
+    std::vector<Config> cfgs() const {
+        std::vector<Config> out;
+        for (size_t i = 0; i < m_n; ++i) {
+            loki_config_t c = nullptr;
+            check(loki_region_planner_config(m_h.get(), i, &c));
+            out.emplace_back(c);
+        }
+        return out;
+    }
+    size_t nregions() const { return m_n; }
+    RegionStats stats() const {
+        RegionStats st;
+        check(loki_region_planner_stats(m_h.get(), &st.s));
+        return st;
+    }
+    py::list chunk_stats() const {
+        py::list out;
+        for (size_t i = 0; i < m_n; ++i) {
+            loki_ffa_chunk_stats c{};
+            check(loki_region_planner_chunk(m_h.get(), i, &c));
+            py::dict d;
+            d["nominal_f_start"]  = c.nominal_f_start;
+            d["nominal_f_end"]    = c.nominal_f_end;
+            d["actual_f_start"]   = c.actual_f_start;
+            d["actual_f_end"]     = c.actual_f_end;
+            d["nominal_width"]    = c.nominal_width;
+            d["actual_width"]     = c.actual_width;
+            d["total_memory_gb"]  = c.total_memory_gb;
+            d["overlap_fraction"] = c.overlap_fraction;
+            out.append(d);
+        }
+        return out;
+    }
+
+private:
+    std::shared_ptr<loki_region_planner_s> m_h;
+    size_t m_n = 0;
+    void adopt(loki_region_planner_t h) {
+        m_h.reset(h, loki_region_planner_destroy);
+        check(loki_region_planner_nregions(h, &m_n));
+    }
+};
+
+// FFAFreqSweepCUDA (ffa_freq_sweep_cuda.cu:21-365, bound at loki_python_cuda.cpp:205-217).
+class FreqSweep {
+public:
+    FreqSweep(const Config& cfg, int device_id) : m_cfg(cfg) {
+        loki_freq_sweep_t h = nullptr;
+        check(loki_freq_sweep_create(cfg.handle(), device_id, &h));
+        m_h.reset(h, loki_freq_sweep_destroy);
+    }
+    /// Runs every chunk and writes {outdir}/{file_prefix}_ffa_results.npz (the reference writes
+    /// the same datasets to {file_prefix}_ffa_results.h5); returns the archive's path.
+    py::object execute(const PyArrayT<float>& ts_e, const PyArrayT<float>& ts_v,
+                       const std::string& outdir, const std::string& file_prefix) {
+        if (ts_e.size() != ts_v.size()) {
+            throw std::invalid_argument("ts_e and ts_v must have the same size");
+        }
+        size_t n = 0;
+        {
+            py::gil_scoped_release nogil;
+            check(loki_freq_sweep_execute(m_h.get(), ts_e.data(), ts_v.data(),
+                                          static_cast<size_t>(ts_e.size()), &n));
+        }
+        auto [snr, ps] = results();
+        auto names     = m_cfg.param_names();
+        names.emplace_back("width");
+        auto path = py::module_::import("pathlib").attr("Path")(outdir) /
+                    py::str(file_prefix + "_ffa_results.npz");
+        py::module_::import("loki_amd.cands")
+            .attr("write_results")(path, names, m_cfg.score_widths(), snr, ps);
+        return path;
+    }
+    std::pair<py::array_t<float>, py::array_t<double>> results() const {
+        size_t n = 0;
+        check(loki_freq_sweep_count(m_h.get(), &n));
+        const auto np = static_cast<py::ssize_t>(m_cfg.params().nparams + 1);
+        py::array_t<float> snr(static_cast<py::ssize_t>(n));
+        py::array_t<double> ps({static_cast<py::ssize_t>(n), np});
+        check(loki_freq_sweep_results(m_h.get(), snr.mutable_data(), ps.mutable_data(), n));
+        return {snr, ps};
+    }
+    std::vector<size_t> per_region() const {
+        std::vector<size_t> v(planner().nregions());
+        check(loki_freq_sweep_per_region(m_h.get(), v.data(), v.size()));
+        return v;
+    }
+    RegionPlanner planner() const {
+        loki_region_planner_t p = nullptr;
+        check(loki_freq_sweep_planner(m_h.get(), &p));
+        return RegionPlanner(p);
+    }
+
+private:
+    Config m_cfg;
+    std::shared_ptr<loki_freq_sweep_s> m_h;
+};
+
+}  // namespace
+
 PYBIND11_MODULE(_loki_amd, m) {
     m.doc() = "MI355X (gfx950) FFA hot path: loki-compatible bindings over the loki_amd C-ABI";
 
@@ -641,6 +766,80 @@ PYBIND11_MODULE(_loki_amd, m) {
             return t ? p.info().gflops_return_in_time : p.info().gflops;
         }, py::arg("return_in_time") = false);
     py::class_<PlanTime, Plan>(m_plans, "FFAPlanTime").def(py::init<const Config&>(), py::arg("cfg"));
+    py::class_<RegionStats>(m_plans, "FFARegionStats")
+        .def(py::init([](size_t max_buffer_size, size_t max_coord_size, size_t max_ncoords,
+                         size_t max_ffa_levels, size_t n_widths, size_t n_params,
+                         size_t n_samps, size_t max_passing_candidates, bool use_fourier,
+                         bool use_gpu) {
+                 RegionStats st;
+                 check(loki_region_stats_compute(max_buffer_size, max_coord_size, max_ncoords,
+                                                 max_ffa_levels, n_widths, n_params, n_samps,
+                                                 max_passing_candidates, use_fourier ? 1 : 0,
+                                                 use_gpu ? 1 : 0, &st.s));
+                 return st;
+             }),
+             py::arg("max_buffer_size"), py::arg("max_coord_size"), py::arg("max_ncoords"),
+             py::arg("max_ffa_levels"), py::arg("n_widths"), py::arg("n_params"),
+             py::arg("n_samps"), py::arg("max_passing_candidates"), py::arg("use_fourier"),
+             py::arg("use_gpu") = false)
+        .def_property_readonly("max_buffer_size",
+                               [](const RegionStats& r) { return r.s.max_buffer_size; })
+        .def_property_readonly("max_coord_size",
+                               [](const RegionStats& r) { return r.s.max_coord_size; })
+        .def_property_readonly("max_ncoords", [](const RegionStats& r) { return r.s.max_ncoords; })
+        .def_property_readonly("max_ffa_levels",
+                               [](const RegionStats& r) { return r.s.max_ffa_levels; })
+        .def_property_readonly("max_buffer_size_time",
+                               [](const RegionStats& r) { return r.s.max_buffer_size_time; })
+        .def_property_readonly("max_scores_size",
+                               [](const RegionStats& r) { return r.s.max_scores_size; })
+        .def_property_readonly("write_param_sets_size",
+                               [](const RegionStats& r) { return r.s.write_param_sets_size; })
+        .def_property_readonly("buffer_memory_usage",
+                               [](const RegionStats& r) { return r.s.buffer_memory_usage; })
+        .def_property_readonly("coord_memory_usage",
+                               [](const RegionStats& r) { return r.s.coord_memory_usage; })
+        .def_property_readonly("extra_memory_usage",
+                               [](const RegionStats& r) { return r.s.extra_memory_usage; })
+        .def_property_readonly("freq_sweep_memory_usage",
+                               [](const RegionStats& r) { return r.s.freq_sweep_memory_usage; });
+    py::class_<RegionPlanner>(m_plans, "FFARegionPlanner")
+        .def(py::init<const Config&, int>(), py::arg("cfg"), py::arg("fold_type") = 0)
+        .def_property_readonly("cfgs", &RegionPlanner::cfgs)
+        .def_property_readonly("nregions", &RegionPlanner::nregions)
+        .def_property_readonly("stats", &RegionPlanner::stats)
+        .def_property_readonly("chunk_stats", &RegionPlanner::chunk_stats);
+    m_plans.def(
+        "FFARegionPlannerTime",
+        [](const Config& cfg) { return RegionPlanner(cfg, LOKI_FOLD_TIME); }, py::arg("cfg"));
+    m_plans.def(
+        "FFARegionPlannerFourier",
+        [](const Config& cfg) { return RegionPlanner(cfg, LOKI_FOLD_FOURIER); }, py::arg("cfg"));
+    m_plans.def(
+        "generate_ffa_regions",
+        [](double p_min, double p_max, double tsamp, size_t nbins_min, double eta_min,
+           double growth_factor, std::optional<size_t> nbins_max) {
+            size_t n = 0;
+            const size_t cap_bins = nbins_max.value_or(0);
+            check(loki_generate_ffa_regions(p_min, p_max, tsamp, nbins_min, eta_min,
+                                            growth_factor, cap_bins, nullptr, 0, &n));
+            std::vector<loki_ffa_region> r(n);
+            check(loki_generate_ffa_regions(p_min, p_max, tsamp, nbins_min, eta_min,
+                                            growth_factor, cap_bins, r.data(), n, &n));
+            // a structured array of coord.hpp FFARegion (registered at loki_python.cpp:380)
+            py::list fields;
+            fields.append(py::make_tuple("f_start", "<f8"));
+            fields.append(py::make_tuple("f_end", "<f8"));
+            fields.append(py::make_tuple("nbins", "<u8"));
+            fields.append(py::make_tuple("eta", "<f8"));
+            py::dtype dt = py::module_::import("numpy").attr("dtype")(fields).cast<py::dtype>();
+            py::array out(dt, std::vector<py::ssize_t>{static_cast<py::ssize_t>(n)});
+            static_assert(sizeof(loki_ffa_region) == 32, "packed region record");
+            std::memcpy(out.mutable_data(), r.data(), n * sizeof(loki_ffa_region));
+            return out;
+        },
+        py::arg("p_min"), py::arg("p_max"), py::arg("tsamp"), py::arg("nbins_min"),
+        py::arg("eta_min"), py::arg("growth_factor") = 2.0, py::arg("nbins_max") = py::none());
     py::class_<PlanFourier, Plan>(m_plans, "FFAPlanFourier")
         .def(py::init<const Config&>(), py::arg("cfg"));
 
@@ -860,6 +1059,13 @@ PYBIND11_MODULE(_loki_amd, m) {
         },
         py::arg("ts_e"), py::arg("ts_v"), py::arg("cfg"), py::arg("device_id") = 0,
         py::arg("quiet") = false, py::arg("fold_storage") = "f32");
+    py::class_<FreqSweep>(m_ffa, "FFAFreqSweepCUDA")
+        .def(py::init<const Config&, int>(), py::arg("cfg"), py::arg("device_id") = 0)
+        .def("execute", &FreqSweep::execute, py::arg("ts_e"), py::arg("ts_v"),
+             py::arg("outdir"), py::arg("file_prefix") = "test")
+        .def("results", &FreqSweep::results)
+        .def_property_readonly("per_region", &FreqSweep::per_region)
+        .def_property_readonly("planner", &FreqSweep::planner);
     py::class_<Batch>(m_ffa, "FFABatchCUDA")
         .def(py::init<const std::vector<Config>&, std::optional<PyArrayT<size_t>>, int>(),
              py::arg("cfgs"), py::arg("widths") = std::nullopt, py::arg("device_id") = 0)

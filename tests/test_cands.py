@@ -20,7 +20,7 @@ def test_param_sets_follow_final_grid_cell_centres():
 
     cfg, plan = _plan()
     widths = np.array([1, 2, 4, 8], np.uint64)
-    na, nf = (len(v) for v in plan.params_dict.values())
+    na, nf = (int(c) for c in plan.param_counts[-1])
     # trial (ia, jf) with width index iw -> index (ia * nf + jf) * nw + iw
     picks = [(0, 0, 0), (3, 17, 2), (na - 1, nf - 1, 3)]
     idx = np.array([(ia * nf + jf) * 4 + iw for ia, jf, iw in picks], np.uint32)
@@ -28,10 +28,13 @@ def test_param_sets_follow_final_grid_cell_centres():
     lo_a, hi_a = -60000.0, 60000.0
     lo_f, hi_f = 100.0, 110.0
     for (ia, jf, iw), row in zip(picks, rows):
-        # psr_utils.hpp:283-293 cell centres
-        assert row[0] == pytest.approx(lo_a + (hi_a - lo_a) / na * (ia + 0.5), rel=1e-12)
-        assert row[1] == pytest.approx(lo_f + (hi_f - lo_f) / nf * (jf + 0.5), rel=1e-12)
+        # psr_utils.hpp:283-293 cell centres, in the reference's operation order
+        assert row[0] == lo_a + (hi_a - lo_a) / na * (ia + 0.5)
+        assert row[1] == lo_f + (hi_f - lo_f) / nf * (jf + 0.5)
         assert row[2] == widths[iw]
+        # and within half a cell of the plan's grid value
+        ga, gf = plan.params_dict.values()
+        assert abs(row[0] - ga[ia]) < 1e-9 * (hi_a - lo_a) and abs(row[1] - gf[jf]) < 1e-9
     with pytest.raises(ValueError):
         cands.param_sets(plan, np.array([na * nf * 4], np.uint32), widths)
 
