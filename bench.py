@@ -106,6 +106,8 @@ def parse() -> argparse.Namespace:
                     help="skip the score-and-filter timing")
     ap.add_argument("--filter-threshold", type=float, default=6.0,
                     help="S/N threshold of the score-and-filter timing")
+    ap.add_argument("--no-seeds", action="store_true",
+                    help="skip the pruning-seed kernel timing (time-domain workloads)")
     ap.add_argument("--no-refine", action="store_true",
                     help="Fourier workload: skip the gradient-ascent refinement leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -468,6 +470,89 @@ def run_batch(args, wl) -> int:
     return 0
 
 
+def run_seeds(dev, stream, steps: int) -> dict:
+    """Pruning seeds (kernels_cuda.cu:16-277) over synthetic final-level rows in HBM: 2^21
+    leaves, nbins 64, a final fold of 8 segments x 2^16 coordinates.  Per launch, the linear
+    seed moves 24 nbins + 13 bytes per leaf (tree row, rotated FFA row, output row, two indices,
+    shift, mask) and the ascending seed 8 nbins (G + 1) + 8 G bytes for G segments; rows that
+    several leaves share may come from the caches, so the minimum HBM bytes (every distinct
+    row once) are given beside them."""
+    import ctypes
+
+    import torch
+
+    import loki_amd
+
+    lib = ctypes.CDLL(str(loki_amd.LIBRARY_PATH))
+    vp, sz, u32 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint32
+    lib.loki_shift_add_linear_batch_device.argtypes = [vp] * 7 + [sz, sz, u32, u32, sz, vp]
+    lib.loki_shift_add_ascend_linear_batch_device.argtypes = [vp] * 5 + [sz] * 5 + [vp]
+    lib.loki_shift_add_linear_complex_batch_device.argtypes = [vp] * 7 + [sz, sz, sz, u32, u32,
+                                                                         sz, vp]
+    nbins, nleaves, G, nci = 64, 1 << 21, 8, 1 << 16
+    nbf = nbins // 2 + 1
+    g = torch.Generator(device=dev).manual_seed(7)
+    tree = torch.randn(nleaves, 2, nbins, device=dev, generator=g)
+    ffa = torch.randn(G, nci, 2, nbins, device=dev, generator=g)
+    out = torch.empty(nleaves, 2, nbins, device=dev)
+    # a permutation: every tree row is read once
+    itree = torch.randperm(nleaves, device=dev, generator=g).to(torch.int32)
+    iffa = torch.randint(0, G * nci, (nleaves,), device=dev, dtype=torch.int32, generator=g)
+    iffa_a = torch.randint(0, nci, (G, nleaves), device=dev, dtype=torch.int32, generator=g)
+    iseg = torch.arange(G, device=dev, dtype=torch.int32)
+    shift = torch.rand(nleaves, device=dev, generator=g) * nbins
+    shift_a = torch.rand(G, nleaves, device=dev, generator=g) * nbins
+    mask = torch.ones(nleaves, device=dev, dtype=torch.uint8)
+    tree_c = torch.randn(nleaves, 2, nbf, 2, device=dev, generator=g)
+    ffa_c = torch.randn(G * nci, 2, nbf, 2, device=dev, generator=g)
+    out_c = torch.empty(nleaves, 2, nbf, 2, device=dev)
+    torch.cuda.synchronize()
+    s = stream.cuda_stream
+
+    def linear():
+        return lib.loki_shift_add_linear_batch_device(
+            tree.data_ptr(), itree.data_ptr(), mask.data_ptr(), ffa.data_ptr(), iffa.data_ptr(),
+            shift.data_ptr(), out.data_ptr(), nbins, nleaves, 0, nleaves, G * nci, s)
+
+    def ascend():
+        return lib.loki_shift_add_ascend_linear_batch_device(
+            ffa.data_ptr(), iseg.data_ptr(), iffa_a.data_ptr(), shift_a.data_ptr(),
+            out.data_ptr(), nbins, nci, nleaves, G, G, s)
+
+    def linear_c():
+        return lib.loki_shift_add_linear_complex_batch_device(
+            tree_c.data_ptr(), itree.data_ptr(), mask.data_ptr(), ffa_c.data_ptr(),
+            iffa.data_ptr(), shift.data_ptr(), out_c.data_ptr(), nbf, nbins, nleaves, 0,
+            nleaves, G * nci, s)
+
+    res = {"nbins": nbins, "n_leaves": nleaves, "segments": G, "ffa_rows": G * nci}
+    ffa_rows_b = G * nci * 2 * nbins * 4
+    for name, fn, logical, hbm_min in (
+        ("linear", linear, nleaves * (24 * nbins + 13),
+         nleaves * (16 * nbins + 13) + min(ffa_rows_b, nleaves * 8 * nbins)),
+        ("ascend", ascend, nleaves * (8 * nbins * (G + 1) + 8 * G),
+         nleaves * (8 * nbins + 8 * G) + ffa_rows_b),
+        ("linear_complex", linear_c, nleaves * (48 * nbf + 13),
+         nleaves * (32 * nbf + 13) + min(G * nci * 16 * nbf, nleaves * 16 * nbf)),
+    ):
+        assert fn() == 0, lib.loki_last_error()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        res[name] = {"avg_ms": round(ms, 4), "logical_bytes": logical,
+                     "logical_GBps": round(logical / (ms * 1e-3) / 1e9, 1),
+                     "hbm_min_bytes": hbm_min,
+                     "hbm_min_GBps": round(hbm_min / (ms * 1e-3) / 1e9, 1),
+                     "hbm_min_frac": round(hbm_min / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 3)}
+    del tree, ffa, out, tree_c, ffa_c, out_c
+    torch.cuda.empty_cache()
+    return res
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -601,6 +686,10 @@ def main() -> int:
                         "param_trials_per_s": nc / (f_ms * 1e-3),
                         "candidates": int(f_cnt.item()), "capacity": cap}
 
+    seeds = None
+    if not args.no_seeds and not fourier:
+        seeds = run_seeds(dev, stream, args.steps)
+
     # signal recovery check (outside the timed region)
     best = int(torch.argmax(scores).item())
     shape = [int(x) for x in plan.fold_shapes[-1][1:-2]]
@@ -661,6 +750,7 @@ def main() -> int:
         "kernel_ms": kern_ms,
         "storage_compare": storage_cmp,
         "score_filter": score_filter,
+        "pruning_seeds": seeds,
         "found": found,
         "injected": {"freq": sig["f0"], "accel": sig["accel"], "snr": sig["snr"]},
     }

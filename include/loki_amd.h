@@ -284,6 +284,48 @@ int loki_batch_set_schedule(loki_batch_t batch, const loki_batch_schedule* sched
 int loki_batch_autotune(loki_batch_t batch, void* stream, int repeats,
                         loki_batch_schedule* chosen);
 
+/* --- Pruning seeds from final-level FFA folds ---------------------------- */
+/* Rows are [e: W][v: W]: W = nbins floats (time) or nbins_f = nbins/2+1 complex64 (Fourier,
+ * passed as void*).  Asynchronous on `stream`; every pointer is device memory.  Beyond the
+ * reference's arguments each entry takes the number of rows (or segments) of folds_ffa: an
+ * index at or beyond it, or a tree index still >= capacity after one wrap, writes NaN to that
+ * leaf's bins instead of reading out of bounds. */
+/* kernels.hpp:225-236 shift_add_linear_batch_cuda (kernels_cuda.cu:16-65, 1236-1261):
+ * for each leaf l with validation_mask[l] != 0 (NULL: every leaf),
+ *   folds_out[l] = folds_tree[(indices_tree[l] + physical_start_idx) wrapped at capacity]
+ *                + rotate_right(folds_ffa[indices_ffa[l]], u32(phase_shift[l] + 0.5)). */
+int loki_shift_add_linear_batch_device(const float* folds_tree, const uint32_t* indices_tree,
+                                       const uint8_t* validation_mask, const float* folds_ffa,
+                                       const uint32_t* indices_ffa, const float* phase_shift,
+                                       float* folds_out, size_t nbins, size_t n_leaves,
+                                       uint32_t physical_start_idx, uint32_t capacity,
+                                       size_t ffa_rows, void* stream);
+/* kernels.hpp:238-251 shift_add_linear_complex_batch_cuda (kernels_cuda.cu:141-203): as above
+ * with the head multiplied by exp(-2 pi i k phase_shift[l] / nbins) per harmonic k. */
+int loki_shift_add_linear_complex_batch_device(
+    const void* folds_tree, const uint32_t* indices_tree, const uint8_t* validation_mask,
+    const void* folds_ffa, const uint32_t* indices_ffa, const float* phase_shift,
+    void* folds_out, size_t nbins_f, size_t nbins, size_t n_leaves, uint32_t physical_start_idx,
+    uint32_t capacity, size_t ffa_rows, void* stream);
+/* kernels.hpp:253-263 shift_add_ascend_linear_batch_cuda (kernels_cuda.cu:67-139): for every
+ * leaf l, folds_tree[l] = sum over g < n_segments, in order, of
+ *   rotate_right(folds_ffa[indices_segment[g]][indices_ffa[g * n_leaves + l]],
+ *                u32(phase_shift[g * n_leaves + l] + 0.5));
+ * folds_ffa is [ffa_segments][n_coords_init][2][nbins]. */
+int loki_shift_add_ascend_linear_batch_device(const float* folds_ffa,
+                                              const uint32_t* indices_segment,
+                                              const uint32_t* indices_ffa,
+                                              const float* phase_shift, float* folds_tree,
+                                              size_t nbins, size_t n_coords_init,
+                                              size_t n_leaves, size_t n_segments,
+                                              size_t ffa_segments, void* stream);
+/* kernels.hpp:265-276 shift_add_ascend_linear_complex_batch_cuda (kernels_cuda.cu:205-277). */
+int loki_shift_add_ascend_linear_complex_batch_device(
+    const void* folds_ffa, const uint32_t* indices_segment, const uint32_t* indices_ffa,
+    const float* phase_shift, void* folds_tree, size_t nbins_f, size_t nbins,
+    size_t n_coords_init, size_t n_leaves, size_t n_segments, size_t ffa_segments,
+    void* stream);
+
 /* --- Frequency sweep: region planner and driver -------------------------- */
 /* regions.hpp:40-46 generate_ffa_regions (coord.hpp FFARegion): octave bands of the period
  * range [p_min, p_max] with nbins growing so that one bin keeps its width (p_min / nbins_min)

@@ -1050,6 +1050,95 @@ int loki_snr_boxcar_3d_filter_device(const float* folds, size_t nprof, size_t nb
     });
 }
 
+namespace {
+void check_seed(size_t nbins, size_t nleaves) {
+    REQ(nbins >= 1 && nbins <= 0xFFFFFFFFULL, "nbins must be in [1, 2^32)");
+    REQ(nleaves < (1ULL << 32), "n_leaves must be < 2^32");
+}
+}  // namespace
+
+int loki_shift_add_linear_batch_device(const float* folds_tree, const uint32_t* indices_tree,
+                                       const uint8_t* validation_mask, const float* folds_ffa,
+                                       const uint32_t* indices_ffa, const float* phase_shift,
+                                       float* folds_out, size_t nbins, size_t n_leaves,
+                                       uint32_t physical_start_idx, uint32_t capacity,
+                                       size_t ffa_rows, void* stream) {
+    return guarded([&] {
+        check_seed(nbins, n_leaves);
+        REQ(n_leaves == 0 || (folds_tree && indices_tree && folds_ffa && indices_ffa &&
+                              phase_shift && folds_out),
+            "null argument");
+        loki_amd::kern::seed_linear(folds_tree, indices_tree, validation_mask, folds_ffa,
+                                    indices_ffa, phase_shift, folds_out,
+                                    static_cast<uint32_t>(nbins), n_leaves, physical_start_idx,
+                                    capacity, ffa_rows, as_stream(stream));
+        loki_amd::hip_check(hipGetLastError(), "shift_add_linear launch");
+    });
+}
+
+int loki_shift_add_linear_complex_batch_device(
+    const void* folds_tree, const uint32_t* indices_tree, const uint8_t* validation_mask,
+    const void* folds_ffa, const uint32_t* indices_ffa, const float* phase_shift,
+    void* folds_out, size_t nbins_f, size_t nbins, size_t n_leaves, uint32_t physical_start_idx,
+    uint32_t capacity, size_t ffa_rows, void* stream) {
+    return guarded([&] {
+        check_seed(nbins, n_leaves);
+        REQ(nbins_f == nbins / 2 + 1, "nbins_f must be nbins / 2 + 1");
+        REQ(n_leaves == 0 || (folds_tree && indices_tree && folds_ffa && indices_ffa &&
+                              phase_shift && folds_out),
+            "null argument");
+        loki_amd::kern::seed_linear_complex(
+            static_cast<const float2*>(folds_tree), indices_tree, validation_mask,
+            static_cast<const float2*>(folds_ffa), indices_ffa, phase_shift,
+            static_cast<float2*>(folds_out), static_cast<uint32_t>(nbins_f),
+            static_cast<uint32_t>(nbins), n_leaves, physical_start_idx, capacity, ffa_rows,
+            as_stream(stream));
+        loki_amd::hip_check(hipGetLastError(), "shift_add_linear_complex launch");
+    });
+}
+
+int loki_shift_add_ascend_linear_batch_device(const float* folds_ffa,
+                                              const uint32_t* indices_segment,
+                                              const uint32_t* indices_ffa,
+                                              const float* phase_shift, float* folds_tree,
+                                              size_t nbins, size_t n_coords_init,
+                                              size_t n_leaves, size_t n_segments,
+                                              size_t ffa_segments, void* stream) {
+    return guarded([&] {
+        check_seed(nbins, n_leaves);
+        REQ(n_segments <= 0xFFFFFFFFULL, "n_segments must be < 2^32");
+        REQ(n_leaves == 0 || (folds_tree && (n_segments == 0 || (folds_ffa && indices_segment &&
+                                                                 indices_ffa && phase_shift))),
+            "null argument");
+        loki_amd::kern::seed_ascend(folds_ffa, indices_segment, indices_ffa, phase_shift,
+                                    folds_tree, static_cast<uint32_t>(nbins), n_coords_init,
+                                    n_leaves, static_cast<uint32_t>(n_segments), ffa_segments,
+                                    as_stream(stream));
+        loki_amd::hip_check(hipGetLastError(), "shift_add_ascend_linear launch");
+    });
+}
+
+int loki_shift_add_ascend_linear_complex_batch_device(
+    const void* folds_ffa, const uint32_t* indices_segment, const uint32_t* indices_ffa,
+    const float* phase_shift, void* folds_tree, size_t nbins_f, size_t nbins,
+    size_t n_coords_init, size_t n_leaves, size_t n_segments, size_t ffa_segments,
+    void* stream) {
+    return guarded([&] {
+        check_seed(nbins, n_leaves);
+        REQ(nbins_f == nbins / 2 + 1, "nbins_f must be nbins / 2 + 1");
+        REQ(n_segments <= 0xFFFFFFFFULL, "n_segments must be < 2^32");
+        REQ(n_leaves == 0 || (folds_tree && (n_segments == 0 || (folds_ffa && indices_segment &&
+                                                                 indices_ffa && phase_shift))),
+            "null argument");
+        loki_amd::kern::seed_ascend_complex(
+            static_cast<const float2*>(folds_ffa), indices_segment, indices_ffa, phase_shift,
+            static_cast<float2*>(folds_tree), static_cast<uint32_t>(nbins_f),
+            static_cast<uint32_t>(nbins), n_coords_init, n_leaves,
+            static_cast<uint32_t>(n_segments), ffa_segments, as_stream(stream));
+        loki_amd::hip_check(hipGetLastError(), "shift_add_ascend_linear_complex launch");
+    });
+}
+
 int loki_snr_boxcar_3d_filter_host(const float* folds, size_t nprof, size_t nbins,
                                    const size_t* widths, size_t nwidths, float threshold,
                                    int find_max, uint32_t* indices, float* scores,

@@ -233,6 +233,25 @@ void irfft_score(const float2* fold, uint64_t ntrials, uint32_t nbins, const Wid
                  float* scores, bool find_max, bool exact, hipStream_t s,
                  const FilterOut* fo = nullptr);
 
+/// Pruning seeds from final-level FFA folds (kernels_cuda.cu:16-277): rows of
+/// [e: W][v: W], W = nbins floats (time) or nbf float2 (Fourier).  ffa_rows /
+/// ffa_segments bound the indices (an index beyond them gives NaN output).
+void seed_linear(const float* tree, const uint32_t* itree, const uint8_t* mask, const float* ffa,
+                 const uint32_t* iffa, const float* shift, float* out, uint32_t nbins,
+                 uint64_t nleaves, uint32_t start, uint32_t capacity, uint64_t ffa_rows,
+                 hipStream_t s);
+void seed_ascend(const float* ffa, const uint32_t* iseg, const uint32_t* iffa,
+                 const float* shift, float* out, uint32_t nbins, uint64_t ncoords_init,
+                 uint64_t nleaves, uint32_t nsegments, uint64_t ffa_segments, hipStream_t s);
+void seed_linear_complex(const float2* tree, const uint32_t* itree, const uint8_t* mask,
+                         const float2* ffa, const uint32_t* iffa, const float* shift,
+                         float2* out, uint32_t nbf, uint32_t nbins, uint64_t nleaves,
+                         uint32_t start, uint32_t capacity, uint64_t ffa_rows, hipStream_t s);
+void seed_ascend_complex(const float2* ffa, const uint32_t* iseg, const uint32_t* iffa,
+                         const float* shift, float2* out, uint32_t nbf, uint32_t nbins,
+                         uint64_t ncoords_init, uint64_t nleaves, uint32_t nsegments,
+                         uint64_t ffa_segments, hipStream_t s);
+
 /// snr_boxcar_2d_max: folds [nprof, nbins] (already normalised) / stdnoise.
 void snr_boxcar_2d_max(const float* folds, uint64_t nprof, uint32_t nbins, const WidthTable& wt,
                        float stdnoise, float* scores, hipStream_t s);
