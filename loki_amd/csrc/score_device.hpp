@@ -16,17 +16,19 @@ namespace loki_amd::kern {
 /// and -0.0 tie, which cannot change the S/N value.
 template <int NB>
 __device__ __forceinline__ float window_max(const float* __restrict__ hi, const float* ps) {
-    float dmax = -FLT_MAX;
+    // two running maxima, each taking two differences per step (one v_max3_f32)
+    float ma = -FLT_MAX, mb = -FLT_MAX;
+    int j = 0;
 #pragma unroll
-    for (int j = 0; j + 1 < NB; j += 2) {
-        const float d0 = hi[j] - ps[j];
-        const float d1 = hi[j + 1] - ps[j + 1];
-        dmax           = __builtin_fmaxf(dmax, __builtin_fmaxf(d0, d1));
+    for (; j + 3 < NB; j += 4) {
+        ma = __builtin_fmaxf(__builtin_fmaxf(ma, hi[j] - ps[j]), hi[j + 1] - ps[j + 1]);
+        mb = __builtin_fmaxf(__builtin_fmaxf(mb, hi[j + 2] - ps[j + 2]), hi[j + 3] - ps[j + 3]);
     }
-    if constexpr ((NB & 1) != 0) {
-        dmax = __builtin_fmaxf(dmax, hi[NB - 1] - ps[NB - 1]);
+#pragma unroll
+    for (; j < NB; ++j) {
+        ma = __builtin_fmaxf(ma, hi[j] - ps[j]);
     }
-    return dmax;
+    return __builtin_fmaxf(ma, mb);
 }
 
 /// Profile normalisation p = e / sqrt(v).

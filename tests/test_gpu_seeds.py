@@ -67,7 +67,8 @@ def _linear_inputs(rng, nbins, n_tree, n_ffa, n_leaves, start):
     return tree, ffa, itree, iffa, shift, mask
 
 
-@pytest.mark.parametrize("nbins,n_leaves", [(64, 5000), (100, 777), (32, 1 << 16), (8, 3)])
+@pytest.mark.parametrize("nbins,n_leaves", [(64, 5000), (100, 777), (32, 1 << 16), (8, 3),
+                                             (33, 1000), (30, 513)])
 def test_linear_time_bit_exact(nbins, n_leaves):
     import torch
 
@@ -95,7 +96,8 @@ def test_linear_time_bit_exact(nbins, n_leaves):
     np.testing.assert_array_equal(d[6].cpu().numpy(), want)
 
 
-@pytest.mark.parametrize("nbins,nleaves,nseg", [(64, 3000, 4), (48, 513, 7), (32, 1 << 15, 2)])
+@pytest.mark.parametrize("nbins,nleaves,nseg", [(64, 3000, 4), (48, 513, 7), (32, 1 << 15, 2),
+                                               (30, 700, 5), (64, 100, 1)])
 def test_ascend_time_bit_exact(nbins, nleaves, nseg):
     import torch
 
@@ -115,6 +117,29 @@ def test_ascend_time_bit_exact(nbins, nleaves, nseg):
     torch.cuda.synchronize()
     want = seeds.shift_add_ascend_linear(ffa, iseg, iffa, sh, nbins, nci, nleaves)
     np.testing.assert_array_equal(d[4].cpu().numpy(), want)
+
+
+def test_linear_time_unaligned_buffers_take_the_scalar_path():
+    import torch
+
+    from oracle import seeds
+
+    lib = _lib()
+    rng = np.random.default_rng(44)
+    nbins, n_tree, n_ffa, n, start = 64, 50, 40, 300, 7
+    tree, ffa, itree, iffa, shift, mask = _linear_inputs(rng, nbins, n_tree, n_ffa, n, start)
+    # each array one float past a 16-byte boundary
+    buf_t = _dev(np.concatenate([[0.0], tree.ravel()]).astype(np.float32))
+    buf_f = _dev(np.concatenate([[0.0], ffa.ravel()]).astype(np.float32))
+    buf_o = _dev(np.zeros(1 + n * 2 * nbins, np.float32))
+    di, dm, dfi, ds = _dev(itree), _dev(mask), _dev(iffa), _dev(shift)
+    _ok(lib, lib.loki_shift_add_linear_batch_device(
+        buf_t.data_ptr() + 4, di.data_ptr(), dm.data_ptr(), buf_f.data_ptr() + 4, dfi.data_ptr(),
+        ds.data_ptr(), buf_o.data_ptr() + 4, nbins, n, start, n_tree, n_ffa, None))
+    torch.cuda.synchronize()
+    want = seeds.shift_add_linear(tree, itree, mask, ffa, iffa, shift,
+                                  np.zeros((n, 2, nbins), np.float32), nbins, start, n_tree)
+    np.testing.assert_array_equal(buf_o.cpu().numpy()[1:].reshape(n, 2, nbins), want)
 
 
 def test_seeds_from_a_real_ffa_fold():
