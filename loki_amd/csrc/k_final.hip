@@ -19,6 +19,7 @@
 // normalisation.
 #include <cfloat>
 #include <type_traits>
+#include <utility>
 
 #include "merge_device.hpp"
 #include "resolve_device.hpp"
@@ -80,6 +81,42 @@ __device__ __forceinline__ void merge_store(const float4 te, const float4 tv, co
     row[(b0 + 3) & (NB - 1)] = p.w;
 }
 
+/// Widest rows whose boxcar windows are formed in registers (compile-time
+/// window offsets, no prefix-sum row in LDS); wider rows use the LDS row.
+constexpr int kRegWindowMaxNB = 32;
+
+/// max_j (hi_j - ps[j]) for the fixed width W, hi_j = ps[j + W] or, past the
+/// end of the row, ext[j + W - NB] = ps[j + W - NB] + sum: the same operands
+/// as the LDS prefix-sum row of window_max.
+template <int NB, int W>
+__device__ __forceinline__ float window_max_fixed(const float (&ps)[NB],
+                                                  const float (&ext)[NB - 1]) {
+    float ma = -FLT_MAX, mb = -FLT_MAX;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        const float hi = (j + W < NB) ? ps[(j + W) % NB] : ext[(j + W - NB) % (NB - 1)];
+        const float d  = hi - ps[j];
+        if ((j & 2) == 0) {
+            ma = __builtin_fmaxf(ma, d);
+        } else {
+            mb = __builtin_fmaxf(mb, d);
+        }
+    }
+    return __builtin_fmaxf(ma, mb);
+}
+
+/// window_max_fixed<NB, w> for the runtime, wave-uniform width w in [1, NB).
+template <int NB, int... Ws>
+__device__ __forceinline__ float window_max_reg(std::integer_sequence<int, Ws...>,
+                                                const float (&ps)[NB],
+                                                const float (&ext)[NB - 1], uint32_t w) {
+    float r = 0.0F;
+    (void)((w == static_cast<uint32_t>(Ws + 1) ? (r = window_max_fixed<NB, Ws + 1>(ps, ext), true)
+                                               : false) ||
+           ...);
+    return r;
+}
+
 template <int NB, int NP, bool FINDMAX, bool EXACT, typename TI>
 __device__ __forceinline__ void final_block(uint32_t bid, const TI* __restrict__ in,
                                             uint64_t nprev, const ResolveLevel& lv,
@@ -91,7 +128,7 @@ __device__ __forceinline__ void final_block(uint32_t bid, const TI* __restrict__
     constexpr uint32_t LPR = C4;            // lanes per row (4 bins of e and v each)
     constexpr uint32_t RPI = kRows / LPR;   // rows per merge step
     constexpr uint32_t NSTEP = kRows / RPI; // merge steps for a full tile
-    constexpr uint32_t U   = NSTEP >= 8 ? 8 : NSTEP;  // row steps with loads in flight
+    constexpr uint32_t U   = NSTEP >= 4 ? 4 : NSTEP;  // row steps with loads in flight
     extern __shared__ float s_p[];          // [kRows][stride]
     __shared__ FFACoord s_coord[kRows];
     const uint32_t stride = fg.stride;
@@ -156,8 +193,10 @@ __device__ __forceinline__ void final_block(uint32_t bid, const TI* __restrict__
         return;
     }
 
-    // 3. boxcar S/N of row `lane`: the row's slots become the circular
-    //    prefix sum (n + wmax entries, stride >= n + wmax).
+    // 3. boxcar S/N of row `lane`: circular prefix sum ps and its wrap
+    //    ext[k] = ps[k] + sum (score.cpp:143-156).  Up to kRegWindowMaxNB bins
+    //    both stay in registers; wider rows become the row's LDS slots
+    //    (n + wmax entries, stride >= n + wmax).
     float* __restrict__ row = s_p + lane * stride;
     float ps[NB];
     float acc = 0.0F;
@@ -168,16 +207,30 @@ __device__ __forceinline__ void final_block(uint32_t bid, const TI* __restrict__
         ps[j]         = acc;
     }
     const float sum = ps[NB - 1];
+    [[maybe_unused]] float ext[NB - 1];
+    if constexpr (NB <= kRegWindowMaxNB) {
 #pragma unroll
-    for (int j = 0; j < NB; ++j) {
-        row[j] = ps[j];
-    }
-    for (uint32_t j = 0; j < wt.wmax; ++j) {
-        row[NB + j] = row[j] + sum;
+        for (int k = 0; k < NB - 1; ++k) {
+            ext[k] = ps[k] + sum;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            row[j] = ps[j];
+        }
+        for (uint32_t j = 0; j < wt.wmax; ++j) {
+            row[NB + j] = row[j] + sum;
+        }
     }
     float best = -FLT_MAX;
     for (uint32_t iw = 0; iw < wt.n; ++iw) {
-        const float dmax = window_max<NB>(row + wt.w[iw], ps);
+        float dmax;
+        if constexpr (NB <= kRegWindowMaxNB) {
+            dmax = window_max_reg<NB>(std::make_integer_sequence<int, NB - 1>{}, ps, ext,
+                                      wt.w[iw]);
+        } else {
+            dmax = window_max<NB>(row + wt.w[iw], ps);
+        }
         const float snr  = ((wt.h[iw] + wt.b[iw]) * dmax) - (wt.b[iw] * sum);
         if constexpr (FINDMAX) {
             best = (best < snr) ? snr : best;
@@ -361,7 +414,9 @@ void final_grouped(const FinalTask* tasks, uint32_t ntasks, uint32_t nblocks, ui
 }
 
 uint32_t final_lds_stride(uint32_t nbins, uint32_t wmax) {
-    return (nbins + wmax) | 1U;  // odd: conflict-free rows; n + wmax prefix-sum slots
+    // odd: conflict-free rows; n profile slots (windows in registers) or
+    // n + wmax prefix-sum slots
+    return (nbins <= static_cast<uint32_t>(kRegWindowMaxNB) ? nbins : nbins + wmax) | 1U;
 }
 
 }  // namespace loki_amd::kern
