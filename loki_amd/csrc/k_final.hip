@@ -11,8 +11,13 @@
 //      profile p = e * (1/sqrt(v)) (score.cpp:139-141); loads of several rows
 //      are in flight at once;
 //   3. lane i scores row i like snr_boxcar_3d (score.cpp:95-172): sequential
-//      circular prefix sum in VGPRs + its LDS row, max window difference per
-//      width, snr = (h + b) * dmax - b * sum.
+//      circular prefix sum, max window difference per width, snr = (h + b) *
+//      dmax - b * sum.  For nbins <= 32 the prefix sum and its wrap stay in
+//      VGPRs and each width's window offsets are compile-time (one unrolled
+//      window per possible width, picked by the wave-uniform width), so the
+//      LDS holds only the 64 profile rows -- the shift-table records live in
+//      the rows' slots until the profiles replace them -- and four waves fit
+//      on each SIMD; wider rows keep the prefix sum in an LDS row.
 // Every FP32 operation matches the unfused path and the CPU oracle bit for
 // bit.  With FP16/BF16 parents (storage.hpp) the loads are 8-byte chunks
 // widened to FP32 and the merged sums are unscaled exactly before step 2's
@@ -130,8 +135,37 @@ __device__ __forceinline__ void final_block(uint32_t bid, const TI* __restrict__
     constexpr uint32_t NSTEP = kRows / RPI; // merge steps for a full tile
     constexpr uint32_t U   = NSTEP >= 4 ? 4 : NSTEP;  // row steps with loads in flight
     extern __shared__ float s_p[];          // [kRows][stride]
-    __shared__ FFACoord s_coord[kRows];
+    // With register windows a row's LDS slots hold its shift-table record
+    // until its profile replaces it (row r's record is read, in program
+    // order, before any lane stores row r), so the workgroup needs no LDS
+    // beyond the kRows x stride profile rows.
+    constexpr bool kCoordInRow = NB <= kRegWindowMaxNB;
+    __shared__ FFACoord s_coord[kCoordInRow ? 1 : kRows];
     const uint32_t stride = fg.stride;
+    auto coord_store = [&](uint32_t r, const FFACoord& cd) {
+        if constexpr (kCoordInRow) {
+            auto* q = reinterpret_cast<uint32_t*>(s_p + r * stride);
+            q[0] = cd.i_tail;
+            q[1] = __float_as_uint(cd.shift_tail);
+            q[2] = cd.i_head;
+            q[3] = __float_as_uint(cd.shift_head);
+        } else {
+            s_coord[r] = cd;
+        }
+    };
+    auto coord_load = [&](uint32_t r) {
+        if constexpr (kCoordInRow) {
+            const auto* q = reinterpret_cast<const uint32_t*>(s_p + r * stride);
+            FFACoord cd;
+            cd.i_tail     = q[0];
+            cd.shift_tail = __uint_as_float(q[1]);
+            cd.i_head     = q[2];
+            cd.shift_head = __uint_as_float(q[3]);
+            return cd;
+        } else {
+            return s_coord[r];
+        }
+    };
 
     const Tile tl = tile_of(ts, xcd_remap(bid, ts.nblocks), kRows);
     if (tl.count == 0) {
@@ -143,7 +177,7 @@ __device__ __forceinline__ void final_block(uint32_t bid, const TI* __restrict__
 
     // 1. shift tables for this tile (FP64)
     if (lane < nrow) {
-        s_coord[lane] = resolve_pair<NP>(lv, fg.dv, c);
+        coord_store(lane, resolve_pair<NP>(lv, fg.dv, c));
     }
     __syncthreads();
 
@@ -160,7 +194,7 @@ __device__ __forceinline__ void final_block(uint32_t bid, const TI* __restrict__
             for (uint32_t u = 0; u < U; ++u) {
                 const uint32_t r = rsub + (s0 + u) * RPI;
                 if (r < nrow) {
-                    const FFACoord cd = s_coord[r];
+                    const FFACoord cd = coord_load(r);
                     const CI* __restrict__ trow = reinterpret_cast<const CI*>(
                         in + static_cast<uint64_t>(cd.i_tail) * (2 * NB));
                     const CI* __restrict__ hrow = reinterpret_cast<const CI*>(
