@@ -101,13 +101,21 @@ __device__ __forceinline__ Tile tile_of(const TileSchedule& ts, uint32_t t, uint
     return r;
 }
 
-/// Elements r..r+3 of the 8-element concatenation (a, b).
+/// Elements r..r+3 (r < 4) of the 8-element concatenation (a, b), as two
+/// shift stages: by 2 when r & 2 (five selects), then by 1 when r & 1 (four).
 __device__ __forceinline__ float4 funnel4(const float4 a, const float4 b, uint32_t r) {
+    const bool s2 = (r & 2U) != 0;
+    const bool s1 = (r & 1U) != 0;
+    const float t0 = s2 ? a.z : a.x;
+    const float t1 = s2 ? a.w : a.y;
+    const float t2 = s2 ? b.x : a.z;
+    const float t3 = s2 ? b.y : a.w;
+    const float t4 = s2 ? b.z : b.x;
     float4 o;
-    o.x = r == 0 ? a.x : (r == 1 ? a.y : (r == 2 ? a.z : a.w));
-    o.y = r == 0 ? a.y : (r == 1 ? a.z : (r == 2 ? a.w : b.x));
-    o.z = r == 0 ? a.z : (r == 1 ? a.w : (r == 2 ? b.x : b.y));
-    o.w = r == 0 ? a.w : (r == 1 ? b.x : (r == 2 ? b.y : b.z));
+    o.x = s1 ? t1 : t0;
+    o.y = s1 ? t2 : t1;
+    o.z = s1 ? t3 : t2;
+    o.w = s1 ? t4 : t3;
     return o;
 }
 
