@@ -260,6 +260,10 @@ __device__ __forceinline__ void final_block(uint32_t bid, const TI* __restrict__
     for (uint32_t iw = 0; iw < wt.n; ++iw) {
         float dmax;
         if constexpr (NB <= kRegWindowMaxNB) {
+            // The windows do not depend on the width loop; hoisted out of it,
+            // all NB - 1 of them would be formed for every row.  An empty
+            // volatile asm on one operand ties them to this iteration.
+            __asm__ volatile("" : "+v"(ps[0]));
             dmax = window_max_reg<NB>(std::make_integer_sequence<int, NB - 1>{}, ps, ext,
                                       wt.w[iw]);
         } else {
