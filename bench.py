@@ -78,6 +78,15 @@ WORKLOADS = {
         batch=True, nseries=256, seed=7000, tsamp=64e-6, nsamps=2**22, nbins=128, eta=1.0,
         cpu_sample=dict(nseries=12),
     ),
+    # Frequency sweep over four octaves (25-400 Hz) with an accel range: the region planner's
+    # chunks (nbins 32..256, eta 1..8) each run FFA + score-and-filter at S/N > 7 into one
+    # candidate list, written to a result archive (FFAFreqSweepCUDA).
+    "freq_sweep_accel": dict(
+        sweep=True, nsamps=2**22, tsamp=64e-6, nbins=32, eta=1.0,
+        param_limits=[(-30.0, 30.0), (25.0, 400.0)], snr_min=7.0, max_process_memory_gb=64.0,
+        injections=[dict(f0=f, fdot=0.0, fddot=0.0, fwhm=0.05, snr=15.0)
+                    for f in (31.7, 77.3, 151.9, 333.3)], seed=8080,
+    ),
     "ffa_freq_only": dict(
         nsamps=2**20, tsamp=64e-6, nbins=64, eta=1.0, param_limits=[(0.5, 2.0)],
         signal=dict(f0=1.0, accel=0.0, jerk=0.0, fwhm=0.05, snr=20.0), seed=101,
@@ -360,6 +369,57 @@ def storage_error_vs_fp64(wl, ts_e, ts_v, widths) -> dict:
     return out
 
 
+def run_sweep(args, wl) -> int:
+    """The sweep workload: value = final trials of all chunks per second, end to end (host
+    series in, candidates and the result archive out)."""
+    import tempfile
+
+    from loki_amd import cands, libculoki, libloki
+    from loki_amd.synth import make_injected_series
+
+    cfg = libloki.configs.PulsarSearchConfig(
+        nsamps=wl["nsamps"], tsamp=wl["tsamp"], nbins=wl["nbins"], eta=wl["eta"],
+        param_limits=np.array(wl["param_limits"]), use_fourier=False, snr_min=wl["snr_min"],
+        max_process_memory_gb=wl["max_process_memory_gb"])
+    ts_e, ts_v, _ = make_injected_series(wl["nsamps"], wl["tsamp"], wl["seed"],
+                                         wl["injections"], nbins=wl["nbins"])
+    sweep = libculoki.ffa.FFAFreqSweepCUDA(cfg, device_id=0)
+    planner = sweep.planner
+    trials = sum(int(libloki.plans.FFAPlanTime(c).ncoords[-1]) for c in planner.cfgs)
+    with tempfile.TemporaryDirectory() as out:
+        for _ in range(args.warmup):
+            sweep.execute(ts_e, ts_v, out, "bench")
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            path = sweep.execute(ts_e, ts_v, out, "bench")
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        res = cands.read_results(path)
+    snr, rows = res["snr"], res["param_sets"]
+    found = []
+    for sg in wl["injections"]:
+        near = np.abs(rows[:, -2] - sg["f0"]) < 0.05
+        found.append(float(snr[near].max()) if near.any() else None)
+    result = {
+        "metric": "FFA param trials/s (frequency sweep)", "value": trials / (ms * 1e-3),
+        "unit": "param trials/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": args.workload, "nsamps": wl["nsamps"], "tsamp": wl["tsamp"],
+                   "param_limits": wl["param_limits"], "snr_min": wl["snr_min"],
+                   "max_process_memory_gb": wl["max_process_memory_gb"],
+                   "chunks": [{"nbins": int(c.nbins), "eta": float(c.eta),
+                               "freq": [float(x) for x in c.param_limits[-1]]}
+                              for c in planner.cfgs],
+                   "trials_per_step": trials},
+        "roofline": None,
+        "candidates": int(snr.size),
+        "per_chunk_candidates": [int(x) for x in sweep.per_region],
+        "injected_snr_found": found,
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
 def run_batch(args, wl) -> int:
     """The batched workload: value = final trials of all series per second."""
     import torch
@@ -569,6 +629,8 @@ def main() -> int:
     wl = WORKLOADS[args.workload]
     if wl.get("batch"):
         return run_batch(args, wl)
+    if wl.get("sweep"):
+        return run_sweep(args, wl)
     nbins = wl["nbins"]
     widths = np.array([w for w in WIDTHS if w < nbins], dtype=np.uint64)
     sig = wl["signal"]
