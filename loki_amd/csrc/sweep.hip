@@ -20,6 +20,11 @@ FFAFreqSweep::FFAFreqSweep(const SearchConfig& cfg, int device_id)
       m_device(device_id),
       m_planner(cfg, m_kind, /*use_gpu=*/true) {
     LOKI_REQUIRE(m_planner.nregions() > 0, "FFAFreqSweep: the region planner found no chunks");
+    double total_gb = 0.0;
+    for (const auto& c : m_planner.chunk_stats()) {
+        total_gb += c.total_memory_gb;
+    }
+    m_keep_executors = total_gb <= cfg.get_max_process_memory_gb() - 0.5;
 }
 
 SizeType FFAFreqSweep::execute(const float* ts_e, const float* ts_v, SizeType nsamps) {
@@ -28,14 +33,31 @@ SizeType FFAFreqSweep::execute(const float* ts_e, const float* ts_v, SizeType ns
     LOKI_REQUIRE(nsamps == m_base.get_nsamps(), "FFAFreqSweep: ts_e must have size nsamps");
     const SizeType cap = m_planner.stats().max_scores_size();
     LOKI_REQUIRE(cap < (1ULL << 32), "FFAFreqSweep: candidate buffer exceeds 32-bit indices");
-    DeviceBuffer de(nsamps * 4), dv(nsamps * 4), didx(cap * 4), dsnr(cap * 4), dcnt(4);
+    if (m_de.get() == nullptr) {
+        m_de   = DeviceBuffer(nsamps * 4);
+        m_dv   = DeviceBuffer(nsamps * 4);
+        m_didx = DeviceBuffer(cap * 4);
+        m_dsnr = DeviceBuffer(cap * 4);
+        m_dcnt = DeviceBuffer(4);
+    }
+    DeviceBuffer &de = m_de, &dv = m_dv, &didx = m_didx, &dsnr = m_dsnr, &dcnt = m_dcnt;
     hip_check(hipMemcpy(de.get(), ts_e, nsamps * 4, hipMemcpyHostToDevice), "copy ts_e");
     hip_check(hipMemcpy(dv.get(), ts_v, nsamps * 4, hipMemcpyHostToDevice), "copy ts_v");
     const auto& widths = m_base.get_scoring_widths();
     m_per_region.assign(m_planner.nregions(), 0);
+    if (m_keep_executors && m_exec.size() != m_planner.nregions()) {
+        m_exec.clear();
+        for (const auto& c : m_planner.cfgs()) {
+            m_exec.push_back(std::make_unique<FFAExecutor>(c, m_kind, m_device));
+        }
+    }
     SizeType total = 0;
     for (SizeType r = 0; r < m_planner.nregions(); ++r) {
-        FFAExecutor ffa(m_planner.cfgs()[r], m_kind, m_device);
+        std::unique_ptr<FFAExecutor> own;
+        if (!m_keep_executors) {
+            own = std::make_unique<FFAExecutor>(m_planner.cfgs()[r], m_kind, m_device);
+        }
+        FFAExecutor& ffa = m_keep_executors ? *m_exec[r] : *own;
         kern::FilterOut fo{};
         fo.threshold = static_cast<float>(m_base.get_snr_min());
         fo.capacity  = static_cast<uint32_t>(cap - total);

@@ -1,6 +1,7 @@
 // Frequency sweep over the region planner's chunks (ffa_freq_sweep_cuda.cu:21-342).
 #pragma once
 
+#include <memory>
 #include <vector>
 
 #include "device.hpp"
@@ -26,6 +27,12 @@ private:
     FoldKind m_kind;
     int m_device;
     FFARegionPlanner m_planner;
+    /// Chunk executors kept between calls while their device buffers together
+    /// fit the process memory limit (else each chunk builds and frees its own,
+    /// as the planner's memory model assumes); series and candidate buffers.
+    bool m_keep_executors = false;
+    std::vector<std::unique_ptr<FFAExecutor>> m_exec;
+    DeviceBuffer m_de, m_dv, m_didx, m_dsnr, m_dcnt;
     std::vector<uint32_t> m_idx;
     std::vector<float> m_snr;
     std::vector<double> m_rows;
