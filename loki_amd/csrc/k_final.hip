@@ -377,6 +377,8 @@ void launch_typed(const void* in, float unscale, uint64_t ncur, uint64_t nprev,
         return launch_nb<32>(p, unscale, ncur, nprev, lv, wt, scores, find_max, exact, fo, s);
     case 64:
         return launch_nb<64>(p, unscale, ncur, nprev, lv, wt, scores, find_max, exact, fo, s);
+    case 128:
+        return launch_nb<128>(p, unscale, ncur, nprev, lv, wt, scores, find_max, exact, fo, s);
     default: break;
     }
 }
@@ -384,7 +386,7 @@ void launch_typed(const void* in, float unscale, uint64_t ncur, uint64_t nprev,
 }  // namespace
 
 bool final_merge_score_supported(uint32_t nbins) {
-    return nbins == 8 || nbins == 16 || nbins == 32 || nbins == 64;
+    return nbins == 8 || nbins == 16 || nbins == 32 || nbins == 64 || nbins == 128;
 }
 
 void final_merge_score(const void* in, FoldStorage si, float unscale, uint64_t ncur,

@@ -27,6 +27,7 @@ CASES = {
     "jerk20": (2**20, 32, [(-50.0, 50.0), (-500.0, 500.0), (60.0, 61.0)], {}),
     "acc_eta2_bseg": (2**18, 16, [(-20000.0, 20000.0), (30.0, 40.0)],
                       {"eta": 2.0, "bseg_brute": 2048}),
+    "acc17_nb128": (2**17, 128, [(-60000.0, 60000.0), (60.0, 64.0)], {}),
 }
 
 
@@ -155,7 +156,7 @@ def test_ffa_scores_bit_exact(oracle, name):
 
 
 @pytest.mark.parametrize("name", ["freq16", "acc17w", "acc20", "jerk17w", "jerk20",
-                                  "freq18_nb100", "acc_eta2_bseg"])
+                                  "freq18_nb100", "acc_eta2_bseg", "acc17_nb128"])
 @pytest.mark.parametrize("fused", [True, False])
 @pytest.mark.parametrize("find_max", [False, True])
 def test_scores_fused_and_unfused_paths(oracle, name, fused, find_max):
