@@ -197,8 +197,8 @@ typedef struct {
 } loki_kernel_timing;
 int loki_ffa_set_timing(loki_ffa_t ffa, int enable);
 /* Scores path: 1 (default) fuses the last level's shift tables, merge and S/N into one kernel
- * (the final fold cube is never stored); 0 runs the unfused merge + snr_boxcar_3d.  Both give
- * identical results.  For Fourier plans with nbins 16, 32 or 64, 1 fuses the inverse real DFT
+ * (the final fold cube is never stored) for nbins 8, 16, 32, 64 and 128 (other nbins run
+ * unfused); 0 runs the unfused merge + snr_boxcar_3d.  Both give identical results.  For Fourier plans with nbins 16, 32 or 64, 1 fuses the inverse real DFT
  * of the last level with the S/N (the time-domain final fold is never stored); both orders
  * stay within the Fourier S/N bound, not bitwise equal.  No reference counterpart (the
  * reference always stores the final fold, ffa_cuda.cu:670-692). */
