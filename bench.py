@@ -115,6 +115,8 @@ def parse() -> argparse.Namespace:
                     help="skip the score-and-filter timing")
     ap.add_argument("--filter-threshold", type=float, default=6.0,
                     help="S/N threshold of the score-and-filter timing")
+    ap.add_argument("--dft", default="f32", choices=["f32", "bf16x3"],
+                    help="Fourier brute fold on FP32 matrix cores or split-bf16 ones")
     ap.add_argument("--no-seeds", action="store_true",
                     help="skip the pruning-seed kernel timing (time-domain workloads)")
     ap.add_argument("--no-refine", action="store_true",
@@ -656,6 +658,7 @@ def main() -> int:
     if fourier:
         ffa = libculoki.ffa.FFACOMPLEXCUDA(cfg, device_id=0)
         ffa.set_fused(not args.unfused)
+        ffa.set_dft_precision(args.dft)
     else:
         ffa = libculoki.ffa.FFACUDA(cfg, device_id=0)
         ffa.set_exact_norm(not args.fast_norm)
@@ -803,6 +806,7 @@ def main() -> int:
             "n_levels": int(plan.n_levels), "trials_per_step": nc,
             "domain": "fourier" if fourier else "time",
             "fold_storage": "c64" if fourier else args.storage,
+            "dft": args.dft if fourier else None,
             "score_norm": "hw_rsqrt" if args.fast_norm else "exact",
             "final_level": ("unfused" if args.unfused else
                             "fused_irfft_score" if fourier else "fused"),

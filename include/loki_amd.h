@@ -208,6 +208,11 @@ int loki_ffa_set_fused(loki_ffa_t ffa, int enable);
  * (|dS/N| ~ 1e-6, the analogue of the reference CUDA path's fast-math e/sqrtf(v),
  * score_cuda.cu:82).  No reference counterpart as a switch. */
 int loki_ffa_set_exact_norm(loki_ffa_t ffa, int exact);
+/* Fourier brute fold precision: 0 (default) FP32 matrix cores; 1 bf16 matrix cores with each
+ * operand split into high and low bf16 parts (three products per term, FP32 accumulation,
+ * ~2^-17 relative per product) for nbins <= 129.  No reference counterpart (the reference's
+ * Fourier brute fold is FP32, kernels_cuda.cu:780-965). */
+int loki_ffa_set_dft_precision(loki_ffa_t ffa, int mode);
 /* Storage precision of the intermediate fold levels 0..L-2 of a time-domain FFA object:
  * LOKI_STORAGE_F32 (default), LOKI_STORAGE_F16 or LOKI_STORAGE_BF16.  Each add is done in FP32
  * and each stored value is rounded once (round to nearest even); level l is stored at the exact

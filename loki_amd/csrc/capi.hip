@@ -529,6 +529,14 @@ int loki_ffa_set_fused(loki_ffa_t ffa, int enable) {
     });
 }
 
+int loki_ffa_set_dft_precision(loki_ffa_t ffa, int mode) {
+    return guarded([&] {
+        REQ(ffa != nullptr, "null argument");
+        REQ(mode == 0 || mode == 1, "dft precision must be 0 (f32) or 1 (split bf16)");
+        ffa->exec.set_dft_split_bf16(mode == 1);
+    });
+}
+
 int loki_ffa_set_exact_norm(loki_ffa_t ffa, int exact) {
     return guarded([&] {
         REQ(ffa != nullptr, "null ffa");

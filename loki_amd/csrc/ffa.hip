@@ -251,7 +251,7 @@ void FFAExecutor::init_level(const float* ts_e, const float* ts_v, void* out, vo
                                  m_bf_freqs.as<double>(),
                                  static_cast<uint32_t>(m_plan.param_counts()[0].back()),
                                  static_cast<uint32_t>(nbins), m_plan.config().get_tsamp(),
-                                 t_ref, static_cast<float2*>(out), s);
+                                 t_ref, static_cast<float2*>(out), s, m_dft_split);
         check_launch("Fourier brute fold launch");
         m_timer.stop(bslot, s);
         return;

@@ -107,6 +107,10 @@ public:
     /// Profile normalisation of the fused scores path: exact (default, the
     /// CPU's correctly rounded e * (1/sqrt(v))) or the hardware rsqrt.
     void set_exact_norm(bool on) noexcept { m_exact_norm = on; }
+    /// Fourier brute fold on bf16 matrix cores with split (high + low) bf16
+    /// operands and FP32 accumulation instead of FP32 matrix cores (default
+    /// off; nbins <= 129, else FP32).
+    void set_dft_split_bf16(bool on) noexcept { m_dft_split = on; }
     /// Storage precision of fold levels 0..L-2 (time domain only); the final
     /// level is always returned and scored in FP32.  Default FP32.
     void set_fold_storage(FoldStorage st);
@@ -131,6 +135,7 @@ private:
     bool m_buckets_ready{false};
     bool m_fused{true};
     bool m_exact_norm{true};
+    bool m_dft_split{false};
     FoldStorage m_storage{FoldStorage::kF32};
     int m_k0{0};  // level l of a reduced-precision tree is stored at scale 2^-(k0 + l)
     LaunchTimer m_timer;

@@ -274,6 +274,234 @@ k_bf_fourier_mfma(const float* __restrict__ ts_e, const float* __restrict__ ts_v
     }
 }
 
+// Split-bf16 variant: X = Xh + Xl and B = Bh + Bl with bf16 parts (the low
+// part is the bf16 rounding of the FP32 remainder, so each sum carries ~16
+// significant bits), and X B ~ Xh Bh + Xh Bl + Xl Bh on
+// v_mfma_f32_16x16x32_bf16 with FP32 accumulation: three bf16 products per
+// term at 16x the FP32 matrix rate.  Per product the dropped and unrepresented
+// terms are ~2^-17 relative.  Lane l of a 16x16x32 MFMA holds
+// A[l%16][8(l/16) + j] and B[8(l/16) + j][l%16] (j = 0..7) and the same D
+// layout as the FP32 kernel.  Chunks of 64 samples (two k-steps); the twiddle
+// block is kept in LDS as bf16 parts, column-major with 8 consecutive samples
+// per 16 bytes.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+constexpr int kSteps3 = 2;             // MFMA k-steps (32 samples each) per chunk
+constexpr int kKc3    = 32 * kSteps3;  // samples per chunk
+constexpr int kKs3    = kKc3 + 8;      // bf16 per LDS column (16-byte aligned, bank spread)
+constexpr uint32_t kBlockChunks3 = 128 / kKc3;  // chunks per matrix-core partial sum
+
+__device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
+    hi = static_cast<__bf16>(x);
+    lo = static_cast<__bf16>(x - static_cast<float>(hi));
+}
+
+template <int NCB>
+__global__ void __launch_bounds__(kDftBlock)
+k_bf_fourier_mfma3(const float* __restrict__ ts_e, const float* __restrict__ ts_v,
+                   float2* __restrict__ fold, const double* __restrict__ freqs, uint32_t nfreqs,
+                   uint32_t nseg, uint32_t seglen, uint32_t nbf, double tsamp, double t_ref) {
+    constexpr int MRB   = dft_mrb(NCB);
+    constexpr int NCOLS = 16 * NCB;
+    constexpr int WROWS = 16 * MRB;
+    constexpr int GROWS = WROWS * kDftWaves;
+    constexpr int TBL   = NCOLS * kKs3;  // bf16 per table
+    __shared__ __align__(16) __bf16 bt[2][2][TBL];  // [buffer][hi, lo][col][k]
+
+    const uint32_t nrows = 2 * nseg;
+    const uint32_t fi    = blockIdx.x % nfreqs;
+    const uint32_t rblk  = blockIdx.x / nfreqs;
+    const uint32_t lane  = threadIdx.x & 63U;
+    const uint32_t wave  = threadIdx.x >> 6;
+    const uint32_t g     = lane >> 4;
+    const uint32_t li    = lane & 15U;
+    const uint32_t nh    = nbf - 1;
+    const uint32_t ncols = 2 * nh;
+    const double f       = freqs[fi];
+    const bool vec_ok    = (seglen % kKc3) == 0 &&
+                        ((reinterpret_cast<uintptr_t>(ts_e) | reinterpret_cast<uintptr_t>(ts_v)) &
+                         15U) == 0;
+
+    for (int i = threadIdx.x; i < 2 * 2 * TBL; i += kDftBlock) {
+        (&bt[0][0][0])[i] = static_cast<__bf16>(0.0F);
+    }
+
+    const uint32_t row_base = rblk * GROWS + wave * WROWS;
+    const float* xrow[MRB];
+#pragma unroll
+    for (int rb = 0; rb < MRB; ++rb) {
+        const uint32_t r  = row_base + rb * 16 + li;
+        const uint32_t rr = r < nrows ? r : 0;
+        xrow[rb] = ((rr & 1U) ? ts_v : ts_e) + static_cast<uint64_t>(rr >> 1) * seglen;
+    }
+
+    // k-step st: samples k0 + 32 st + 8g .. + 7 of the lane's rows, as bf16
+    // high and low parts (index rb * kSteps3 + st)
+    auto load_x = [&](uint32_t k0, bf16x8* xh, bf16x8* xl) {
+#pragma unroll
+        for (int q = 0; q < MRB * kSteps3; ++q) {
+            const int rb     = q / kSteps3;
+            const int st     = q % kSteps3;
+            const uint32_t r = row_base + rb * 16 + li;
+            const uint32_t k = k0 + 32 * st + 8 * g;
+            float v[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                v[i] = 0.0F;
+            }
+            if (r < nrows) {
+                if (vec_ok) {
+                    const f32x4 a = *reinterpret_cast<const f32x4*>(xrow[rb] + k);
+                    const f32x4 b = *reinterpret_cast<const f32x4*>(xrow[rb] + k + 4);
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        v[i]     = a[i];
+                        v[4 + i] = b[i];
+                    }
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        v[i] = (k + i < seglen) ? xrow[rb][k + i] : 0.0F;
+                    }
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                __bf16 h, l;
+                split_bf16(v[i], h, l);
+                xh[q][i] = h;
+                xl[q][i] = l;
+            }
+        }
+    };
+
+    // Twiddles of the chunk: column 2(m-1) = cos, 2(m-1)+1 = -sin of harmonic m.
+    // Per sample, harmonics come in groups of kGrp: the first of a group is the
+    // accurate twiddle of its FP64-reduced angle, the others follow by FP32
+    // complex products with the harmonic-1 twiddle (a few ulp after kGrp - 1
+    // steps, far below the bf16 split's 2^-17).
+    constexpr uint32_t kGrp = 8;
+    auto gen_b = [&](uint32_t k0, __bf16* th, __bf16* tl) {
+        const uint32_t ngrp  = (nh + kGrp - 1) / kGrp;
+        const uint32_t items = kKc3 * ngrp;
+        for (uint32_t p = threadIdx.x; p < items; p += kDftBlock) {
+            const uint32_t kk = p / ngrp;
+            const uint32_t m0 = (p % ngrp) * kGrp + 1;
+            const uint32_t m1 = m0 + kGrp <= nh + 1 ? m0 + kGrp : nh + 1;
+            const double ph   = f * ((static_cast<double>(k0 + kk) * tsamp) - t_ref);
+            const double fr   = ph - floor(ph);
+            const float2 w1   = twiddle_turns(fr);
+            float2 w          = twiddle_turns(fr * static_cast<double>(m0));
+            for (uint32_t m = m0; m < m1; ++m) {
+                const uint32_t c0 = (2 * (m - 1)) * kKs3 + kk;
+                split_bf16(w.x, th[c0], tl[c0]);
+                split_bf16(w.y, th[c0 + kKs3], tl[c0 + kKs3]);
+                w = make_float2(fmaf(w.x, w1.x, -(w.y * w1.y)), fmaf(w.x, w1.y, w.y * w1.x));
+            }
+        }
+    };
+
+    f32x4 acc[MRB][NCB], tot[MRB][NCB];
+#pragma unroll
+    for (int rb = 0; rb < MRB; ++rb) {
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb) {
+            acc[rb][cb] = f32x4{0.0F, 0.0F, 0.0F, 0.0F};
+            tot[rb][cb] = f32x4{0.0F, 0.0F, 0.0F, 0.0F};
+        }
+    }
+
+    const uint32_t nchunks = (seglen + kKc3 - 1) / kKc3;
+    __syncthreads();  // zero-fill of the tables done
+    gen_b(0, bt[0][0], bt[0][1]);
+    bf16x8 xh[MRB * kSteps3], xl[MRB * kSteps3];
+    load_x(0, xh, xl);
+    __syncthreads();
+    for (uint32_t ch = 0; ch < nchunks; ++ch) {
+        const __bf16* th = bt[ch & 1U][0];
+        const __bf16* tl = bt[ch & 1U][1];
+        bf16x8 nxh[MRB * kSteps3], nxl[MRB * kSteps3];
+        const bool more = ch + 1 < nchunks;
+        if (more) {
+            load_x((ch + 1) * kKc3, nxh, nxl);
+            gen_b((ch + 1) * kKc3, bt[(ch + 1) & 1U][0], bt[(ch + 1) & 1U][1]);
+        }
+#pragma unroll
+        for (int st = 0; st < kSteps3; ++st) {
+            bf16x8 bh[NCB], bl[NCB];
+#pragma unroll
+            for (int cb = 0; cb < NCB; ++cb) {
+                const uint32_t o = (cb * 16 + li) * kKs3 + 32 * st + 8 * g;
+                bh[cb] = *reinterpret_cast<const bf16x8*>(th + o);
+                bl[cb] = *reinterpret_cast<const bf16x8*>(tl + o);
+            }
+#pragma unroll
+            for (int rb = 0; rb < MRB; ++rb) {
+                const int q = rb * kSteps3 + st;
+#pragma unroll
+                for (int cb = 0; cb < NCB; ++cb) {
+                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl[q], bh[cb],
+                                                                          acc[rb][cb], 0, 0, 0);
+                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[q], bl[cb],
+                                                                          acc[rb][cb], 0, 0, 0);
+                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[q], bh[cb],
+                                                                          acc[rb][cb], 0, 0, 0);
+                }
+            }
+        }
+        if (more) {
+#pragma unroll
+            for (int q = 0; q < MRB * kSteps3; ++q) {
+                xh[q] = nxh[q];
+                xl[q] = nxl[q];
+            }
+        }
+        if ((ch + 1) % kBlockChunks3 == 0 || !more) {
+#pragma unroll
+            for (int rb = 0; rb < MRB; ++rb) {
+#pragma unroll
+                for (int cb = 0; cb < NCB; ++cb) {
+                    tot[rb][cb] += acc[rb][cb];
+                    acc[rb][cb] = f32x4{0.0F, 0.0F, 0.0F, 0.0F};
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int rb = 0; rb < MRB; ++rb) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const uint32_t r = row_base + rb * 16 + 4 * g + v;
+            if (r >= nrows) {
+                continue;
+            }
+            const uint64_t s = r >> 1;
+            const uint32_t q = r & 1U;
+            float* out = reinterpret_cast<float*>(fold + ((s * nfreqs + fi) * 2 + q) * nbf) + 2;
+#pragma unroll
+            for (int cb = 0; cb < NCB; ++cb) {
+                const uint32_t n = cb * 16 + li;
+                if (n < ncols) {
+                    out[n] = tot[rb][cb][v];
+                }
+            }
+        }
+    }
+}
+
+template <int NCB>
+void launch_bf_mfma3(const float* ts_e, const float* ts_v, float2* fold, const double* freqs,
+                     uint32_t nfreqs, uint32_t nseg, uint32_t seglen, uint32_t nbf, double tsamp,
+                     double t_ref, hipStream_t s) {
+    constexpr uint32_t GROWS = 16 * dft_mrb(NCB) * kDftWaves;
+    const uint64_t nrb = (2ULL * nseg + GROWS - 1) / GROWS;
+    const uint64_t nb  = nrb * nfreqs;
+    LOKI_REQUIRE(nb < (1ULL << 31), "Fourier brute fold grid too large");
+    k_bf_fourier_mfma3<NCB><<<static_cast<uint32_t>(nb), kDftBlock, 0, s>>>(
+        ts_e, ts_v, fold, freqs, nfreqs, nseg, seglen, nbf, tsamp, t_ref);
+}
+
 template <int NCB>
 void launch_bf_mfma(const float* ts_e, const float* ts_v, float2* fold, const double* freqs,
                     uint32_t nfreqs, uint32_t nseg, uint32_t seglen, uint32_t nbf, double tsamp,
@@ -432,7 +660,7 @@ unsigned grid_for(uint64_t total) {
 
 void brute_fold_fourier(const float* ts_e, const float* ts_v, uint64_t nseg, uint32_t seglen,
                         const double* freqs_dev, uint32_t nfreqs, uint32_t nbins, double tsamp,
-                        double t_ref, float2* fold, hipStream_t s) {
+                        double t_ref, float2* fold, hipStream_t s, bool split_bf16) {
     LOKI_REQUIRE(nbins >= 2 && nbins <= 256, "Fourier brute fold: nbins must be in [2, 256]");
     LOKI_REQUIRE(seglen >= 1 && nfreqs >= 1, "Fourier brute fold: empty input");
     LOKI_REQUIRE(nseg >= 1 && 2 * nseg < (1ULL << 31), "Fourier brute fold: too many segments");
@@ -441,6 +669,22 @@ void brute_fold_fourier(const float* ts_e, const float* ts_v, uint64_t nseg, uin
                                                                    nfreqs, nbf);
     const uint32_t ncols = 2 * (nbf - 1);
     const auto ns        = static_cast<uint32_t>(nseg);
+    if (split_bf16 && ncols <= 128) {
+        if (ncols <= 16) {
+            launch_bf_mfma3<1>(ts_e, ts_v, fold, freqs_dev, nfreqs, ns, seglen, nbf, tsamp, t_ref,
+                               s);
+        } else if (ncols <= 32) {
+            launch_bf_mfma3<2>(ts_e, ts_v, fold, freqs_dev, nfreqs, ns, seglen, nbf, tsamp, t_ref,
+                               s);
+        } else if (ncols <= 64) {
+            launch_bf_mfma3<4>(ts_e, ts_v, fold, freqs_dev, nfreqs, ns, seglen, nbf, tsamp, t_ref,
+                               s);
+        } else {
+            launch_bf_mfma3<8>(ts_e, ts_v, fold, freqs_dev, nfreqs, ns, seglen, nbf, tsamp, t_ref,
+                               s);
+        }
+        return;
+    }
     if (ncols <= 16) {
         launch_bf_mfma<1>(ts_e, ts_v, fold, freqs_dev, nfreqs, ns, seglen, nbf, tsamp, t_ref, s);
     } else if (ncols <= 32) {

@@ -78,9 +78,12 @@ void merge_time(const void* in, void* out, const FFACoord* coords,
 /// Fourier brute fold: fold [nseg, nfreqs, 2, nbins/2+1] complex (float2) of
 /// exp(-2 pi i m f (k tsamp - t_ref)) sums; FP32 matrix-core DFT with FP64
 /// phases, harmonic 0 by FP64 reduction.  nbins in [2, 256].
+/// split_bf16: harmonics on bf16 matrix cores with each operand split into
+/// high and low bf16 parts (three products per term, FP32 accumulation) for
+/// nbins <= 129; otherwise FP32 matrix cores.
 void brute_fold_fourier(const float* ts_e, const float* ts_v, uint64_t nseg, uint32_t seglen,
                         const double* freqs_dev, uint32_t nfreqs, uint32_t nbins, double tsamp,
-                        double t_ref, float2* fold, hipStream_t s);
+                        double t_ref, float2* fold, hipStream_t s, bool split_bf16 = false);
 
 /// Fourier-domain merge of one level (freq-only when coords_freq != nullptr):
 /// phase rotation by the fractional shifts, rows of [2, nbins/2+1] float2.

@@ -316,6 +316,12 @@ public:
     void set_timing(bool on) { check(loki_ffa_set_timing(m_h.get(), on ? 1 : 0)); }
     void set_fused(bool on) { check(loki_ffa_set_fused(m_h.get(), on ? 1 : 0)); }
     void set_exact_norm(bool on) { check(loki_ffa_set_exact_norm(m_h.get(), on ? 1 : 0)); }
+    void set_dft_precision(const std::string& p) {
+        if (p != "f32" && p != "bf16x3") {
+            throw std::invalid_argument("dft precision must be 'f32' or 'bf16x3'");
+        }
+        check(loki_ffa_set_dft_precision(m_h.get(), p == "bf16x3" ? 1 : 0));
+    }
     void set_fold_storage(const std::string& st) {
         int k = -1;
         if (st == "f32" || st == "float32") {
@@ -1036,6 +1042,7 @@ PYBIND11_MODULE(_loki_amd, m) {
         .def("set_timing", &FFA::set_timing, py::arg("enable"))
         .def("set_fused", &FFA::set_fused, py::arg("enable"))
         .def("set_exact_norm", &FFA::set_exact_norm, py::arg("exact"))
+        .def("set_dft_precision", &FFA::set_dft_precision, py::arg("precision"))
         .def("scores_filter", &FFA::scores_filter_host, py::arg("ts_e"), py::arg("ts_v"),
              py::arg("widths"), py::arg("threshold"), py::arg("find_max") = false)
         .def("scores_filter_device", &FFA::scores_filter_device, py::arg("ts_e_ptr"),

@@ -229,3 +229,29 @@ def test_fourier_errors():
                                                        TSAMP)
     with pytest.raises(RuntimeError, match="nsamps"):
         libculoki.ffa.compute_ffa_fourier_cuda(ts_e[:100], ts_v[:100], cfg)
+
+
+@pytest.mark.parametrize("name", ["freq16", "acc17w", "acc20", "jerk17w", "freq18_nb100"])
+def test_split_bf16_dft_within_bound(oracle, name):
+    """The split-bf16 matrix-core brute fold (set_dft_precision("bf16x3"): each operand as a
+    high and a low bf16 part, three products per term, FP32 accumulation; ~2^-17 relative per
+    product) keeps the final Fourier fold within the FP32 path's bound, 1e-5 of the row norm,
+    and the S/N within 1e-4 + 1e-5 |S/N|."""
+    from loki_amd import libculoki
+
+    cfg, ocfg = _cfgs(name, use_fourier=True)
+    ts_e, ts_v = _series(cfg.nsamps, seed=7)
+    ffa = libculoki.ffa.FFACOMPLEXCUDA(cfg, device_id=0)
+    ffa.set_dft_precision("bf16x3")
+    fold_c = np.zeros(ffa.plan.buffer_size, dtype=np.complex64)
+    ffa.execute(ts_e, ts_v, fold_c)
+    want, _ = oracle.compute_ffa(ts_e, ts_v, ocfg, fourier=True)
+    _row_check(fold_c[: ffa.plan.fold_size], want, cfg.nbins // 2 + 1, 1e-5, f"bf16x3 {name}")
+    widths = np.array([w for w in (1, 2, 3, 5, 8, 13) if w < cfg.nbins], dtype=np.uint64)
+    got = ffa.scores(ts_e, ts_v, widths)
+    want_s, _ = oracle.compute_ffa_scores(ts_e, ts_v, ocfg, widths=widths)
+    d = np.abs(got.astype(np.float64) - want_s)
+    print(f"bf16x3 scores {name}: max |dS/N| = {d.max():.3e}")
+    assert np.all(d <= 1e-4 + 1e-5 * np.abs(want_s))
+    with pytest.raises(ValueError):
+        ffa.set_dft_precision("f16")
