@@ -289,6 +289,8 @@ constexpr int kSteps3 = 2;             // MFMA k-steps (32 samples each) per chu
 constexpr int kKc3    = 32 * kSteps3;  // samples per chunk
 constexpr int kKs3    = kKc3 + 8;      // bf16 per LDS column (16-byte aligned, bank spread)
 constexpr uint32_t kBlockChunks3 = 128 / kKc3;  // chunks per matrix-core partial sum
+constexpr int kDft3Block = 256;  // 4 waves: two workgroups per CU overlap their barriers
+constexpr int kDft3Waves = kDft3Block / 64;
 
 __device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
     hi = static_cast<__bf16>(x);
@@ -296,14 +298,14 @@ __device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
 }
 
 template <int NCB>
-__global__ void __launch_bounds__(kDftBlock)
+__global__ void __launch_bounds__(kDft3Block)
 k_bf_fourier_mfma3(const float* __restrict__ ts_e, const float* __restrict__ ts_v,
                    float2* __restrict__ fold, const double* __restrict__ freqs, uint32_t nfreqs,
                    uint32_t nseg, uint32_t seglen, uint32_t nbf, double tsamp, double t_ref) {
     constexpr int MRB   = dft_mrb(NCB);
     constexpr int NCOLS = 16 * NCB;
     constexpr int WROWS = 16 * MRB;
-    constexpr int GROWS = WROWS * kDftWaves;
+    constexpr int GROWS = WROWS * kDft3Waves;
     constexpr int TBL   = NCOLS * kKs3;  // bf16 per table
     __shared__ __align__(16) __bf16 bt[2][2][TBL];  // [buffer][hi, lo][col][k]
 
@@ -321,7 +323,7 @@ k_bf_fourier_mfma3(const float* __restrict__ ts_e, const float* __restrict__ ts_
                         ((reinterpret_cast<uintptr_t>(ts_e) | reinterpret_cast<uintptr_t>(ts_v)) &
                          15U) == 0;
 
-    for (int i = threadIdx.x; i < 2 * 2 * TBL; i += kDftBlock) {
+    for (int i = threadIdx.x; i < 2 * 2 * TBL; i += kDft3Block) {
         (&bt[0][0][0])[i] = static_cast<__bf16>(0.0F);
     }
 
@@ -383,7 +385,7 @@ k_bf_fourier_mfma3(const float* __restrict__ ts_e, const float* __restrict__ ts_
     auto gen_b = [&](uint32_t k0, __bf16* th, __bf16* tl) {
         const uint32_t ngrp  = (nh + kGrp - 1) / kGrp;
         const uint32_t items = kKc3 * ngrp;
-        for (uint32_t p = threadIdx.x; p < items; p += kDftBlock) {
+        for (uint32_t p = threadIdx.x; p < items; p += kDft3Block) {
             const uint32_t kk = p / ngrp;
             const uint32_t m0 = (p % ngrp) * kGrp + 1;
             const uint32_t m1 = m0 + kGrp <= nh + 1 ? m0 + kGrp : nh + 1;
@@ -494,11 +496,11 @@ template <int NCB>
 void launch_bf_mfma3(const float* ts_e, const float* ts_v, float2* fold, const double* freqs,
                      uint32_t nfreqs, uint32_t nseg, uint32_t seglen, uint32_t nbf, double tsamp,
                      double t_ref, hipStream_t s) {
-    constexpr uint32_t GROWS = 16 * dft_mrb(NCB) * kDftWaves;
+    constexpr uint32_t GROWS = 16 * dft_mrb(NCB) * kDft3Waves;
     const uint64_t nrb = (2ULL * nseg + GROWS - 1) / GROWS;
     const uint64_t nb  = nrb * nfreqs;
     LOKI_REQUIRE(nb < (1ULL << 31), "Fourier brute fold grid too large");
-    k_bf_fourier_mfma3<NCB><<<static_cast<uint32_t>(nb), kDftBlock, 0, s>>>(
+    k_bf_fourier_mfma3<NCB><<<static_cast<uint32_t>(nb), kDft3Block, 0, s>>>(
         ts_e, ts_v, fold, freqs, nfreqs, nseg, seglen, nbf, tsamp, t_ref);
 }
 
