@@ -297,6 +297,17 @@ __device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
     lo = static_cast<__bf16>(x - static_cast<float>(hi));
 }
 
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+/// Two values at once (packed conversions, v_cvt_pk_bf16_f32): high parts and
+/// the bf16 roundings of the exact FP32 remainders.
+__device__ __forceinline__ void split_bf16x2(f32x2 x, bf16x2& hi, bf16x2& lo) {
+    hi = __builtin_convertvector(x, bf16x2);
+    const f32x2 back = __builtin_convertvector(hi, f32x2);
+    lo = __builtin_convertvector(x - back, bf16x2);
+}
+
 template <int NCB>
 __global__ void __launch_bounds__(kDft3Block)
 k_bf_fourier_mfma3(const float* __restrict__ ts_e, const float* __restrict__ ts_v,
@@ -366,13 +377,17 @@ k_bf_fourier_mfma3(const float* __restrict__ ts_e, const float* __restrict__ ts_
                     }
                 }
             }
+            bf16x2 h[4], l[4];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                __bf16 h, l;
-                split_bf16(v[i], h, l);
-                xh[q][i] = h;
-                xl[q][i] = l;
+            for (int i = 0; i < 4; ++i) {
+                split_bf16x2(f32x2{v[2 * i], v[2 * i + 1]}, h[i], l[i]);
             }
+            xh[q] = __builtin_shufflevector(__builtin_shufflevector(h[0], h[1], 0, 1, 2, 3),
+                                            __builtin_shufflevector(h[2], h[3], 0, 1, 2, 3),
+                                            0, 1, 2, 3, 4, 5, 6, 7);
+            xl[q] = __builtin_shufflevector(__builtin_shufflevector(l[0], l[1], 0, 1, 2, 3),
+                                            __builtin_shufflevector(l[2], l[3], 0, 1, 2, 3),
+                                            0, 1, 2, 3, 4, 5, 6, 7);
         }
     };
 
@@ -386,8 +401,10 @@ k_bf_fourier_mfma3(const float* __restrict__ ts_e, const float* __restrict__ ts_
         const uint32_t ngrp  = (nh + kGrp - 1) / kGrp;
         const uint32_t items = kKc3 * ngrp;
         for (uint32_t p = threadIdx.x; p < items; p += kDft3Block) {
-            const uint32_t kk = p / ngrp;
-            const uint32_t m0 = (p % ngrp) * kGrp + 1;
+            // consecutive lanes take consecutive samples: their 2-byte writes fill
+            // whole LDS words of one column (no bank conflicts)
+            const uint32_t kk = p % kKc3;
+            const uint32_t m0 = (p / kKc3) * kGrp + 1;
             const uint32_t m1 = m0 + kGrp <= nh + 1 ? m0 + kGrp : nh + 1;
             const double ph   = f * ((static_cast<double>(k0 + kk) * tsamp) - t_ref);
             const double fr   = ph - floor(ph);
@@ -395,8 +412,12 @@ k_bf_fourier_mfma3(const float* __restrict__ ts_e, const float* __restrict__ ts_
             float2 w          = twiddle_turns(fr * static_cast<double>(m0));
             for (uint32_t m = m0; m < m1; ++m) {
                 const uint32_t c0 = (2 * (m - 1)) * kKs3 + kk;
-                split_bf16(w.x, th[c0], tl[c0]);
-                split_bf16(w.y, th[c0 + kKs3], tl[c0 + kKs3]);
+                bf16x2 h, l;
+                split_bf16x2(f32x2{w.x, w.y}, h, l);
+                th[c0]        = h[0];
+                tl[c0]        = l[0];
+                th[c0 + kKs3] = h[1];
+                tl[c0 + kKs3] = l[1];
                 w = make_float2(fmaf(w.x, w1.x, -(w.y * w1.y)), fmaf(w.x, w1.y, w.y * w1.x));
             }
         }
