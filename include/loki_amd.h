@@ -156,6 +156,18 @@ int loki_plan_resolve_coordinates_host(loki_plan_t plan, void* coords, size_t co
 
 /* --- FFA executor (ffa.hpp:100-188 FFACUDA<T>) ------------------------- */
 int loki_ffa_create(loki_config_t cfg, int fold_type, int device_id, loki_ffa_t* out);
+/* workspace.hpp:169-206 FFAWorkspaceCUDA and the FFACUDA(workspace, cfg, device_id) constructor
+ * (ffa.hpp:100-153): FFAs made with one workspace share its device buffers (the ping-pong
+ * partner fold, the second scores buffer and the shift tables), each growing them to its plan,
+ * so many plans (the chunks of a frequency sweep) hold one set of buffers sized by the largest.
+ * They must not run concurrently.  buffer_bytes / coord_bytes (0: none) preallocate.  The
+ * workspace lives until it and every FFA made with it are destroyed. */
+typedef struct loki_ffa_workspace_s* loki_ffa_workspace_t;
+int loki_ffa_workspace_create(int device_id, size_t buffer_bytes, size_t coord_bytes,
+                              loki_ffa_workspace_t* out);
+void loki_ffa_workspace_destroy(loki_ffa_workspace_t ws);
+int loki_ffa_create_with_workspace(loki_config_t cfg, int fold_type, loki_ffa_workspace_t ws,
+                                   int device_id, loki_ffa_t* out);
 void loki_ffa_destroy(loki_ffa_t ffa);
 int loki_ffa_plan(loki_ffa_t ffa, loki_plan_t* out_plan_view); /* borrowed; do not destroy */
 /* FFACUDA::execute(device spans, stream) (ffa.hpp:130-136): fold holds

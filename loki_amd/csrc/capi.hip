@@ -27,6 +27,10 @@ struct loki_ffa_s {
     loki_amd::FFAExecutor exec;
     loki_plan_s plan_view;
 };
+struct loki_ffa_workspace_s {
+    std::shared_ptr<loki_amd::FFAWorkspace> ws;
+    int device;
+};
 struct loki_region_planner_s {
     loki_amd::FFARegionPlanner planner;
 };
@@ -328,6 +332,37 @@ int loki_ffa_create(loki_config_t cfg, int fold_type, int device_id, loki_ffa_t*
 }
 
 void loki_ffa_destroy(loki_ffa_t ffa) { delete ffa; }
+
+int loki_ffa_workspace_create(int device_id, size_t buffer_bytes, size_t coord_bytes,
+                              loki_ffa_workspace_t* out) {
+    return guarded([&] {
+        REQ(out != nullptr, "null argument");
+        loki_amd::DeviceGuard g(device_id);
+        auto ws = std::make_shared<loki_amd::FFAWorkspace>();
+        if (buffer_bytes > 0) {
+            ws->internal.reserve(buffer_bytes);
+            ws->result.reserve(buffer_bytes);
+        }
+        if (coord_bytes > 0) {
+            ws->coords.reserve(coord_bytes);
+        }
+        *out = new loki_ffa_workspace_s{std::move(ws), device_id};
+    });
+}
+
+void loki_ffa_workspace_destroy(loki_ffa_workspace_t ws) { delete ws; }
+
+int loki_ffa_create_with_workspace(loki_config_t cfg, int fold_type, loki_ffa_workspace_t ws,
+                                   int device_id, loki_ffa_t* out) {
+    return guarded([&] {
+        REQ(cfg != nullptr && ws != nullptr && out != nullptr, "null argument");
+        REQ(ws->device == device_id, "workspace belongs to another device");
+        auto kind = fold_kind(fold_type);
+        loki_amd::FFAExecutor exec(cfg->cfg, kind, device_id, ws->ws);
+        loki_amd::FFAPlan plan(cfg->cfg, kind);
+        *out = new loki_ffa_s{std::move(exec), loki_plan_s{std::move(plan)}};
+    });
+}
 
 int loki_ffa_plan(loki_ffa_t ffa, loki_plan_t* out) {
     return guarded([&] {

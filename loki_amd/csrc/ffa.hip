@@ -158,12 +158,14 @@ void check_launch(const char* what) { hip_check(hipGetLastError(), what); }
 
 }  // namespace
 
-FFAExecutor::FFAExecutor(const SearchConfig& cfg, FoldKind kind, int device_id)
+FFAExecutor::FFAExecutor(const SearchConfig& cfg, FoldKind kind, int device_id,
+                         std::shared_ptr<FFAWorkspace> ws)
     : m_plan(cfg, kind),
       m_device(device_id),
       m_freq_only(cfg.get_nparams() == 1),
       m_fourier(kind == FoldKind::kFourier),
-      m_lossy(kind == FoldKind::kFourier && cfg.get_nbins() > cfg.get_nbins_min_lossy_bf()) {
+      m_lossy(kind == FoldKind::kFourier && cfg.get_nbins() > cfg.get_nbins_min_lossy_bf()),
+      m_ws(ws ? std::move(ws) : std::make_shared<FFAWorkspace>()) {
     DeviceGuard g(device_id);
     LOKI_REQUIRE(cfg.get_nbins() <= 1024, "nbins must be <= 1024");
     LOKI_REQUIRE(!m_fourier || m_lossy || (cfg.get_nbins() >= 2 && cfg.get_nbins() <= 256),
@@ -277,9 +279,9 @@ void* FFAExecutor::run_levels(const float* ts_e, const float* ts_v, void* buf_ev
     // lands in the caller's buffer, ffa.cpp:261-276).
     LOKI_REQUIRE(last_level < m_plan.n_levels(), "level out of range");
     m_timer.reset();
-    m_coords.reserve(m_plan.coord_size() * coord_record_bytes());
+    m_ws->coords.reserve(m_plan.coord_size() * coord_record_bytes());
     prepare_brute_fold(s);
-    resolve_coordinates(m_coords.get(), s, last_level);
+    resolve_coordinates(m_ws->coords.get(), s, last_level);
 
     void* even           = buf_even;
     void* odd            = buf_odd;
@@ -291,9 +293,9 @@ void* FFAExecutor::run_levels(const float* ts_e, const float* ts_v, void* buf_ev
         const FFACoord* cp = nullptr;
         const FFACoordFreq* cfp = nullptr;
         if (m_freq_only) {
-            cfp = m_coords.as<FFACoordFreq>() + off;
+            cfp = m_ws->coords.as<FFACoordFreq>() + off;
         } else {
-            cp = m_coords.as<FFACoord>() + off;
+            cp = m_ws->coords.as<FFACoord>() + off;
         }
         void* in       = (l % 2 == 1) ? even : odd;
         void* out      = (l % 2 == 1) ? odd : even;
@@ -335,11 +337,11 @@ void FFAExecutor::execute(const float* ts_e, const float* ts_v, void* fold, Size
     LOKI_REQUIRE(fold_len == m_plan.buffer_size(),
                  "FFA::execute: fold must have size buffer_size");
     const SizeType L = m_plan.n_levels();
-    m_internal.reserve(m_plan.buffer_size() * elem_bytes());
+    m_ws->internal.reserve(m_plan.buffer_size() * elem_bytes());
     if ((L - 1) % 2 == 0) {
-        run_levels(ts_e, ts_v, fold, m_internal.get(), L - 1, true, s);
+        run_levels(ts_e, ts_v, fold, m_ws->internal.get(), L - 1, true, s);
     } else {
-        run_levels(ts_e, ts_v, m_internal.get(), fold, L - 1, true, s);
+        run_levels(ts_e, ts_v, m_ws->internal.get(), fold, L - 1, true, s);
     }
 }
 
@@ -355,10 +357,10 @@ void FFAExecutor::execute_time(const float* ts_e, const float* ts_v, float* fold
     // The caller's buffer (buffer_size complex elements) is the ping-pong
     // partner; the last level lands in the internal buffer and is returned to
     // the time domain into the caller's buffer (ffa_cuda.cu:248-290).
-    m_internal.reserve(m_plan.buffer_size() * elem_bytes());
+    m_ws->internal.reserve(m_plan.buffer_size() * elem_bytes());
     const void* fin = ((L - 1) % 2 == 0)
-                          ? run_levels(ts_e, ts_v, m_internal.get(), fold, L - 1, true, s)
-                          : run_levels(ts_e, ts_v, fold, m_internal.get(), L - 1, true, s);
+                          ? run_levels(ts_e, ts_v, m_ws->internal.get(), fold, L - 1, true, s)
+                          : run_levels(ts_e, ts_v, fold, m_ws->internal.get(), L - 1, true, s);
     const SizeType nbins = m_plan.config().get_nbins();
     const int slot       = m_timer.start(KernelKind::kTransform, static_cast<int>(L - 1), s);
     kern::irfft_rows(static_cast<const float2*>(fin), fold, m_plan.fold_size_time() / nbins,
@@ -400,10 +402,10 @@ void FFAExecutor::execute_scores(const float* ts_e, const float* ts_v,
         // Levels 0..L-2 through HBM, the last level fused with its shift
         // tables and the S/N: the final fold cube is never stored.
         const SizeType need = max_level_size(L - 2) * storage_bytes(m_storage);
-        m_internal.reserve(need);
-        m_result.reserve(need);
+        m_ws->internal.reserve(need);
+        m_ws->result.reserve(need);
         const void* parent =
-            run_levels(ts_e, ts_v, m_result.get(), m_internal.get(), L - 2, false, s);
+            run_levels(ts_e, ts_v, m_ws->result.get(), m_ws->internal.get(), L - 2, false, s);
         const auto lv  = resolve_level(m_plan, L - 1);
         const int slot = m_timer.start(KernelKind::kMergeScore, static_cast<int>(L - 1), s);
         kern::final_merge_score(parent, m_storage, std::ldexp(1.0F, m_k0 + static_cast<int>(L) - 2),
@@ -413,10 +415,10 @@ void FFAExecutor::execute_scores(const float* ts_e, const float* ts_v,
         m_timer.stop(slot, s);
         return;
     }
-    m_internal.reserve(m_plan.buffer_size() * elem_bytes());
-    m_result.reserve(m_plan.buffer_size() * elem_bytes());
+    m_ws->internal.reserve(m_plan.buffer_size() * elem_bytes());
+    m_ws->result.reserve(m_plan.buffer_size() * elem_bytes());
     const void* last =
-        run_levels(ts_e, ts_v, m_result.get(), m_internal.get(), L - 1, true, s);
+        run_levels(ts_e, ts_v, m_ws->result.get(), m_ws->internal.get(), L - 1, true, s);
     const float* fin = static_cast<const float*>(last);
     if (m_fourier && m_fused && kern::irfft_score_supported(static_cast<uint32_t>(nbins))) {
         const int slot =
@@ -430,7 +432,7 @@ void FFAExecutor::execute_scores(const float* ts_e, const float* ts_v,
     if (m_fourier) {
         // Back to the time domain into the other ping-pong buffer: it holds
         // buffer_size complex = 2*buffer_size floats >= fold_size_time.
-        void* other     = last == m_result.get() ? m_internal.get() : m_result.get();
+        void* other     = last == m_ws->result.get() ? m_ws->internal.get() : m_ws->result.get();
         auto* tf        = static_cast<float*>(other);
         const int tslot = m_timer.start(KernelKind::kTransform, static_cast<int>(L - 1), s);
         kern::irfft_rows(static_cast<const float2*>(last), tf, m_plan.fold_size_time() / nbins,

@@ -63,9 +63,21 @@ private:
     size_t m_used{0};
 };
 
+/// Device buffers an FFA borrows (workspace.hpp:169-206 FFAWorkspaceCUDA): the
+/// ping-pong partner, the second scores buffer and the shift tables.  FFAs
+/// that share one run one at a time; each grows the buffers to its plan.
+struct FFAWorkspace {
+    DeviceBuffer internal;
+    DeviceBuffer result;
+    DeviceBuffer coords;
+};
+
 class FFAExecutor {
 public:
-    FFAExecutor(const SearchConfig& cfg, FoldKind kind, int device_id);
+    /// `ws`: shared workspace (ffa.hpp:100-153, the FFACUDA(workspace, cfg,
+    /// device_id) constructor); null gives the FFA its own.
+    FFAExecutor(const SearchConfig& cfg, FoldKind kind, int device_id,
+                std::shared_ptr<FFAWorkspace> ws = nullptr);
 
     void set_timing(bool on) noexcept { m_timer.set_enabled(on); }
     /// Per-kernel durations of the last execute()/execute_scores() call.
@@ -118,7 +130,7 @@ public:
     bool fused_scores_supported() const;
 
     /// Device pointer to the last shift tables resolved by execute().
-    const void* coords_device() const noexcept { return m_coords.get(); }
+    const void* coords_device() const noexcept { return m_ws->coords.get(); }
     SizeType coord_record_bytes() const noexcept;
 
 private:
@@ -127,9 +139,7 @@ private:
     bool m_freq_only;
     bool m_fourier;            // complex folds (float2 elements)
     bool m_lossy;              // Fourier init = time brute fold + rfft (ffa_cuda.cu:342-353)
-    DeviceBuffer m_internal;   // ping-pong partner (buffer_size elements)
-    DeviceBuffer m_result;     // second buffer for execute_scores
-    DeviceBuffer m_coords;     // all levels' shift tables
+    std::shared_ptr<FFAWorkspace> m_ws;  // ping-pong partner, scores buffer, shift tables
     DeviceBuffer m_bf_freqs;   // level-0 frequencies (FP64)
     kern::BruteFoldLists m_bf; // transposed bucket lists of the brute fold
     bool m_buckets_ready{false};

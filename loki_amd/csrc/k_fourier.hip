@@ -292,11 +292,6 @@ constexpr uint32_t kBlockChunks3 = 128 / kKc3;  // chunks per matrix-core partia
 constexpr int kDft3Block = 256;  // 4 waves: two workgroups per CU overlap their barriers
 constexpr int kDft3Waves = kDft3Block / 64;
 
-__device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
-    hi = static_cast<__bf16>(x);
-    lo = static_cast<__bf16>(x - static_cast<float>(hi));
-}
-
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 

@@ -243,12 +243,33 @@ struct PlanFourier : Plan {
     explicit PlanFourier(const Config& c) : Plan(c, LOKI_FOLD_FOURIER) {}
 };
 
+/// workspace.hpp:169-206 FFAWorkspaceCUDA: device buffers FFAs can share.
+class Workspace {
+public:
+    Workspace(int device_id, size_t buffer_bytes, size_t coord_bytes) : m_device(device_id) {
+        loki_ffa_workspace_t h = nullptr;
+        check(loki_ffa_workspace_create(device_id, buffer_bytes, coord_bytes, &h));
+        m_h.reset(h, loki_ffa_workspace_destroy);
+    }
+    loki_ffa_workspace_t handle() const { return m_h.get(); }
+    int device() const { return m_device; }
+
+private:
+    int m_device;
+    std::shared_ptr<loki_ffa_workspace_s> m_h;
+};
+
 class FFA {
 public:
-    FFA(const Config& cfg, int device_id, int fold_type)
+    FFA(const Config& cfg, int device_id, int fold_type, const Workspace* ws = nullptr)
         : m_cfg(cfg), m_device(device_id), m_fold_type(fold_type), m_plan(cfg, fold_type) {
         loki_ffa_t h = nullptr;
-        check(loki_ffa_create(cfg.handle(), fold_type, device_id, &h));
+        if (ws != nullptr) {
+            check(loki_ffa_create_with_workspace(cfg.handle(), fold_type, ws->handle(),
+                                                 device_id, &h));
+        } else {
+            check(loki_ffa_create(cfg.handle(), fold_type, device_id, &h));
+        }
         m_h.reset(h, loki_ffa_destroy);
     }
     const Plan& plan() const { return m_plan; }
@@ -405,7 +426,8 @@ private:
 /// Distinct C++ type for the complex-fold class (pybind11 binds one Python
 /// class per C++ type).
 struct FFAComplex : FFA {
-    FFAComplex(const Config& cfg, int device_id) : FFA(cfg, device_id, LOKI_FOLD_FOURIER) {}
+    FFAComplex(const Config& cfg, int device_id, const Workspace* ws = nullptr)
+        : FFA(cfg, device_id, LOKI_FOLD_FOURIER, ws) {}
 };
 
 /// libculoki.ffa.FFABatchCUDA: many frequency-only searches advanced together.
@@ -999,11 +1021,15 @@ PYBIND11_MODULE(_loki_amd, m) {
 
     // -------------------------------------------------------------------- ffa
     auto m_ffa = m.def_submodule("ffa", "FFA submodule");
+    py::class_<Workspace>(m_ffa, "FFAWorkspaceCUDA")
+        .def(py::init<int, size_t, size_t>(), py::arg("device_id") = 0,
+             py::arg("buffer_bytes") = 0, py::arg("coord_bytes") = 0);
     py::class_<FFA>(m_ffa, "FFACUDA")
-        .def(py::init([](const Config& cfg, int device_id) {
-                 return std::make_unique<FFA>(cfg, device_id, LOKI_FOLD_TIME);
+        .def(py::init([](const Config& cfg, int device_id, const Workspace* ws) {
+                 return std::make_unique<FFA>(cfg, device_id, LOKI_FOLD_TIME, ws);
              }),
-             py::arg("cfg"), py::arg("device_id") = 0)
+             py::arg("cfg"), py::arg("device_id") = 0, py::arg("workspace") = nullptr,
+             py::keep_alive<1, 4>())
         .def_property_readonly("plan", &FFA::plan)
         .def("execute", &FFA::execute, py::arg("ts_e"), py::arg("ts_v"), py::arg("fold"))
         .def("execute_device", &FFA::execute_device, py::arg("ts_e_ptr"), py::arg("ts_v_ptr"),
@@ -1025,7 +1051,8 @@ PYBIND11_MODULE(_loki_amd, m) {
              py::arg("count_ptr"), py::arg("stream") = 0)
         .def("timings", &FFA::timings);
     py::class_<FFAComplex>(m_ffa, "FFACOMPLEXCUDA")
-        .def(py::init<const Config&, int>(), py::arg("cfg"), py::arg("device_id") = 0)
+        .def(py::init<const Config&, int, const Workspace*>(), py::arg("cfg"),
+             py::arg("device_id") = 0, py::arg("workspace") = nullptr, py::keep_alive<1, 4>())
         .def_property_readonly("plan", &FFA::plan)
         .def("execute", &FFA::execute_complex, py::arg("ts_e"), py::arg("ts_v"), py::arg("fold"))
         .def("execute", &FFA::execute, py::arg("ts_e"), py::arg("ts_v"), py::arg("fold"))

@@ -20,11 +20,6 @@ FFAFreqSweep::FFAFreqSweep(const SearchConfig& cfg, int device_id)
       m_device(device_id),
       m_planner(cfg, m_kind, /*use_gpu=*/true) {
     LOKI_REQUIRE(m_planner.nregions() > 0, "FFAFreqSweep: the region planner found no chunks");
-    double total_gb = 0.0;
-    for (const auto& c : m_planner.chunk_stats()) {
-        total_gb += c.total_memory_gb;
-    }
-    m_keep_executors = total_gb <= cfg.get_max_process_memory_gb() - 0.5;
 }
 
 SizeType FFAFreqSweep::execute(const float* ts_e, const float* ts_v, SizeType nsamps) {
@@ -45,19 +40,16 @@ SizeType FFAFreqSweep::execute(const float* ts_e, const float* ts_v, SizeType ns
     hip_check(hipMemcpy(dv.get(), ts_v, nsamps * 4, hipMemcpyHostToDevice), "copy ts_v");
     const auto& widths = m_base.get_scoring_widths();
     m_per_region.assign(m_planner.nregions(), 0);
-    if (m_keep_executors && m_exec.size() != m_planner.nregions()) {
+    if (m_exec.size() != m_planner.nregions()) {
+        m_ws = std::make_shared<FFAWorkspace>();
         m_exec.clear();
         for (const auto& c : m_planner.cfgs()) {
-            m_exec.push_back(std::make_unique<FFAExecutor>(c, m_kind, m_device));
+            m_exec.push_back(std::make_unique<FFAExecutor>(c, m_kind, m_device, m_ws));
         }
     }
     SizeType total = 0;
     for (SizeType r = 0; r < m_planner.nregions(); ++r) {
-        std::unique_ptr<FFAExecutor> own;
-        if (!m_keep_executors) {
-            own = std::make_unique<FFAExecutor>(m_planner.cfgs()[r], m_kind, m_device);
-        }
-        FFAExecutor& ffa = m_keep_executors ? *m_exec[r] : *own;
+        FFAExecutor& ffa = *m_exec[r];
         kern::FilterOut fo{};
         fo.threshold = static_cast<float>(m_base.get_snr_min());
         fo.capacity  = static_cast<uint32_t>(cap - total);

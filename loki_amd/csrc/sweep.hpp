@@ -27,10 +27,11 @@ private:
     FoldKind m_kind;
     int m_device;
     FFARegionPlanner m_planner;
-    /// Chunk executors kept between calls while their device buffers together
-    /// fit the process memory limit (else each chunk builds and frees its own,
-    /// as the planner's memory model assumes); series and candidate buffers.
-    bool m_keep_executors = false;
+    /// Chunk executors, kept between calls; they share one workspace (the
+    /// fold and shift-table buffers, sized by the largest chunk: the
+    /// planner's memory model, ffa_freq_sweep_cuda.cu:44-66); series and
+    /// candidate buffers.
+    std::shared_ptr<FFAWorkspace> m_ws;
     std::vector<std::unique_ptr<FFAExecutor>> m_exec;
     DeviceBuffer m_de, m_dv, m_didx, m_dsnr, m_dcnt;
     std::vector<uint32_t> m_idx;
