@@ -6,9 +6,9 @@
 // Here the bucket lists are built once per plan (FP64 phases, one block per
 // frequency) in a transposed layout list[f][k][b] = k-th sample of bin b, so
 // that lanes holding consecutive bins read consecutive words at every step.
-// The fold runs one lane per (segment, freq, bin), summing its bucket in
-// ascending order from samples staged in LDS -- bit-identical to the CPU
-// order and deterministic.  Reduced-precision storage rounds each FP32 bucket
+// The fold runs one lane per (freq, bin) and one to eight segments, summing
+// each bucket in ascending order from samples staged in LDS -- bit-identical
+// to the CPU order and deterministic.  Reduced-precision storage rounds each FP32 bucket
 // sum once (storage.hpp).
 #include <algorithm>
 #include <type_traits>
@@ -419,6 +419,83 @@ k_bf_fold(const float* __restrict__ ts_e, const float* __restrict__ ts_v, uint32
                                list, base_f, counts, fold, scale);
 }
 
+// Several segments per block: the bucket lists do not depend on the segment,
+// so each list entry read from L2 serves NS gathers from NS staged segments
+// (dynamic LDS: NS x seglen (e, v) pairs, one 8-byte LDS read per sample).
+// Every bucket is still summed in ascending sample order, so the result is
+// the one of k_bf_fold.
+template <int NS, typename TO>
+__global__ void __launch_bounds__(kBlock)
+k_bf_fold_ms(const float* __restrict__ ts_e, const float* __restrict__ ts_v, uint32_t seglen,
+             uint64_t nseg, uint32_t nfreqs, uint32_t nbins, uint32_t blocks_per_seg,
+             const uint32_t* __restrict__ list, const uint64_t* __restrict__ base_f,
+             const uint32_t* __restrict__ counts, TO* __restrict__ fold, float scale) {
+    extern __shared__ float2 s_ev2[];  // [NS][seglen] (e, v)
+    const uint64_t sg0   = static_cast<uint64_t>(blockIdx.x / blocks_per_seg) * NS;
+    const uint32_t pbase = (blockIdx.x % blocks_per_seg) * kPairsBlk;
+    const uint32_t nsv   = static_cast<uint32_t>(nseg - sg0 < NS ? nseg - sg0 : NS);
+    for (uint32_t q = 0; q < nsv; ++q) {
+        const float* __restrict__ ge = ts_e + (sg0 + q) * seglen;
+        const float* __restrict__ gv = ts_v + (sg0 + q) * seglen;
+        float2* __restrict__ l2      = s_ev2 + q * seglen;
+        for (uint32_t k = threadIdx.x; k < seglen; k += kBlock) {
+            l2[k] = make_float2(ge[k], gv[k]);
+        }
+    }
+    __syncthreads();
+    const uint32_t npair = nfreqs * nbins;
+    const uint32_t pend  = min(pbase + static_cast<uint32_t>(kPairsBlk), npair);
+    for (uint32_t p = pbase + threadIdx.x; p < pend; p += kBlock) {
+        const uint32_t f   = p / nbins;
+        const uint32_t b   = p - f * nbins;
+        const uint32_t cnt = counts[p];
+        const uint32_t* __restrict__ lst = list + base_f[f] + b;
+        float se[NS], sv[NS];
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+            se[q] = 0.0F;
+            sv[q] = 0.0F;
+        }
+        uint32_t k = 0;
+        for (; k + 2 <= cnt; k += 2) {
+            const uint32_t i0 = lst[static_cast<uint64_t>(k) * nbins];
+            const uint32_t i1 = lst[static_cast<uint64_t>(k + 1) * nbins];
+#pragma unroll
+            for (int q = 0; q < NS; ++q) {
+                const float2* __restrict__ l2 = s_ev2 + q * seglen;
+                const float2 x0 = l2[i0];
+                const float2 x1 = l2[i1];
+                se[q] = se[q] + x0.x;
+                sv[q] = sv[q] + x0.y;
+                se[q] = se[q] + x1.x;
+                sv[q] = sv[q] + x1.y;
+            }
+        }
+        if (k < cnt) {
+            const uint32_t i = lst[static_cast<uint64_t>(k) * nbins];
+#pragma unroll
+            for (int q = 0; q < NS; ++q) {
+                const float2 x = s_ev2[q * seglen + i];
+                se[q] = se[q] + x.x;
+                sv[q] = sv[q] + x.y;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+            if (static_cast<uint32_t>(q) < nsv) {
+                TO* __restrict__ out = fold + ((sg0 + q) * nfreqs + f) * 2 * nbins;
+                if constexpr (std::is_same_v<TO, float>) {
+                    out[b]         = se[q];
+                    out[nbins + b] = sv[q];
+                } else {
+                    out[b]         = narrow1<TO>(se[q] * scale);
+                    out[nbins + b] = narrow1<TO>(sv[q] * scale);
+                }
+            }
+        }
+    }
+}
+
 // Grouped brute folds of many series (one task each, sorted by block0).
 template <bool USE_LDS, typename TO>
 __global__ void __launch_bounds__(kBlock)
@@ -512,6 +589,17 @@ BruteFoldLists build_brute_fold_lists(const double* freqs_dev, uint32_t nfreqs, 
 
 namespace {
 
+/// Segments per block of the list fold: as many as fit 32 KB of LDS (8 B per
+/// sample), at most 8; 1 selects the one-segment kernel.
+int fold_segs_per_block(uint32_t seglen) {
+    constexpr uint32_t kMsLdsSamps = 4096;
+    if (seglen == 0 || seglen > kMsLdsSamps / 2) {
+        return 1;
+    }
+    const uint32_t ns = kMsLdsSamps / seglen;
+    return ns >= 8 ? 8 : (ns >= 4 ? 4 : 2);
+}
+
 template <typename TO>
 void launch_fold(const float* ts_e, const float* ts_v, uint64_t blocks, uint32_t bps,
                  const BruteFoldLists& L, TO* fold, float scale, hipStream_t s) {
@@ -520,6 +608,26 @@ void launch_fold(const float* ts_e, const float* ts_v, uint64_t blocks, uint32_t
         const auto* fr = L.freqs.as<double>();
         k_bf_units<TO><<<g, kBlock, unit_lds_bytes(L.nbins), s>>>(
             ts_e, ts_v, L.seglen, L.nfreqs, L.nbins, bps, fr, L.tsamp, L.t_ref, fold, scale);
+    } else if (fold_segs_per_block(L.seglen) > 1) {
+        // blocks = nseg * bps; the multi-segment grid covers ceil(nseg / NS) groups
+        const uint64_t nseg = blocks / bps;
+        const int ns        = fold_segs_per_block(L.seglen);
+        const uint64_t grid = (nseg + ns - 1) / ns * bps;
+        const size_t lds    = static_cast<size_t>(ns) * L.seglen * sizeof(float2);
+        const auto g        = static_cast<unsigned>(grid);
+        const auto* li = L.list.as<uint32_t>();
+        const auto* ba = L.base.as<uint64_t>();
+        const auto* co = L.counts.as<uint32_t>();
+        if (ns == 8) {
+            k_bf_fold_ms<8, TO><<<g, kBlock, lds, s>>>(ts_e, ts_v, L.seglen, nseg, L.nfreqs,
+                                                       L.nbins, bps, li, ba, co, fold, scale);
+        } else if (ns == 4) {
+            k_bf_fold_ms<4, TO><<<g, kBlock, lds, s>>>(ts_e, ts_v, L.seglen, nseg, L.nfreqs,
+                                                       L.nbins, bps, li, ba, co, fold, scale);
+        } else {
+            k_bf_fold_ms<2, TO><<<g, kBlock, lds, s>>>(ts_e, ts_v, L.seglen, nseg, L.nfreqs,
+                                                       L.nbins, bps, li, ba, co, fold, scale);
+        }
     } else {
         k_bf_fold<true, TO><<<static_cast<unsigned>(blocks), kBlock, 0, s>>>(
             ts_e, ts_v, L.seglen, L.nfreqs, L.nbins, bps, L.list.as<uint32_t>(),
