@@ -112,11 +112,22 @@ def test_brute_fold_time_run_path_bit_exact(oracle, nbins, segment_len, t_ref, f
     _check_brute_fold(oracle, nbins, segment_len, t_ref, np.linspace(f_lo, f_hi, nf))
 
 
-def _check_brute_fold(oracle, nbins, segment_len, t_ref, freqs):
+# Segments of at most 2048 samples are folded several per block (8, 4 or 2 as the segment
+# length allows): segment counts that leave the last block of segments partly empty.
+@pytest.mark.parametrize(("nbins", "segment_len", "nsamps", "t_ref"), [
+    (32, 512, 13 * 512, 0.0), (64, 1024, 9 * 1024, 0.21), (37, 2000, 5 * 2000, 0.0),
+    (16, 300, 11 * 300, 0.05), (8, 64, 3 * 64, 0.0), (32, 1024, 1024, 0.0),
+])
+def test_brute_fold_time_ragged_segment_groups(oracle, nbins, segment_len, nsamps, t_ref):
+    _check_brute_fold(oracle, nbins, segment_len, t_ref, np.linspace(140.0, 145.0, 37),
+                      nsamps=nsamps)
+
+
+def _check_brute_fold(oracle, nbins, segment_len, t_ref, freqs, nsamps=2**17):
     """Brute fold vs the ascending-order bucket sums (fold.cpp:113-142, kernels.cpp:550-580)."""
     from loki_amd import libculoki
 
-    ts_e, ts_v = _series(2**17, seed=7)
+    ts_e, ts_v = _series(nsamps, seed=7)
     got = libculoki.fold.compute_brute_fold_time_cuda(
         ts_e, ts_v, freqs, segment_len=segment_len, nbins=nbins, tsamp=TSAMP, t_ref=t_ref,
         device_id=0)
