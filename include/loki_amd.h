@@ -472,6 +472,20 @@ int loki_snr_boxcar_2d_max_host(const float* folds, size_t nprof, size_t nbins,
                                 const size_t* widths, size_t nwidths, float stdnoise,
                                 float* scores, int device_id);
 
+/* Self-tests of exact scoring's normalisation (no counterpart in the reference, whose CPU path
+ * computes 1.0F / std::sqrt(v) with IEEE operations, score.cpp:139-141).  The exact-mode kernels
+ * replace sqrtf and the division by short correctly rounded sequences inside their ranges
+ * (v in [2^-96, FLT_MAX] for the root, s in [2^-48, 2^64) for the reciprocal).  The sweep checks
+ * both against the device's IEEE sqrtf and 1.0f / s for every float bit pattern in
+ * [bits_lo, bits_hi): counts[0] / counts[1] = sqrt / reciprocal mismatches, counts[2] /
+ * counts[3] = patterns checked, first_bad[0] / [1] = lowest mismatching pattern (0xFFFFFFFF if
+ * none).  counts has 4 entries, first_bad 2.  Synchronous. */
+int loki_selftest_exact_norm_sweep(int device_id, uint32_t bits_lo, uint32_t bits_hi,
+                                   uint64_t* counts, uint32_t* first_bad);
+/* out[4i..4i+3] = device sqrtf(x), 1.0f / x, short root, short reciprocal of x = bits[i], so
+ * the device IEEE operations can be compared with the host's.  Synchronous. */
+int loki_selftest_exact_norm_values(int device_id, const uint32_t* bits, size_t n, float* out);
+
 /* --- Differentiable candidate refinement -------------------------------- */
 /* No counterpart in the reference: a continuous-parameter generalisation of the Fourier brute
  * fold (kernels_cuda.cu:780-853, one segment spanning the series) with the Taylor phase model of

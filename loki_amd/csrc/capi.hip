@@ -1078,6 +1078,43 @@ int loki_snr_boxcar_3d_host(const float* folds, size_t nprof, size_t nbins,
     });
 }
 
+int loki_selftest_exact_norm_sweep(int device_id, uint32_t bits_lo, uint32_t bits_hi,
+                                   uint64_t* counts, uint32_t* first_bad) {
+    return guarded([&] {
+        REQ(counts != nullptr && first_bad != nullptr, "null argument");
+        loki_amd::DeviceGuard g(device_id);
+        loki_amd::DeviceBuffer dc(4 * sizeof(uint64_t)), df(2 * sizeof(uint32_t));
+        const uint64_t zero[4]  = {0, 0, 0, 0};
+        const uint32_t none[2]  = {0xFFFFFFFFU, 0xFFFFFFFFU};
+        loki_amd::hip_check(hipMemcpy(dc.get(), zero, sizeof(zero), hipMemcpyHostToDevice), "H2D");
+        loki_amd::hip_check(hipMemcpy(df.get(), none, sizeof(none), hipMemcpyHostToDevice), "H2D");
+        loki_amd::kern::selftest_exact_norm_sweep(bits_lo, bits_hi,
+                                                  dc.as<unsigned long long>(),
+                                                  df.as<uint32_t>(), nullptr);
+        loki_amd::hip_check(hipGetLastError(), "self-test launch");
+        loki_amd::hip_check(hipMemcpy(counts, dc.get(), sizeof(zero), hipMemcpyDeviceToHost),
+                            "D2H");
+        loki_amd::hip_check(hipMemcpy(first_bad, df.get(), sizeof(none), hipMemcpyDeviceToHost),
+                            "D2H");
+    });
+}
+
+int loki_selftest_exact_norm_values(int device_id, const uint32_t* bits, size_t n, float* out) {
+    return guarded([&] {
+        REQ(bits != nullptr && out != nullptr, "null argument");
+        loki_amd::DeviceGuard g(device_id);
+        if (n == 0) {
+            return;
+        }
+        loki_amd::DeviceBuffer db(n * 4), dout(n * 16);
+        loki_amd::hip_check(hipMemcpy(db.get(), bits, n * 4, hipMemcpyHostToDevice), "H2D");
+        loki_amd::kern::selftest_exact_norm_values(db.as<uint32_t>(), n, dout.as<float>(),
+                                                   nullptr);
+        loki_amd::hip_check(hipGetLastError(), "self-test launch");
+        loki_amd::hip_check(hipMemcpy(out, dout.get(), n * 16, hipMemcpyDeviceToHost), "D2H");
+    });
+}
+
 int loki_snr_boxcar_3d_filter_device(const float* folds, size_t nprof, size_t nbins,
                                      const size_t* widths, size_t nwidths, float threshold,
                                      int find_max, uint32_t* indices, float* scores,

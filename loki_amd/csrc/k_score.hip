@@ -247,4 +247,78 @@ void zero_counter(uint32_t* count, hipStream_t s) {
     k_zero_u32<<<1, 1, 0, s>>>(count);
 }
 
+namespace {
+constexpr float kRcpLo = 0x1p-48F;  // rcp_rn_normal's range [2^-48, 2^64)
+constexpr float kRcpHi = 0x1p64F;
+
+__global__ void __launch_bounds__(256)
+    k_selftest_exact_norm_sweep(uint32_t lo, uint32_t hi, unsigned long long* counts,
+                                uint32_t* first_bad) {
+    unsigned long long bad_s = 0, bad_r = 0, n_s = 0, n_r = 0;
+    uint32_t first_s = 0xFFFFFFFFU, first_r = 0xFFFFFFFFU;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t u = lo + static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; u < hi;
+         u += stride) {
+        const uint32_t b = static_cast<uint32_t>(u);
+        const float x    = __uint_as_float(b);
+        if (rn_normal_range(x)) {
+            ++n_s;
+            if (__float_as_uint(sqrt_rn_normal(x)) != __float_as_uint(sqrtf(x))) {
+                ++bad_s;
+                first_s = min(first_s, b);
+            }
+        }
+        if (x >= kRcpLo && x < kRcpHi) {
+            ++n_r;
+            if (__float_as_uint(rcp_rn_normal(x)) != __float_as_uint(1.0F / x)) {
+                ++bad_r;
+                first_r = min(first_r, b);
+            }
+        }
+    }
+    atomicAdd(&counts[2], n_s);
+    atomicAdd(&counts[3], n_r);
+    if (bad_s != 0) {
+        atomicAdd(&counts[0], bad_s);
+        atomicMin(&first_bad[0], first_s);
+    }
+    if (bad_r != 0) {
+        atomicAdd(&counts[1], bad_r);
+        atomicMin(&first_bad[1], first_r);
+    }
+}
+
+__global__ void __launch_bounds__(256)
+    k_selftest_exact_norm_values(const uint32_t* bits, uint64_t n, float* out) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    const float x  = __uint_as_float(bits[i]);
+    out[4 * i]     = sqrtf(x);
+    out[4 * i + 1] = 1.0F / x;
+    out[4 * i + 2] = sqrt_rn_normal(x);
+    out[4 * i + 3] = rcp_rn_normal(x);
+}
+}  // namespace
+
+void selftest_exact_norm_sweep(uint32_t bits_lo, uint32_t bits_hi, unsigned long long* counts,
+                               uint32_t* first_bad, hipStream_t s) {
+    LOKI_REQUIRE(counts != nullptr && first_bad != nullptr, "null argument");
+    if (bits_hi <= bits_lo) {
+        return;
+    }
+    k_selftest_exact_norm_sweep<<<8192, 256, 0, s>>>(bits_lo, bits_hi, counts, first_bad);
+}
+
+void selftest_exact_norm_values(const uint32_t* bits, uint64_t n, float* out, hipStream_t s) {
+    LOKI_REQUIRE(bits != nullptr && out != nullptr, "null argument");
+    if (n == 0) {
+        return;
+    }
+    const uint64_t blocks = (n + 255) / 256;
+    LOKI_REQUIRE(blocks <= 0x7FFFFFFFULL, "too many values");
+    k_selftest_exact_norm_values<<<static_cast<uint32_t>(blocks), 256, 0, s>>>(bits, n, out);
+}
+
 }  // namespace loki_amd::kern

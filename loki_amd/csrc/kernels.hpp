@@ -127,6 +127,19 @@ struct FilterOut {
 /// Zeroes a device counter with a kernel (capturable in a graph).
 void zero_counter(uint32_t* count, hipStream_t s);
 
+/// Checks the short correctly rounded sequences of exact scoring
+/// (sqrt_rn_normal, rcp_rn_normal) against the IEEE sqrtf and 1.0f / x for
+/// every float bit pattern u in [bits_lo, bits_hi) inside each sequence's
+/// range.  counts[0] / counts[1] (u64, device, zeroed by the caller) receive
+/// the sqrt / reciprocal mismatches and counts[2] / counts[3] the number of
+/// patterns checked; first_bad[0] / [1] (u32, device, preset to 0xFFFFFFFF)
+/// the lowest mismatching pattern.
+void selftest_exact_norm_sweep(uint32_t bits_lo, uint32_t bits_hi, unsigned long long* counts,
+                               uint32_t* first_bad, hipStream_t s);
+/// For n bit patterns: out[4i..4i+3] = sqrtf(x), 1.0f / x, sqrt_rn_normal(x),
+/// rcp_rn_normal(x) as computed on the device.
+void selftest_exact_norm_values(const uint32_t* bits, uint64_t n, float* out, hipStream_t s);
+
 /// Boxcar widths with the reference's template constants (score.cpp:107-116).
 struct WidthTable {
     static constexpr uint32_t kMax = 64;

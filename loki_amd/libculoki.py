@@ -7,18 +7,23 @@ name so that callers switch backends by changing the import only.
 
 from __future__ import annotations
 
-from types import SimpleNamespace
+import sys
+import types
 
 from . import _loki_amd as _n
 
 ffa = _n.ffa
 fold = _n.fold
-scores = SimpleNamespace(
-    snr_boxcar_3d=_n.scores.snr_boxcar_3d,
-    snr_boxcar_3d_max=_n.scores.snr_boxcar_3d_max,
-    snr_boxcar_2d_max=_n.scores.snr_boxcar_2d_max,
-    # score_and_filter[_max]_cuda_d (score_cuda.cu:382-443): (indices, snr) above a threshold
-    snr_boxcar_3d_filter=_n.scores.snr_boxcar_3d_filter,
-)
+scores = types.ModuleType(__name__ + ".scores", "Scores submodule")
+scores.snr_boxcar_3d = _n.scores.snr_boxcar_3d
+scores.snr_boxcar_3d_max = _n.scores.snr_boxcar_3d_max
+scores.snr_boxcar_2d_max = _n.scores.snr_boxcar_2d_max
+# score_and_filter[_max]_cuda_d (score_cuda.cu:382-443): (indices, snr) above a threshold
+scores.snr_boxcar_3d_filter = _n.scores.snr_boxcar_3d_filter
+
+# importable by dotted name (``from loki_amd.libculoki.ffa import FFACUDA``)
+for _name, _mod in (("ffa", ffa), ("fold", fold), ("scores", scores)):
+    sys.modules[f"{__name__}.{_name}"] = _mod
+del _name, _mod
 
 __all__ = ["ffa", "fold", "scores"]

@@ -488,3 +488,55 @@ void orc_snr_boxcar_3d_f64(const double* folds, size_t nprof, size_t nbins,
         free(psum);
     }
 }
+
+/*
+ * Reduced-precision fold storage (the precision addition; no reference
+ * counterpart): x[i] = round(x[i] * scale) to IEEE binary16 (storage 1) or
+ * bfloat16 (storage 2), round to nearest even, returned as the exactly equal
+ * float.  scale is a power of two, so x * scale is exact.  binary16 overflow
+ * (|y| >= 65520) gives inf; NaN stays NaN.
+ */
+static inline float orc_round_f16(float x) {
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    const uint32_t sign = u & 0x80000000u;
+    const uint32_t a = u & 0x7FFFFFFFu;
+    if (a >= 0x7F800000u) return x;                 /* inf, NaN */
+    if (a >= 0x477FF000u) return sign ? -INFINITY : INFINITY;
+    if (a >= 0x38800000u) {                          /* normal binary16: keep 10 bits */
+        uint32_t r = a + 0x0FFFu + ((a >> 13) & 1u);
+        r &= ~0x1FFFu;
+        r |= sign;
+        float y;
+        memcpy(&y, &r, 4);
+        return y;
+    }
+    /* subnormal binary16: multiples of 2^-24; |x| * 2^24 < 2^10 is exact */
+    const float q = rintf(fabsf(x) * 16777216.0f) * (1.0f / 16777216.0f);
+    return sign ? -q : q;
+}
+
+static inline float orc_round_bf16(float x) {
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    if ((u & 0x7FFFFFFFu) > 0x7F800000u) return x;  /* NaN */
+    const uint64_t w = (uint64_t)u + 0x7FFFu + ((u >> 16) & 1u);
+    uint32_t r = (uint32_t)((w >> 16) << 16);
+    float y;
+    memcpy(&y, &r, 4);
+    return y;
+}
+
+void orc_round_storage(float* x, size_t n, float scale, int storage, int nthreads) {
+    const int nt = clamp_threads(nthreads);
+    if (storage == 1) {
+#pragma omp parallel for num_threads(nt) schedule(static)
+        for (size_t i = 0; i < n; ++i) x[i] = orc_round_f16(x[i] * scale);
+    } else if (storage == 2) {
+#pragma omp parallel for num_threads(nt) schedule(static)
+        for (size_t i = 0; i < n; ++i) x[i] = orc_round_bf16(x[i] * scale);
+    } else {
+#pragma omp parallel for num_threads(nt) schedule(static)
+        for (size_t i = 0; i < n; ++i) x[i] = x[i] * scale;
+    }
+}

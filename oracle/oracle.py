@@ -54,6 +54,7 @@ def lib() -> C.CDLL:
         L.orc_diff_max.argtypes = [_f32p, _f32p, _sz]
         L.orc_snr_boxcar_3d.argtypes = [_f32p, _sz, _sz, _u64p, _sz, _f32p, C.c_int, C.c_int]
         L.orc_snr_boxcar_3d_f64.argtypes = [_f64p, _sz, _sz, _u64p, _sz, _f64p, C.c_int]
+        L.orc_round_storage.argtypes = [_f32p, _sz, C.c_float, C.c_int, C.c_int]
         _LIB = L
     return _LIB
 
@@ -187,6 +188,20 @@ def round_storage(x, storage: str):
     raise ValueError(f"unknown storage {storage!r}")
 
 
+_STORAGE_CODE = {"f32": 0, "f16": 1, "bf16": 2}
+
+
+def scale_round_storage(x, scale: float, storage: str, nthreads=None):
+    """round_storage(x * scale, storage) in place on a float32 array, in C with OpenMP (the
+    same rounding; used on whole fold levels, where numpy's temporaries would not fit)."""
+    if x.dtype != np.float32 or not x.flags.c_contiguous:
+        raise ValueError("scale_round_storage needs a C-contiguous float32 array")
+    flat = x.reshape(-1)
+    lib().orc_round_storage(flat, flat.size, float(scale), _STORAGE_CODE[storage],
+                            _nthreads(nthreads))
+    return x
+
+
 def storage_k0(cfg: SearchConfig, plan: Plan) -> int:
     """Level-0 scale exponent of reduced-precision storage: floor(log2(seglen // nbins))."""
     per, k0 = plan.segment_lens[0] // cfg.nbins, 0
@@ -244,14 +259,15 @@ def _compute_ffa_storage(ts_e, ts_v, cfg: SearchConfig, nthreads, storage: str):
     last = plan.n_levels - 1
     if last == 0:
         return fold, plan
-    fold = round_storage(fold * np.float32(2.0 ** -k0), storage)
+    fold = scale_round_storage(fold, 2.0 ** -k0, storage, nthreads)
     for lvl in range(1, plan.n_levels):
         out = ffa_iter(fold, coords[lvl], plan.ncoords[lvl], plan.ncoords[lvl - 1],
                        plan.nsegments[lvl], cfg.nbins, nthreads)
+        del fold
         if lvl < last:
-            fold = round_storage(out * np.float32(0.5), storage)
+            fold = scale_round_storage(out, 0.5, storage, nthreads)
         else:
-            fold = out * np.float32(2.0 ** (k0 + lvl - 1))
+            fold = scale_round_storage(out, 2.0 ** (k0 + lvl - 1), "f32", nthreads)
     return fold, plan
 
 
@@ -274,5 +290,5 @@ __all__ = [
     "FFA_COORD_DTYPE", "FFA_COORD_FREQ_DTYPE", "Plan", "SearchConfig", "brute_fold_fourier",
     "brute_fold_time", "build", "circular_prefix_sum", "compute_ffa", "compute_ffa_scores",
     "diff_max", "ffa_iter", "irfft", "lib", "rfft", "round_storage", "snr_boxcar_3d",
-    "snr_boxcar_3d_f64", "storage_k0",
+    "scale_round_storage", "snr_boxcar_3d_f64", "storage_k0",
 ]

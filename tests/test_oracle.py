@@ -486,6 +486,30 @@ def test_round_storage_known_values(oracle):
     assert bf[7] == -2.5 and np.isfinite(bf[9])
 
 
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_scale_round_storage_c_equals_numpy(oracle, storage):
+    """The OpenMP C rounding used on whole levels equals numpy's (float16 cast; RNE on the
+    FP32 pattern) on 2^22 random bit patterns of every finite magnitude, the binary16
+    overflow / subnormal / tie edges and power-of-two scales."""
+    rng = np.random.default_rng(99)
+    bits = rng.integers(0, 0x7F800000, size=2**22, dtype=np.uint32)
+    bits |= rng.integers(0, 2, size=bits.size, dtype=np.uint32) << 31
+    x = bits.view(np.float32)
+    edges = np.array([0.0, -0.0, 65504.0, 65519.996, 65520.0, -65520.0, 2.0**-14, 2.0**-24,
+                      2.0**-25, 1.5 * 2.0**-24, 2.5 * 2.0**-24, 1.0 + 2.0**-11,
+                      1.0 + 3 * 2.0**-11, 1.0 + 2.0**-8, 1.0 + 3 * 2.0**-8, 3.0e38, -3.4e38,
+                      np.inf, -np.inf], np.float32)
+    x = np.concatenate([x, edges])
+    finite_scaled = {}
+    for scale in (1.0, 0.5, 2.0**-9, 2.0**12):
+        with np.errstate(over="ignore"):
+            want = oracle.round_storage(x * np.float32(scale), storage)
+        got = oracle.scale_round_storage(x.copy(), scale, storage)
+        np.testing.assert_array_equal(got, want)
+        finite_scaled[scale] = int(np.isfinite(got).sum())
+    assert min(finite_scaled.values()) > 0
+
+
 def test_reduced_storage_merge_is_scaled_sum(oracle):
     """Level l of the emulated tree equals round((t + h) * 0.5) of the level below, and the
     returned last level equals the unrounded sum times 2^(k0 + L - 2)."""
