@@ -170,3 +170,32 @@ def test_resolve_coordinates_needs_matching_kind():
                                              use_fourier=False)
     with pytest.raises(RuntimeError, match="nparams>=2"):
         libloki.plans.FFAPlanTime(cfg).resolve_coordinates()
+
+
+def test_libloki_ffa_namespace_matches_the_cpu_module():
+    """search.py:7 imports ``loki.libloki.ffa.compute_ffa_scores``; the drop-in module has the
+    names of src/loki_python.cpp:518-592 with the same argument names and defaults, importable
+    by dotted name."""
+    import inspect
+
+    from loki_amd.libloki.ffa import (FFAFourier, FFAFreqSweep, FFATime, compute_ffa_fourier,
+                                      compute_ffa_fourier_return_to_time, compute_ffa_scores,
+                                      compute_ffa_time)
+
+    for fn in (compute_ffa_time, compute_ffa_fourier, compute_ffa_fourier_return_to_time,
+               compute_ffa_scores):
+        sig = inspect.signature(fn)
+        assert list(sig.parameters) == ["ts_e", "ts_v", "cfg", "quiet", "show_progress"]
+        assert sig.parameters["quiet"].default is False
+        assert sig.parameters["show_progress"].default is False
+    for cls in (FFATime, FFAFourier, FFAFreqSweep):
+        sig = inspect.signature(cls.__init__)
+        assert list(sig.parameters) == ["self", "cfg", "show_progress"]
+        assert sig.parameters["show_progress"].default is True
+    assert list(inspect.signature(FFATime.execute).parameters) == ["self", "ts_e", "ts_v", "fold"]
+    assert list(inspect.signature(FFAFreqSweep.execute).parameters) == [
+        "self", "ts_e", "ts_v", "outdir", "file_prefix"]
+    from loki_amd.libloki.configs import PulsarSearchConfig  # noqa: F401
+    from loki_amd.libloki.plans import FFAPlanTime  # noqa: F401
+    from loki_amd.libloki.scores import generate_box_width_trials  # noqa: F401
+    from loki_amd.libculoki.ffa import FFACUDA, compute_ffa_scores_cuda  # noqa: F401
