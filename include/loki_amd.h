@@ -215,6 +215,11 @@ int loki_ffa_set_timing(loki_ffa_t ffa, int enable);
  * stay within the Fourier S/N bound, not bitwise equal.  No reference counterpart (the
  * reference always stores the final fold, ffa_cuda.cu:670-692). */
 int loki_ffa_set_fused(loki_ffa_t ffa, int enable);
+/* Levels fused into the scores kernel of the time-domain path (nbins 8, 16 or 32): 1 the last
+ * level only (its parent level is stored in HBM), 2 (default) the parent level too, merged on
+ * chip from the level below it and never stored.  Same bits either way; other nbins keep the
+ * one-level fused kernel.  No reference counterpart (ffa_cuda.cu:650-692 stores every level). */
+int loki_ffa_set_fuse_depth(loki_ffa_t ffa, int depth);
 /* Profile normalisation in the fused scores path: 1 (default) the CPU's correctly rounded
  * e * (1/sqrt(v)), bit-identical to the reference CPU; 0 the hardware reciprocal square root
  * (|dS/N| ~ 1e-6, the analogue of the reference CUDA path's fast-math e/sqrtf(v),

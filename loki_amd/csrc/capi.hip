@@ -564,6 +564,13 @@ int loki_ffa_set_fused(loki_ffa_t ffa, int enable) {
     });
 }
 
+int loki_ffa_set_fuse_depth(loki_ffa_t ffa, int depth) {
+    return guarded([&] {
+        REQ(ffa != nullptr, "null ffa");
+        ffa->exec.set_fuse_depth(depth);
+    });
+}
+
 int loki_ffa_set_dft_precision(loki_ffa_t ffa, int mode) {
     return guarded([&] {
         REQ(ffa != nullptr, "null argument");

@@ -9,6 +9,7 @@
 #include <string>
 
 #include "kernels.hpp"
+#include "resolve_device.hpp"
 
 namespace loki_amd {
 
@@ -273,7 +274,7 @@ void FFAExecutor::init_level(const float* ts_e, const float* ts_v, void* out, vo
 
 void* FFAExecutor::run_levels(const float* ts_e, const float* ts_v, void* buf_even,
                               void* buf_odd, SizeType last_level, bool last_fp32,
-                              hipStream_t s) {
+                              hipStream_t s, SizeType coords_level) {
     // Level k lands in buf_even for even k and in buf_odd for odd k (the
     // reference picks its ping-pong start side the same way so the last level
     // lands in the caller's buffer, ffa.cpp:261-276).
@@ -281,7 +282,7 @@ void* FFAExecutor::run_levels(const float* ts_e, const float* ts_v, void* buf_ev
     m_timer.reset();
     m_ws->coords.reserve(m_plan.coord_size() * coord_record_bytes());
     prepare_brute_fold(s);
-    resolve_coordinates(m_ws->coords.get(), s, last_level);
+    resolve_coordinates(m_ws->coords.get(), s, std::max(last_level, coords_level));
 
     void* even           = buf_even;
     void* odd            = buf_odd;
@@ -376,6 +377,107 @@ bool FFAExecutor::fused_scores_supported() const {
            m_plan.nsegments().back() == 1;
 }
 
+void FFAExecutor::set_fuse_depth(int depth) {
+    LOKI_REQUIRE(depth == 1 || depth == 2, "fuse depth must be 1 or 2");
+    m_fuse_depth = depth;
+}
+
+bool FFAExecutor::final2_usable() const {
+    return !m_fourier && kern::final2_supported(static_cast<uint32_t>(m_plan.config().get_nbins())) &&
+           m_plan.nsegments().back() == 1 && m_plan.n_levels() >= 2;
+}
+
+namespace {
+
+FFACoord resolve_rc(const kern::ResolveLevel& lv, const double* dv, uint32_t row,
+                          uint32_t jf) {
+    switch (lv.np) {
+    case 1: return kern::resolve_pair_rc<1>(lv, dv, row, jf);
+    case 2: return kern::resolve_pair_rc<2>(lv, dv, row, jf);
+    default: return kern::resolve_pair_rc<3>(lv, dv, row, jf);
+    }
+}
+
+}  // namespace
+
+void FFAExecutor::prepare_final2(hipStream_t s) {
+    if (m_f2_ready) {
+        return;
+    }
+    const SizeType L   = m_plan.n_levels();
+    const auto lv      = resolve_level(m_plan, L - 1);
+    double dv[3];
+    for (int k = 0; k < 3; ++k) {
+        dv[k] = (lv.hi[k] - lv.lo[k]) / static_cast<double>(lv.n_cur[k]);
+    }
+    const uint32_t n_f    = lv.n_cur[2];
+    const uint32_t n_pf   = lv.n_prev[2];
+    const SizeType n_rows = m_plan.ncoords()[L - 1] / n_f;
+    LOKI_REQUIRE(n_rows < (1ULL << 32), "final level: too many grid rows");
+    // parent rows (tail, head) of every final row: they depend on the row's
+    // (jerk, accel) cell only
+    std::vector<uint64_t> pt(n_rows), ph(n_rows);
+    for (SizeType r = 0; r < n_rows; ++r) {
+        const auto c = resolve_rc(lv, dv, static_cast<uint32_t>(r), 0);
+        pt[r]        = c.i_tail / n_pf;
+        ph[r]        = c.i_head / n_pf;
+    }
+    // runs of consecutive rows with the same parents
+    std::vector<std::pair<uint32_t, uint32_t>> groups;  // (row0, size)
+    for (SizeType r = 0; r < n_rows;) {
+        SizeType e = r + 1;
+        while (e < n_rows && pt[e] == pt[r] && ph[e] == ph[r]) {
+            ++e;
+        }
+        groups.emplace_back(static_cast<uint32_t>(r), static_cast<uint32_t>(e - r));
+        r = e;
+    }
+    std::vector<uint32_t> sizes;
+    for (const auto& g : groups) {
+        sizes.push_back(g.second);
+    }
+    std::nth_element(sizes.begin(), sizes.begin() + sizes.size() / 2, sizes.end());
+    const uint32_t med = sizes[sizes.size() / 2];
+    m_f2_rows_per_block = med >= 4 ? 4U : (med >= 2 ? 2U : 1U);
+    m_f2_chunks_host.clear();
+    for (const auto& g : groups) {
+        for (uint32_t o = 0; o < g.second; o += m_f2_rows_per_block) {
+            m_f2_chunks_host.push_back(
+                {g.first + o, std::min(m_f2_rows_per_block, g.second - o)});
+        }
+    }
+    const uint32_t n_ft    = (n_f + 63) / 64;
+    const uint64_t nblocks = static_cast<uint64_t>(m_f2_chunks_host.size()) * n_ft;
+    LOKI_REQUIRE(nblocks < (1ULL << 31), "final level: grid too large");
+    m_f2_nblocks = static_cast<uint32_t>(nblocks);
+    m_f2_chunks.reserve(m_f2_chunks_host.size() * sizeof(kern::RowChunk));
+    hip_check(hipMemcpyAsync(m_f2_chunks.get(), m_f2_chunks_host.data(),
+                             m_f2_chunks_host.size() * sizeof(kern::RowChunk),
+                             hipMemcpyHostToDevice, s),
+              "copy final-level row chunks");
+    // parent rows staged per side: the exact widest range of any block, from
+    // the device resolver itself (one pass per plan; synchronises s)
+    kern::Final2Args a{};
+    a.lv = lv;
+    for (int k = 0; k < 3; ++k) {
+        a.fg.dv[k] = dv[k];
+    }
+    a.chunks   = m_f2_chunks.as<kern::RowChunk>();
+    a.n_chunks = static_cast<uint32_t>(m_f2_chunks_host.size());
+    DeviceBuffer d_cap(sizeof(uint32_t));
+    kern::zero_counter(d_cap.as<uint32_t>(), s);
+    kern::final2_ranges(a, m_f2_nblocks, d_cap.as<uint32_t>(), s);
+    check_launch("final-level range launch");
+    uint32_t cap = 0;
+    hip_check(hipMemcpyAsync(&cap, d_cap.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, s),
+              "copy final-level range");
+    hip_check(hipStreamSynchronize(s), "final-level range");
+    m_f2_cap = cap;
+    m_f2_ok  = cap > 0 && kern::final2_lds_bytes(static_cast<uint32_t>(m_plan.config().get_nbins()),
+                                                  cap) <= 64U * 1024U;
+    m_f2_ready = true;
+}
+
 void FFAExecutor::execute_scores(const float* ts_e, const float* ts_v,
                                  std::span<const SizeType> widths, float* scores,
                                  SizeType scores_len, bool find_max, hipStream_t s,
@@ -397,6 +499,56 @@ void FFAExecutor::execute_scores(const float* ts_e, const float* ts_v,
                          (fo->capacity == 0 || (fo->indices != nullptr && fo->snr != nullptr)),
                      "score filter: null output buffer");
         kern::zero_counter(fo->count, s);
+    }
+    if (m_fused && final2_usable()) {
+        prepare_final2(s);
+    }
+    if (m_fused && final2_usable() && m_f2_ok) {
+        // One lane per trial: the last level fused with its shift tables and
+        // the S/N; with depth 2 its parent level is merged on chip as well.
+        // Neither the final fold cube nor (depth 2) the parent level is stored.
+        const bool grand    = m_fuse_depth == 2 && L >= 3;
+        const SizeType top  = grand ? L - 3 : L - 2;  // last level through HBM
+        const SizeType need = max_level_size(top) * storage_bytes(m_storage);
+        m_ws->internal.reserve(need);
+        m_ws->result.reserve(need);
+        const void* stored =
+            run_levels(ts_e, ts_v, m_ws->result.get(), m_ws->internal.get(), top, false, s, L - 2);
+        kern::Final2Args a{};
+        a.in   = stored;
+        a.n_p1 = m_plan.ncoords()[L - 2];
+        a.n_p2 = grand ? m_plan.ncoords()[L - 3] : 0;
+        if (grand) {
+            const SizeType off = m_plan.ncoords_offsets64()[L - 2];
+            if (m_freq_only) {
+                a.coords1f = m_ws->coords.as<FFACoordFreq>() + off;
+            } else {
+                a.coords1 = m_ws->coords.as<FFACoord>() + off;
+            }
+        }
+        a.lv = resolve_level(m_plan, L - 1);
+        for (int k = 0; k < 3; ++k) {
+            a.fg.dv[k] = (a.lv.hi[k] - a.lv.lo[k]) / static_cast<double>(a.lv.n_cur[k]);
+        }
+        a.wt             = wt;
+        a.scores         = scores;
+        a.fo             = fo != nullptr ? *fo : kern::FilterOut{};
+        a.chunks         = m_f2_chunks.as<kern::RowChunk>();
+        a.n_chunks       = static_cast<uint32_t>(m_f2_chunks_host.size());
+        a.rows_per_block = m_f2_rows_per_block;
+        a.cap            = m_f2_cap;
+        a.grand          = grand ? 1U : 0U;
+        a.unscale        = std::ldexp(1.0F, m_k0 + static_cast<int>(L) - 2);
+        a.scale1         = 0.5F;
+        if (fo != nullptr) {
+            LOKI_REQUIRE(ncoords * (find_max ? 1U : widths.size()) <= 0xFFFFFFFFULL,
+                         "score filter: trial index exceeds 32 bits");
+        }
+        const int slot = m_timer.start(KernelKind::kMergeScore, static_cast<int>(L - 1), s);
+        kern::final2_merge_score(a, m_storage, m_f2_nblocks, find_max, m_exact_norm, s);
+        check_launch("final-level merge+score launch");
+        m_timer.stop(slot, s);
+        return;
     }
     if (m_fused && fused_scores_supported()) {
         // Levels 0..L-2 through HBM, the last level fused with its shift

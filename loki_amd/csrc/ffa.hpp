@@ -116,6 +116,12 @@ public:
     /// Use the fused last-level kernel in execute_scores() when nbins allows
     /// (default on).  Off: unfused merge + snr_boxcar_3d (same bits).
     void set_fused(bool on) noexcept { m_fused = on; }
+    /// Levels fused into the scoring kernel of execute_scores(): 1 (the last
+    /// level; its parent level is stored) or 2 (the parent level is merged on
+    /// chip as well and never stored; nbins 8, 16 or 32 and at least three
+    /// levels).  Same bits either way.  Default 2.
+    void set_fuse_depth(int depth);
+    int fuse_depth() const noexcept { return m_fuse_depth; }
     /// Profile normalisation of the fused scores path: exact (default, the
     /// CPU's correctly rounded e * (1/sqrt(v))) or the hardware rsqrt.
     void set_exact_norm(bool on) noexcept { m_exact_norm = on; }
@@ -148,15 +154,31 @@ private:
     bool m_dft_split{false};
     FoldStorage m_storage{FoldStorage::kF32};
     int m_k0{0};  // level l of a reduced-precision tree is stored at scale 2^-(k0 + l)
+    int m_fuse_depth{2};
     LaunchTimer m_timer;
+    // one-lane-per-trial last level (k_final2.hip): row chunks sharing parent
+    // rows, built once per plan
+    bool m_f2_ready{false};
+    bool m_f2_ok{false};  // parent ranges fit the LDS budget
+    std::vector<kern::RowChunk> m_f2_chunks_host;
+    DeviceBuffer m_f2_chunks;
+    uint32_t m_f2_rows_per_block{1};
+    uint32_t m_f2_cap{0};
+    uint32_t m_f2_nblocks{0};
+
+    void prepare_final2(hipStream_t s);
+    bool final2_usable() const;
 
     void prepare_brute_fold(hipStream_t s);
     /// Brute fold + merge levels 1..last_level; returns the buffer holding
     /// last_level (buf_even for even levels, buf_odd for odd levels).
     /// last_fp32: last_level is stored in FP32 at its true scale even when
     /// the tree uses reduced-precision storage.
+    /// coords_level: shift tables are resolved up to max(last_level,
+    /// coords_level).
     void* run_levels(const float* ts_e, const float* ts_v, void* buf_even, void* buf_odd,
-                     SizeType last_level, bool last_fp32, hipStream_t s);
+                     SizeType last_level, bool last_fp32, hipStream_t s,
+                     SizeType coords_level = 0);
     void init_level(const float* ts_e, const float* ts_v, void* out, void* scratch,
                     bool fp32_out, hipStream_t s);
     SizeType max_level_size(SizeType last_level) const;

@@ -1,0 +1,484 @@
+// Fused last FFA levels, one lane per trial: FP64 shift tables + merge + boxcar
+// S/N, with the parent rows staged once per block in LDS.
+//
+// compute_ffa_scores (ffa.cpp:444-466) writes the final fold cube and reads
+// it back for snr_boxcar_3d.  Here each block owns a chunk of final grid rows
+// (flattened (jerk, accel) cells) that share their parent rows, over a run of
+// consecutive frequencies; one wave covers 64 frequencies of one row and each
+// lane one trial:
+//   0. the lane resolves its trial in FP64 (taylor_ffa.cpp:19-271 order, the
+//      same bits as the table kernels): parent indices and shifts;
+//   1. the block's parent rows -- a contiguous range per side, since the
+//      chunk's rows share parent rows and the parent frequency index is
+//      monotone in the trial frequency -- are staged in LDS as (e, v) pairs
+//      per bin, with bins 0..2 repeated after the last bin so that any four
+//      consecutive bins of a rotated row are one contiguous read.  The rows
+//      are either copied from the stored parent level or, with `grand` set,
+//      merged on chip from the grandparent level through the parent level's
+//      shift table (kernels.cpp:71-146: one FP32 add per element, then the
+//      reduced-precision rounding of a stored level), so the parent level
+//      never reaches HBM;
+//   2. the lane merges its tail and head rows (one FP32 add each), forms
+//      p = e * (1/sqrt(v)) (score.cpp:139-141) and the sequential circular
+//      prefix sum in registers;
+//   3. it scores every width from register windows (score.cpp:95-172):
+//      snr = (h + b) * dmax - b * sum.
+// Every FP32 operation matches the unfused path and the CPU oracle bit for
+// bit.  The LDS allocation holds the widest parent range of any block of
+// the launch, found beforehand by final2_ranges with the same resolver.
+#include <cfloat>
+#include <type_traits>
+#include <utility>
+
+#include "merge_device.hpp"
+#include "resolve_device.hpp"
+#include "score_device.hpp"
+
+namespace loki_amd::kern {
+namespace {
+
+constexpr uint32_t kLanes = 64;
+constexpr uint32_t kBlock = 4 * kLanes;  // largest block: 4 rows
+
+/// Repeated leading bins of a staged row: a rotated read of 4 bins starting
+/// at any bin stays inside the row.
+constexpr uint32_t kPad = 3;
+
+__device__ __forceinline__ uint32_t wave_min(uint32_t x) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        x = min(x, static_cast<uint32_t>(__shfl_xor(static_cast<int>(x), o)));
+    }
+    return x;
+}
+__device__ __forceinline__ uint32_t wave_max(uint32_t x) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        x = max(x, static_cast<uint32_t>(__shfl_xor(static_cast<int>(x), o)));
+    }
+    return x;
+}
+
+/// Reduced-precision rounding of a stored level: FP32 -> TI -> FP32 (exact
+/// round trip of the rounded value); identity for FP32.
+template <typename TI>
+__device__ __forceinline__ float round_stored(float x) {
+    if constexpr (std::is_same_v<TI, float>) {
+        return x;
+    } else {
+        return widen1(narrow1<TI>(x));
+    }
+}
+
+template <int NB>
+__device__ __forceinline__ uint32_t wrap(uint32_t b) {
+    return b & static_cast<uint32_t>(NB - 1);
+}
+
+/// Parent record of level (F-1) coordinate k for the grandparent merge.
+template <int NP>
+__device__ __forceinline__ void parent_record(const Final2Args& a, uint32_t k, uint32_t& it,
+                                              uint32_t& st, uint32_t& ih, uint32_t& sh,
+                                              uint32_t nb) {
+    if constexpr (NP == 1) {
+        const FFACoordFreq r = a.coords1f[k];
+        it = r.idx;
+        ih = r.idx;
+        st = 0;
+        sh = shift_int(r.shift, nb);
+    } else {
+        const FFACoord r = a.coords1[k];
+        it = r.i_tail;
+        ih = r.i_head;
+        st = shift_int(r.shift_tail, nb);
+        sh = shift_int(r.shift_head, nb);
+    }
+}
+
+/// max_j (hi_j - ps[j]) for the fixed width W, hi_j = ps[j + W] or, past the
+/// end of the row, ext[j + W - NB] = ps[j + W - NB] + sum.
+template <int NB, int W>
+__device__ __forceinline__ float window_max_fixed(const float (&ps)[NB],
+                                                  const float (&ext)[NB - 1]) {
+    float ma = -FLT_MAX, mb = -FLT_MAX;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        const float hi = (j + W < NB) ? ps[(j + W) % NB] : ext[(j + W - NB) % (NB - 1)];
+        const float d  = hi - ps[j];
+        if ((j & 2) == 0) {
+            ma = __builtin_fmaxf(ma, d);
+        } else {
+            mb = __builtin_fmaxf(mb, d);
+        }
+    }
+    return __builtin_fmaxf(ma, mb);
+}
+
+template <int NB, int... Ws>
+__device__ __forceinline__ float window_max_reg(std::integer_sequence<int, Ws...>,
+                                                const float (&ps)[NB],
+                                                const float (&ext)[NB - 1], uint32_t w) {
+    float r = 0.0F;
+    (void)((w == static_cast<uint32_t>(Ws + 1) ? (r = window_max_fixed<NB, Ws + 1>(ps, ext), true)
+                                               : false) ||
+           ...);
+    return r;
+}
+
+template <bool EXACT>
+__device__ __forceinline__ float4 normalise4v(const float4 e, const float4 v) {
+    if constexpr (EXACT) {
+        return normalise4_exact(e, v);
+    } else {
+        float4 p;
+        p.x = normalise<false>(e.x, v.x);
+        p.y = normalise<false>(e.y, v.y);
+        p.z = normalise<false>(e.z, v.z);
+        p.w = normalise<false>(e.w, v.w);
+        return p;
+    }
+}
+
+template <int NB, int NP, bool FINDMAX, bool EXACT, typename TI>
+__global__ void __launch_bounds__(kBlock)
+k_final2(Final2Args a) {
+    constexpr uint32_t C4     = NB / 4;           // 4-bin chunks per row
+    constexpr uint32_t STRIDE = NB + kPad;        // (e, v) pairs per staged row
+    extern __shared__ float2 s_rows[];            // [2 * cap][STRIDE]
+    __shared__ uint32_t s_rng[4];                 // lo_t, hi_t, lo_h, hi_h
+
+    const uint32_t wave = threadIdx.x / kLanes;  // row of the chunk
+    const uint32_t lane = threadIdx.x % kLanes;  // frequency of the tile
+    const uint32_t nthr = blockDim.x;
+    const uint32_t id   = xcd_remap(blockIdx.x, gridDim.x);
+    // frequency-tile-major over the chunks: concurrently running blocks cover
+    // every row chunk at neighbouring frequencies, so the grandparent rows
+    // they share stay in L2
+    const uint32_t tile  = id / a.n_chunks;
+    const uint32_t chunk = id % a.n_chunks;
+    const uint32_t row0  = a.chunks[chunk].row0;
+    const uint32_t nrows = a.chunks[chunk].nrows;
+    const uint64_t jf64  = static_cast<uint64_t>(tile) * kLanes + lane;
+    const uint32_t n_f   = a.lv.n_cur[2];
+    const bool active    = wave < nrows && jf64 < n_f;
+    const uint32_t row   = row0 + wave;
+    const uint32_t jf    = static_cast<uint32_t>(jf64);
+
+    // 0. this lane's trial
+    FFACoord cd{0, 0.0F, 0, 0.0F};
+    if (active) {
+        cd = resolve_pair_rc<NP>(a.lv, a.fg.dv, row, jf);
+    }
+    if (threadIdx.x < 4) {
+        s_rng[threadIdx.x] = (threadIdx.x % 2 == 0) ? 0xFFFFFFFFU : 0U;
+    }
+    __syncthreads();
+    {
+        const uint32_t lt = wave_min(active ? cd.i_tail : 0xFFFFFFFFU);
+        const uint32_t ht = wave_max(active ? cd.i_tail : 0U);
+        const uint32_t lh = wave_min(active ? cd.i_head : 0xFFFFFFFFU);
+        const uint32_t hh = wave_max(active ? cd.i_head : 0U);
+        if (lane == 0 && lt != 0xFFFFFFFFU) {
+            atomicMin(&s_rng[0], lt);
+            atomicMax(&s_rng[1], ht);
+            atomicMin(&s_rng[2], lh);
+            atomicMax(&s_rng[3], hh);
+        }
+    }
+    __syncthreads();
+    const uint32_t lo_t = s_rng[0];
+    const uint32_t lo_h = s_rng[2];
+    if (lo_t == 0xFFFFFFFFU) {
+        return;  // no trial in this block (uniform: every wave returns)
+    }
+    const uint32_t n_t = s_rng[1] - lo_t + 1;
+    const uint32_t n_h = s_rng[3] - lo_h + 1;
+    if (n_t > a.cap || n_h > a.cap) {
+        // cannot happen: cap is the exact maximum over all blocks
+        // (final2_ranges); never index past the staged rows
+        if (active) {
+            const uint64_t c_idx = static_cast<uint64_t>(row) * n_f + jf;
+            if (a.fo.count == nullptr) {
+                if constexpr (FINDMAX) {
+                    a.scores[c_idx] = __builtin_nanf("");
+                } else {
+                    for (uint32_t iw = 0; iw < a.wt.n; ++iw) {
+                        a.scores[c_idx * a.wt.n + iw] = __builtin_nanf("");
+                    }
+                }
+            }
+        }
+        return;
+    }
+
+    // 1. stage the parent rows: task (row slot, 4-bin chunk)
+    {
+        const uint32_t ntask = (n_t + n_h) * C4;
+        for (uint32_t t = threadIdx.x; t < ntask; t += nthr) {
+            const uint32_t slot = t / C4;
+            const uint32_t q4   = t % C4;
+            const uint32_t side = slot < n_t ? 0U : 1U;
+            const uint32_t k    = side == 0 ? lo_t + slot : lo_h + (slot - n_t);
+            float2* dst         = s_rows + (side * a.cap + (side == 0 ? slot : slot - n_t)) * STRIDE;
+            using CI            = chunk4_t<TI>;
+            const TI* in        = static_cast<const TI*>(a.in);
+            float4 e, v;
+            uint32_t b0 = 4 * q4;
+            if (!a.grand) {
+                const CI* src = reinterpret_cast<const CI*>(in + ((side * a.n_p1) + k) * (2 * NB));
+                e = widen4<TI>(src[q4]);
+                v = widen4<TI>(src[C4 + q4]);
+            } else {
+                uint32_t it, st, ih, sh;
+                parent_record<NP>(a, k, it, st, ih, sh, NB);
+                const CI* trow = reinterpret_cast<const CI*>(
+                    in + (((2 * side) * a.n_p2) + it) * (2 * NB));
+                const CI* hrow = reinterpret_cast<const CI*>(
+                    in + (((2 * side + 1) * a.n_p2) + ih) * (2 * NB));
+                // parent[j] = tail[j - St] + head[j - Sh]: the tail is read
+                // unrotated and the sums placed at j = 4*q4 + i + St, so only
+                // the head is rotated (by Sh - St)
+                const uint32_t d  = wrap<NB>(sh + NB - st);
+                const float4 te   = widen4<TI>(trow[q4]);
+                const float4 tv   = widen4<TI>(trow[C4 + q4]);
+                const float4 he   = rotated4_st<NB, TI>(hrow, q4, d);
+                const float4 hv   = rotated4_st<NB, TI>(hrow + C4, q4, d);
+                e.x = te.x + he.x;
+                e.y = te.y + he.y;
+                e.z = te.z + he.z;
+                e.w = te.w + he.w;
+                v.x = tv.x + hv.x;
+                v.y = tv.y + hv.y;
+                v.z = tv.z + hv.z;
+                v.w = tv.w + hv.w;
+                if constexpr (!std::is_same_v<TI, float>) {
+                    e.x = round_stored<TI>(e.x * a.scale1);
+                    e.y = round_stored<TI>(e.y * a.scale1);
+                    e.z = round_stored<TI>(e.z * a.scale1);
+                    e.w = round_stored<TI>(e.w * a.scale1);
+                    v.x = round_stored<TI>(v.x * a.scale1);
+                    v.y = round_stored<TI>(v.y * a.scale1);
+                    v.z = round_stored<TI>(v.z * a.scale1);
+                    v.w = round_stored<TI>(v.w * a.scale1);
+                }
+                b0 += st;
+            }
+            const float ev[4][2] = {{e.x, v.x}, {e.y, v.y}, {e.z, v.z}, {e.w, v.w}};
+#pragma unroll
+            for (uint32_t i = 0; i < 4; ++i) {
+                const uint32_t b = wrap<NB>(b0 + i);
+                const float2 pr  = make_float2(ev[i][0], ev[i][1]);
+                dst[b]           = pr;
+                if (b < kPad) {
+                    dst[NB + b] = pr;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (!active) {
+        return;
+    }
+
+    // 2. merge + normalise + prefix sum of this lane's trial.  `get(c, te,
+    //    tv, he, hv)` returns bins 4c..4c+3 of the rotated tail and head rows.
+    const uint32_t St = NP == 1 ? 0U : shift_int(cd.shift_tail, NB);
+    const uint32_t Sh = shift_int(cd.shift_head, NB);
+    float ps[NB];
+    auto prefix = [&](auto&& get) {
+        float acc = 0.0F;
+#pragma unroll
+        for (uint32_t c = 0; c < C4; ++c) {
+            float4 te, tv, he, hv;
+            get(c, te, tv, he, hv);
+            float4 e, v;
+            e.x = te.x + he.x;
+            e.y = te.y + he.y;
+            e.z = te.z + he.z;
+            e.w = te.w + he.w;
+            v.x = tv.x + hv.x;
+            v.y = tv.y + hv.y;
+            v.z = tv.z + hv.z;
+            v.w = tv.w + hv.w;
+            if constexpr (!std::is_same_v<TI, float>) {
+                e.x *= a.unscale;
+                e.y *= a.unscale;
+                e.z *= a.unscale;
+                e.w *= a.unscale;
+                v.x *= a.unscale;
+                v.y *= a.unscale;
+                v.z *= a.unscale;
+                v.w *= a.unscale;
+            }
+            const float4 p = normalise4v<EXACT>(e, v);
+            acc            = (c == 0) ? p.x : acc + p.x;
+            ps[4 * c]      = acc;
+            acc            = acc + p.y;
+            ps[4 * c + 1]  = acc;
+            acc            = acc + p.z;
+            ps[4 * c + 2]  = acc;
+            acc            = acc + p.w;
+            ps[4 * c + 3]  = acc;
+        }
+    };
+    {
+        const float2* trow = s_rows + (cd.i_tail - lo_t) * STRIDE;
+        const float2* hrow = s_rows + (a.cap + (cd.i_head - lo_h)) * STRIDE;
+        prefix([&](uint32_t c, float4& te, float4& tv, float4& he, float4& hv) {
+            const float2* tp = trow + wrap<NB>(4 * c + NB - St);
+            const float2* hp = hrow + wrap<NB>(4 * c + NB - Sh);
+            const float2 t0 = tp[0], t1 = tp[1], t2 = tp[2], t3 = tp[3];
+            const float2 h0 = hp[0], h1 = hp[1], h2 = hp[2], h3 = hp[3];
+            te = make_float4(t0.x, t1.x, t2.x, t3.x);
+            tv = make_float4(t0.y, t1.y, t2.y, t3.y);
+            he = make_float4(h0.x, h1.x, h2.x, h3.x);
+            hv = make_float4(h0.y, h1.y, h2.y, h3.y);
+        });
+    }
+
+    // 3. boxcar S/N over the widths: circular wrap ext[k] = ps[k] + sum
+    //    (score.cpp:143-156), windows in registers
+    const float sum = ps[NB - 1];
+    float ext[NB - 1];
+#pragma unroll
+    for (int k = 0; k < NB - 1; ++k) {
+        ext[k] = ps[k] + sum;
+    }
+    const uint64_t c_idx = static_cast<uint64_t>(row) * n_f + jf;
+    float best           = -FLT_MAX;
+    for (uint32_t iw = 0; iw < a.wt.n; ++iw) {
+        // keep the windows inside the width loop (hoisted, all NB - 1 would be
+        // formed for every trial)
+        __asm__ volatile("" : "+v"(ps[0]));
+        const float dmax =
+            window_max_reg<NB>(std::make_integer_sequence<int, NB - 1>{}, ps, ext, a.wt.w[iw]);
+        const float snr = ((a.wt.h[iw] + a.wt.b[iw]) * dmax) - (a.wt.b[iw] * sum);
+        if constexpr (FINDMAX) {
+            best = (best < snr) ? snr : best;
+        } else if (a.fo.count != nullptr) {
+            filter_emit(a.fo, snr > a.fo.threshold, static_cast<uint32_t>(c_idx * a.wt.n + iw),
+                        snr);
+        } else {
+            a.scores[c_idx * a.wt.n + iw] = snr;
+        }
+    }
+    if constexpr (FINDMAX) {
+        if (a.fo.count != nullptr) {
+            filter_emit(a.fo, best > a.fo.threshold, static_cast<uint32_t>(c_idx), best);
+        } else {
+            a.scores[c_idx] = best;
+        }
+    }
+}
+
+/// Widest parent range (rows per side) over every block of the launch: the
+/// parent index is non-decreasing in the frequency along a grid row, so a
+/// block's range is spanned by the first and last frequency of its rows.
+template <int NP>
+__global__ void __launch_bounds__(256)
+k_final2_ranges(Final2Args a, uint32_t nblocks, uint32_t* out) {
+    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= nblocks) {
+        return;
+    }
+    const uint32_t tile  = id / a.n_chunks;
+    const uint32_t chunk = id % a.n_chunks;
+    const uint32_t n_f   = a.lv.n_cur[2];
+    const uint32_t j0    = tile * kLanes;
+    const uint32_t j1    = min(n_f, j0 + kLanes) - 1;
+    uint32_t lt = 0xFFFFFFFFU, ht = 0, lh = 0xFFFFFFFFU, hh = 0;
+    for (uint32_t r = 0; r < a.chunks[chunk].nrows; ++r) {
+        const uint32_t row = a.chunks[chunk].row0 + r;
+        const FFACoord c0  = resolve_pair_rc<NP>(a.lv, a.fg.dv, row, j0);
+        const FFACoord c1  = resolve_pair_rc<NP>(a.lv, a.fg.dv, row, j1);
+        lt = min(lt, min(c0.i_tail, c1.i_tail));
+        ht = max(ht, max(c0.i_tail, c1.i_tail));
+        lh = min(lh, min(c0.i_head, c1.i_head));
+        hh = max(hh, max(c0.i_head, c1.i_head));
+    }
+    const uint32_t m = max(ht - lt + 1, hh - lh + 1);
+    atomicMax(out, m);
+}
+
+template <int NB, int NP, typename TI>
+void launch3(const Final2Args& a, uint32_t nblocks, bool find_max, bool exact, hipStream_t s) {
+    const size_t lds     = static_cast<size_t>(2) * a.cap * (NB + kPad) * sizeof(float2);
+    const uint32_t block = a.rows_per_block * kLanes;
+    if (find_max && exact) {
+        k_final2<NB, NP, true, true, TI><<<nblocks, block, lds, s>>>(a);
+    } else if (find_max) {
+        k_final2<NB, NP, true, false, TI><<<nblocks, block, lds, s>>>(a);
+    } else if (exact) {
+        k_final2<NB, NP, false, true, TI><<<nblocks, block, lds, s>>>(a);
+    } else {
+        k_final2<NB, NP, false, false, TI><<<nblocks, block, lds, s>>>(a);
+    }
+}
+
+template <int NB, typename TI>
+void launch2(const Final2Args& a, uint32_t nblocks, bool find_max, bool exact, hipStream_t s) {
+    switch (a.lv.np) {
+    case 1: return launch3<NB, 1, TI>(a, nblocks, find_max, exact, s);
+    case 2: return launch3<NB, 2, TI>(a, nblocks, find_max, exact, s);
+    case 3: return launch3<NB, 3, TI>(a, nblocks, find_max, exact, s);
+    default: LOKI_REQUIRE(false, "final-level kernel: np must be 1, 2 or 3");
+    }
+}
+
+template <typename TI>
+void launch1(const Final2Args& a, uint32_t nblocks, bool find_max, bool exact, hipStream_t s) {
+    switch (a.lv.nbins) {
+    case 8: return launch2<8, TI>(a, nblocks, find_max, exact, s);
+    case 16: return launch2<16, TI>(a, nblocks, find_max, exact, s);
+    case 32: return launch2<32, TI>(a, nblocks, find_max, exact, s);
+    default: LOKI_REQUIRE(false, "final-level kernel: nbins must be 8, 16 or 32");
+    }
+}
+
+}  // namespace
+
+void final2_ranges(const Final2Args& a, uint32_t nblocks, uint32_t* out, hipStream_t s) {
+    LOKI_REQUIRE(out != nullptr && a.chunks != nullptr && a.n_chunks > 0, "null argument");
+    if (nblocks == 0) {
+        return;
+    }
+    const uint32_t grid = (nblocks + 255) / 256;
+    switch (a.lv.np) {
+    case 1: k_final2_ranges<1><<<grid, 256, 0, s>>>(a, nblocks, out); break;
+    case 2: k_final2_ranges<2><<<grid, 256, 0, s>>>(a, nblocks, out); break;
+    case 3: k_final2_ranges<3><<<grid, 256, 0, s>>>(a, nblocks, out); break;
+    default: LOKI_REQUIRE(false, "final-level kernel: np must be 1, 2 or 3");
+    }
+}
+
+bool final2_supported(uint32_t nbins) { return nbins == 8 || nbins == 16 || nbins == 32; }
+
+uint32_t final2_lds_bytes(uint32_t nbins, uint32_t cap) {
+    return 2U * cap * (nbins + kPad) * static_cast<uint32_t>(sizeof(float2));
+}
+
+void final2_merge_score(const Final2Args& a, FoldStorage si, uint32_t nblocks, bool find_max,
+                        bool exact, hipStream_t s) {
+    LOKI_REQUIRE(final2_supported(a.lv.nbins), "final-level kernel: unsupported nbins");
+    LOKI_REQUIRE(a.wt.wmax < a.lv.nbins, "boxcar widths must be < nbins");
+    LOKI_REQUIRE(a.rows_per_block == 1 || a.rows_per_block == 2 || a.rows_per_block == 4,
+                 "final-level kernel: rows per block must be 1, 2 or 4");
+    LOKI_REQUIRE(a.n_chunks > 0 && a.chunks != nullptr && a.cap > 0,
+                 "final-level kernel: no row chunks");
+    if (a.grand) {
+        LOKI_REQUIRE(a.lv.np == 1 ? a.coords1f != nullptr : a.coords1 != nullptr,
+                     "final-level kernel: parent shift table missing");
+    }
+    LOKI_REQUIRE(final2_lds_bytes(a.lv.nbins, a.cap) <= 64U * 1024U,
+                 "final-level kernel: parent ranges exceed the LDS budget");
+    if (nblocks == 0) {
+        return;
+    }
+    switch (si) {
+    case FoldStorage::kF32: return launch1<float>(a, nblocks, find_max, exact, s);
+    case FoldStorage::kF16: return launch1<_Float16>(a, nblocks, find_max, exact, s);
+    case FoldStorage::kBF16: return launch1<__bf16>(a, nblocks, find_max, exact, s);
+    }
+}
+
+}  // namespace loki_amd::kern

@@ -164,6 +164,48 @@ void final_merge_score(const void* in, FoldStorage si, float unscale, uint64_t n
                        float* scores, bool find_max, bool exact, hipStream_t s,
                        const FilterOut* fo = nullptr);
 
+/// Chunk of final grid rows (flattened (jerk, accel) cells) whose trials share
+/// their parent rows: rows [row0, row0 + nrows), nrows <= rows_per_block.
+struct RowChunk {
+    uint32_t row0;
+    uint32_t nrows;
+};
+/// Arguments of the one-lane-per-trial fused last level (k_final2.hip).
+/// Parent level F-1: [2, n_p1, 2, nbins] stored as `si`.  Without `grand`,
+/// `in` is that level; with it, `in` is the grandparent level F-2
+/// [4, n_p2, 2, nbins] and the parent rows are merged on chip through the
+/// parent level's shift table (coords1 / coords1f), scaled by scale1 and
+/// rounded to the storage type like a stored level.
+struct Final2Args {
+    const void* in;
+    const FFACoord* coords1;
+    const FFACoordFreq* coords1f;
+    uint64_t n_p1;
+    uint64_t n_p2;
+    ResolveLevel lv;
+    FinalGrid fg;
+    WidthTable wt;
+    float* scores;
+    FilterOut fo;
+    const RowChunk* chunks;  ///< device
+    uint32_t n_chunks;
+    uint32_t rows_per_block;  ///< 1, 2 or 4 waves per block
+    uint32_t cap;             ///< staged parent rows per side
+    uint32_t grand;
+    float unscale;            ///< reduced precision: final sums x 2^(k0 + F - 1)
+    float scale1;             ///< reduced precision: parent merge scale (0.5)
+};
+/// nbins supported by final2_merge_score (register windows).
+bool final2_supported(uint32_t nbins);
+/// LDS bytes of one block staging `cap` rows per side.
+uint32_t final2_lds_bytes(uint32_t nbins, uint32_t cap);
+/// out (device u32, zeroed by the caller) <- the widest parent range (rows
+/// per side) of any of the nblocks blocks: the `cap` final2_merge_score needs.
+void final2_ranges(const Final2Args& a, uint32_t nblocks, uint32_t* out, hipStream_t s);
+/// Launch over nblocks = n_chunks * frequency tiles of 64 (tile-major).
+void final2_merge_score(const Final2Args& a, FoldStorage si, uint32_t nblocks, bool find_max,
+                        bool exact, hipStream_t s);
+
 // ---------------------------------------------------------------------------
 // Grouped launches: many independent FFAs (a batch of series) advanced by one
 // launch per stage.  Each task owns a contiguous range of blocks starting at

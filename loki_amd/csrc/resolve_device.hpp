@@ -49,28 +49,28 @@ __device__ __forceinline__ void resolve_side(const ResolveLevel& lv, double delt
     }
 }
 
-/// Both sides of coordinate i with the per-coordinate terms evaluated once;
-/// every value is produced by the same operations as resolve_side (same
-/// bits).  Cell centres use the precomputed dv_actual = (max - min) / count
-/// (lv.dv), a single IEEE division identical to psr_utils.hpp:290-292.
+/// Both sides of the coordinate in grid row `row` (flattened (jerk, accel)
+/// cell) and frequency index jf, with the per-coordinate terms evaluated once;
+/// every value is produced by the same operations as resolve_side (same bits).
+/// Cell centres use the precomputed dv_actual = (max - min) / count (dv), a
+/// single IEEE division identical to psr_utils.hpp:290-292.  Host and device.
 template <int NP>
-__device__ __forceinline__ FFACoord resolve_pair(const ResolveLevel& lv, const double* dv,
-                                                 uint64_t i) {
+LOKI_HD inline FFACoord resolve_pair_rc(const ResolveLevel& lv, const double* dv, uint32_t row,
+                                        uint32_t jf) {
     FFACoord c;
     const uint32_t n_f = lv.n_cur[2];
-    const uint32_t jf  = static_cast<uint32_t>(i % n_f);
     const double f_cur = (n_f <= 1) ? (lv.hi[2] + lv.lo[2]) / 2.0
                                     : lv.lo[2] + (dv[2] * (static_cast<double>(jf) + 0.5));
     if constexpr (NP == 1) {
+        (void)row;
         c.i_tail     = grid::nearest_idx(f_cur, lv.lo[2], lv.hi[2], lv.n_prev[2]);
         c.i_head     = c.i_tail;
         c.shift_tail = 0.0F;
         c.shift_head = grid::phase_idx(lv.tseg_prev, f_cur, lv.nbins, 0.0);
     } else {
         const uint32_t n_a = lv.n_cur[1];
-        const uint64_t r   = i / n_f;
-        const uint32_t ia  = static_cast<uint32_t>(r % n_a);
-        const uint32_t ij  = static_cast<uint32_t>(r / n_a);
+        const uint32_t ia  = row % n_a;
+        const uint32_t ij  = row / n_a;
         const double a_cur = (n_a <= 1) ? (lv.hi[1] + lv.lo[1]) / 2.0
                                         : lv.lo[1] + (dv[1] * (static_cast<double>(ia) + 0.5));
 #pragma unroll
@@ -79,6 +79,7 @@ __device__ __forceinline__ FFACoord resolve_pair(const ResolveLevel& lv, const d
             uint32_t idx;
             float shift;
             if constexpr (NP == 2) {
+                (void)ij;
                 const double half_delta_t_sq = 0.5 * delta_t * delta_t;
                 const double v_new           = a_cur * delta_t;
                 const double d_new           = a_cur * half_delta_t_sq;
@@ -120,6 +121,15 @@ __device__ __forceinline__ FFACoord resolve_pair(const ResolveLevel& lv, const d
         }
     }
     return c;
+}
+
+/// resolve_pair_rc of flat coordinate i = row * n_f + jf.
+template <int NP>
+__device__ __forceinline__ FFACoord resolve_pair(const ResolveLevel& lv, const double* dv,
+                                                 uint64_t i) {
+    const uint32_t n_f = lv.n_cur[2];
+    return resolve_pair_rc<NP>(lv, dv, static_cast<uint32_t>(i / n_f),
+                               static_cast<uint32_t>(i % n_f));
 }
 
 /// Full merge record of coordinate i of a level (NP == 1: freq-only, stored

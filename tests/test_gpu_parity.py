@@ -28,6 +28,14 @@ CASES = {
     "acc_eta2_bseg": (2**18, 16, [(-20000.0, 20000.0), (30.0, 40.0)],
                       {"eta": 2.0, "bseg_brute": 2048}),
     "acc17_nb128": (2**17, 128, [(-60000.0, 60000.0), (60.0, 64.0)], {}),
+    # the one-lane-per-trial fused kernels (nbins 8, 16, 32): one grid row, 36 rows in
+    # (jerk, accel), and trees of 2 and 3 levels (the parent level is the brute fold, or is
+    # merged on chip from it)
+    "freq17_nb32": (2**17, 32, [(20.0, 30.0)], {}),
+    "acc16_nb8": (2**16, 8, [(-300000.0, 300000.0), (200.0, 230.0)], {}),
+    "jerk18_nb16": (2**18, 16, [(-1e5, 1e5), (-3e4, 3e4), (40.0, 44.0)], {}),
+    "acc_2lvl": (2**16, 32, [(-60000.0, 60000.0), (100.0, 110.0)], {"bseg_brute": 2**15}),
+    "acc_3lvl": (2**16, 32, [(-60000.0, 60000.0), (100.0, 110.0)], {"bseg_brute": 2**14}),
 }
 
 
@@ -167,19 +175,24 @@ def test_ffa_scores_bit_exact(oracle, name):
 
 
 @pytest.mark.parametrize("name", ["freq16", "acc17w", "acc20", "jerk17w", "jerk20",
-                                  "freq18_nb100", "acc_eta2_bseg", "acc17_nb128"])
-@pytest.mark.parametrize("fused", [True, False])
+                                  "freq18_nb100", "acc_eta2_bseg", "acc17_nb128", "acc17_nb48",
+                                  "freq17_nb32", "acc16_nb8", "jerk18_nb16", "acc_2lvl",
+                                  "acc_3lvl"])
+@pytest.mark.parametrize("mode", ["unfused", "depth1", "depth2"])
 @pytest.mark.parametrize("find_max", [False, True])
-def test_scores_fused_and_unfused_paths(oracle, name, fused, find_max):
-    """The fused last-level kernel (FP64 shift tables + merge + S/N, no final fold in HBM) and
-    the unfused merge + snr_boxcar_3d give the oracle's scores bit for bit."""
+def test_scores_fused_and_unfused_paths(oracle, name, mode, find_max):
+    """The fused last-level kernels (FP64 shift tables + merge + S/N, no final fold in HBM;
+    depth 2 also merges the parent level on chip) and the unfused merge + snr_boxcar_3d give
+    the oracle's scores bit for bit."""
     from loki_amd import libculoki
 
     cfg, ocfg = _cfgs(name)
     ts_e, ts_v = _series(cfg.nsamps, seed=17)
     widths = np.array([w for w in (1, 2, 3, 5, 8, 13) if w < cfg.nbins], dtype=np.uint64)
     ffa = libculoki.ffa.FFACUDA(cfg, device_id=0)
-    ffa.set_fused(fused)
+    ffa.set_fused(mode != "unfused")
+    if mode != "unfused":
+        ffa.set_fuse_depth(2 if mode == "depth2" else 1)
     got = ffa.scores(ts_e, ts_v, widths, find_max=find_max)
     want, _ = oracle.compute_ffa_scores(ts_e, ts_v, ocfg, widths=widths)
     want = want.reshape(-1, widths.size)
@@ -307,9 +320,10 @@ def test_reduced_storage_fold_bit_exact(oracle, name, storage):
 
 
 @pytest.mark.parametrize("storage", ["f16", "bf16"])
-@pytest.mark.parametrize("name", ["freq16", "acc17w", "acc20", "jerk17w", "jerk20"])
-@pytest.mark.parametrize("fused", [True, False])
-def test_reduced_storage_scores_bit_exact(oracle, name, storage, fused):
+@pytest.mark.parametrize("name", ["freq16", "acc17w", "acc20", "jerk17w", "jerk20",
+                                  "acc_eta2_bseg", "freq17_nb32", "jerk18_nb16", "acc_3lvl"])
+@pytest.mark.parametrize("mode", ["unfused", "depth1", "depth2"])
+def test_reduced_storage_scores_bit_exact(oracle, name, storage, mode):
     from loki_amd import libculoki
 
     cfg, ocfg = _cfgs(name)
@@ -317,7 +331,9 @@ def test_reduced_storage_scores_bit_exact(oracle, name, storage, fused):
     widths = np.array([w for w in (1, 2, 3, 4, 6, 8, 12, 16, 24) if w < cfg.nbins], np.uint64)
     ffa = libculoki.ffa.FFACUDA(cfg, device_id=0)
     ffa.set_fold_storage(storage)
-    ffa.set_fused(fused)
+    ffa.set_fused(mode != "unfused")
+    if mode != "unfused":
+        ffa.set_fuse_depth(2 if mode == "depth2" else 1)
     got = ffa.scores(ts_e, ts_v, widths)
     want, _ = oracle.compute_ffa_scores(ts_e, ts_v, ocfg, widths=widths, storage=storage)
     np.testing.assert_array_equal(got, want)
