@@ -30,6 +30,11 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBPS = 8000.0
 MFMA_F32_PEAK_TFLOPS = 157.3
+# VALU issue: 256 CUs x 4 SIMDs, a wave64 instruction every 4 cycles (16 lanes per cycle) at
+# 2.4 GHz -> 39.3e12 lane-instructions/s (FP32 FMA: 78.6 TFLOP/s, packed FP32 off)
+SIMDS = 1024
+CLOCK_HZ = 2.4e9
+VALU_LANE_INST_PEAK_T = SIMDS * 16 * CLOCK_HZ / 1e12
 WIDTHS = (1, 2, 3, 4, 6, 8, 12, 16, 24, 32)
 
 WORKLOADS = {
@@ -110,7 +115,11 @@ def parse() -> argparse.Namespace:
                     help="fold storage precision of levels 0..L-2 (adds stay FP32)")
     ap.add_argument("--no-storage-compare", action="store_true",
                     help="skip timing the FP16 / BF16 storage variants beside the main run")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cpu count)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="0: every CPU this process may use (affinity, capped by OMP_NUM_THREADS)")
+    ap.add_argument("--fuse-depth", type=int, default=2, choices=[1, 2],
+                    help="levels fused into the scoring kernel (2: the parent level of the last "
+                         "is merged on chip too)")
     ap.add_argument("--no-score-filter", action="store_true",
                     help="skip the score-and-filter timing")
     ap.add_argument("--filter-threshold", type=float, default=6.0,
@@ -193,6 +202,37 @@ def load_traffic(workload: str, kernel: str):
         return None
 
 
+def cpu_share():
+    """Threads of the CPU baseline: every CPU this process may run on (its affinity mask),
+    capped by OMP_NUM_THREADS when the environment sets it (the host's share for this GPU);
+    plus the host facts the baseline is reported with."""
+    aff = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    n = min(aff, int(omp)) if omp.isdigit() and int(omp) > 0 else aff
+    model = None
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return n, {"host_cpus": os.cpu_count(), "affinity_cpus": aff,
+               "omp_num_threads": omp or None, "cpu_model": model}
+
+
+def load_counters(workload: str, kernel: str):
+    """Per-launch SQ counters of a kernel measured with rocprofv3 --pmc
+    (profiles/valu_counters.json)."""
+    path = ROOT / "profiles" / "valu_counters.json"
+    if not path.exists():
+        return None
+    try:
+        return json.loads(path.read_text()).get(workload, {}).get(kernel)
+    except (OSError, ValueError):
+        return None
+
+
 def cpu_baseline(wl: dict, ts_e, ts_v, widths, threads: int) -> dict:
     """The CPU oracle (C/OpenMP restatement of the reference CPU FFA) on a bounded sample."""
     from oracle import oracle as orc
@@ -210,15 +250,17 @@ def cpu_baseline(wl: dict, ts_e, ts_v, widths, threads: int) -> dict:
     lims = ", ".join(f"[{a:g}, {b:g}]" for a, b in smp["param_limits"])
     note = (" (FP64 DFT brute fold and FP64 phase rotations: slower than an FP32 CPU path)"
             if fourier else "")
+    _, host = cpu_share()
     return {
         "value": nc / dt, "unit": "param trials/s", "cores": threads, "kind": "port",
         "sample": f"same series and grid rules, param_limits {lims}, bseg_brute "
                   f"{ocfg.bseg_brute}: {nc} final trials x {len(widths)} widths in {dt:.2f} s"
                   + note,
+        **host,
     }
 
 
-VALU_F32_PEAK_TFLOPS = 157.3
+VALU_F32_PEAK_TFLOPS = 78.6  # non-packed FP32 FMA rate (this build disables packed FP32)
 REFINE_FLOPS_PER_TERM = 12.0  # three-term power recurrence (2 FMA) + 4 accumulating FMAs
 
 
@@ -348,8 +390,7 @@ def storage_error_vs_fp64(wl, ts_e, ts_v, widths) -> dict:
     ocfg = SearchConfig(wl["nsamps"], wl["tsamp"], nbins, wl["eta"], smp["param_limits"])
     orc.lib()
     t0 = time.perf_counter()
-    fold64, _ = orc.compute_ffa(ts_e, ts_v, ocfg, f64=True,
-                                nthreads=min(16, os.cpu_count() or 1))
+    fold64, _ = orc.compute_ffa(ts_e, ts_v, ocfg, f64=True, nthreads=cpu_share()[0])
     snr64 = orc.snr_boxcar_3d_f64(fold64, widths, nbins).reshape(-1)
     del fold64
     cpu_s = time.perf_counter() - t0
@@ -510,7 +551,7 @@ def run_batch(args, wl) -> int:
         from oracle import oracle as orc
         from oracle.plan import Plan, SearchConfig
 
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads or cpu_share()[0]
         sub = specs[: wl["cpu_sample"]["nseries"]]
         ntr, dt = 0, 0.0
         orc.lib()
@@ -527,6 +568,7 @@ def run_batch(args, wl) -> int:
             "value": ntr / dt, "unit": "param trials/s", "cores": threads, "kind": "port",
             "sample": f"the first {len(sub)} series of the batch, one after another: {ntr} final "
                       f"trials in {dt:.2f} s",
+            **cpu_share()[1],
         }
     print(json.dumps(result), flush=True)
     return 0
@@ -615,6 +657,69 @@ def run_seeds(dev, stream, steps: int) -> dict:
     return res
 
 
+def stored_level_bytes(plan, nbins, level, storage):
+    """Bytes of fold level `level` as stored (time domain)."""
+    s = 4 if storage == "f32" else 2
+    shape = [int(x) for x in plan.fold_shapes[level]]
+    return int(np.prod(shape)) * s
+
+
+def fused_roofline(plan, nbins, nw, args, wl, avg_ms, dom_name, variant, traffic):
+    """Roofline of the fused last level.  It is VALU-issue bound: `frac` is its VALU issue
+    utilisation, SQ_INSTS_VALU per launch (rocprofv3 --pmc, profiles/valu_counters.json) x 64
+    lanes over this run's average launch time, against 1024 SIMDs x 16 lanes x 2.4 GHz.  Its
+    HBM side is reported beside it: the minimum bytes (the stored level it reads, once, plus
+    the scores) and the measured traffic, each over the same time."""
+    nl = int(plan.n_levels)
+    depth = args.fuse_depth if nbins in (8, 16, 32) and nl >= 3 else 1
+    src = nl - 1 - depth
+    nc = int(plan.ncoords[-1])
+    min_bytes = stored_level_bytes(plan, nbins, src, args.storage) + nc * nw * 4
+    t = avg_ms * 1e-3
+    hbm = {"min_bytes": min_bytes, "min_GBps": min_bytes / t / 1e9,
+           "min_frac": min_bytes / t / 1e9 / HBM_PEAK_GBPS,
+           "reads_level": src, "traffic": traffic,
+           "traffic_frac": (traffic / t / 1e9 / HBM_PEAK_GBPS) if traffic else None,
+           "traffic_source": "profiles/hbm_traffic.json (FETCH_SIZE x 2 + WRITE_SIZE)"
+                             if traffic else None}
+    ctr = load_counters(args.workload, variant)
+    if ctr and ctr.get("SQ_INSTS_VALU"):
+        lane_inst = float(ctr["SQ_INSTS_VALU"]) * 64
+        achieved = lane_inst / t / 1e12
+        return {"bound": "valu", "kernel": dom_name, "achieved": achieved,
+                "peak": VALU_LANE_INST_PEAK_T, "unit": "T lane-instructions/s",
+                "frac": achieved / VALU_LANE_INST_PEAK_T, "traffic": traffic,
+                "avg_ms": avg_ms, "valu_lane_inst_per_launch": lane_inst,
+                "valu_lane_inst_per_trial": lane_inst / nc,
+                "counters": {k: v for k, v in ctr.items()},
+                "counters_source": "profiles/valu_counters.json", "hbm": hbm}
+    # without a counter profile for this variant: the HBM view on minimum bytes
+    return {"bound": "hbm", "kernel": dom_name, "achieved": hbm["min_GBps"],
+            "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": hbm["min_frac"], "traffic": traffic,
+            "avg_ms": avg_ms, "alg_bytes_per_launch": min_bytes,
+            "note": "fused last level: minimum bytes (stored level read once + scores)",
+            "hbm": hbm}
+
+
+def step_floor(plan, nbins, nw, nsamps, args, ms_per_step, fourier):
+    """HBM floor of one step: the input series read once, every level that goes through HBM
+    written once and read once, the scores written -- against the measured step."""
+    if fourier:
+        return None
+    nl = int(plan.n_levels)
+    depth = args.fuse_depth if nbins in (8, 16, 32) and nl >= 3 else 1
+    if args.unfused:
+        depth = 0
+    top = nl - 1 - depth
+    b = 2 * nsamps * 4 + int(plan.ncoords[-1]) * nw * 4
+    for lvl in range(0, top + 1):
+        st = args.storage if lvl < nl - 1 else "f32"
+        b += 2 * stored_level_bytes(plan, nbins, lvl, st)
+    floor_ms = b / (HBM_PEAK_GBPS * 1e9) * 1e3
+    return {"min_bytes": b, "floor_ms": floor_ms, "frac": floor_ms / ms_per_step,
+            "levels_through_hbm": top + 1}
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -663,6 +768,7 @@ def main() -> int:
         ffa = libculoki.ffa.FFACUDA(cfg, device_id=0)
         ffa.set_exact_norm(not args.fast_norm)
         ffa.set_fused(not args.unfused)
+        ffa.set_fuse_depth(args.fuse_depth)
         ffa.set_fold_storage(args.storage)
     plan = ffa.plan
     nc = int(plan.ncoords[-1])
@@ -681,19 +787,26 @@ def main() -> int:
             step()
     torch.cuda.synchronize()
 
-    ffa.set_timing(True)
-    per_kernel = defaultdict(list)
+    # timed region: exactly `steps` steps, no per-kernel events
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     with torch.cuda.stream(stream):
         for _ in range(args.steps):
             step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+
+    # per-kernel durations: hipEvent pairs on the launch stream, separate steps
+    ffa.set_timing(True)
+    per_kernel = defaultdict(list)
+    with torch.cuda.stream(stream):
+        for _ in range(min(args.steps, 10)):
+            step()
             for kind, level, ms in ffa.timings():  # waits for this step's events
                 per_kernel[(kind, level)].append(ms)
     torch.cuda.synchronize()
-    t1 = time.perf_counter()
     ffa.set_timing(False)
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
 
     # The same step with the other fold storage precisions (same series, same stream).
     storage_cmp = None
@@ -768,7 +881,10 @@ def main() -> int:
     avg = {k: float(np.mean(v)) for k, v in per_kernel.items()}
     dom = max(avg, key=avg.get)
     dom_name = f"{dom[0]}_L{dom[1]}"
-    traffic = load_traffic(args.workload, dom_name)
+    variant = dom_name if dom[0] != "merge_score" else (
+        f"{dom_name}_depth{args.fuse_depth}_{args.storage}"
+        f"{'' if not args.fast_norm else '_hwrsqrt'}")
+    traffic = load_traffic(args.workload, variant)
     if fourier and dom[0] == "brute_fold":
         # the DFT brute fold runs on the matrix cores: FP32 MFMA roofline
         work = dft_flops(plan, nbins, wl["nsamps"])
@@ -777,20 +893,14 @@ def main() -> int:
                 "unit": "TFLOP/s", "traffic": traffic, "flops_per_launch": work,
                 "avg_ms": avg[dom]}
         roof["frac"] = roof["achieved"] / MFMA_F32_PEAK_TFLOPS
+    elif dom[0] == "merge_score":
+        roof = fused_roofline(plan, nbins, nw, args, wl, avg[dom], dom_name, variant, traffic)
     else:
         dom_bytes = alg_bytes(plan, nbins, nw, dom[0], dom[1], wl["nsamps"], fourier)
         achieved = dom_bytes / (avg[dom] * 1e-3) / 1e9
         roof = {"bound": "hbm", "kernel": dom_name, "achieved": achieved,
                 "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
                 "traffic": traffic, "alg_bytes_per_launch": dom_bytes, "avg_ms": avg[dom]}
-        if dom[0] == "merge_score":
-            # the fused last level never writes or re-reads the final fold that its
-            # algorithmic bytes count, so frac can exceed 1; its real limit is VALU issue
-            roof["note"] = ("fused last level: algorithmic bytes of the unfused merge + score; "
-                            "HBM traffic per launch in 'traffic'; VALU-bound (SQ counters in "
-                            "profiles/)")
-            if traffic:
-                roof["traffic_GBps"] = traffic / (avg[dom] * 1e-3) / 1e9
     kern_ms = {f"{k}_L{l}": round(v, 4) for (k, l), v in sorted(avg.items(), key=lambda x: -x[1])}
     total_alg = pipeline_bytes(plan, nbins, nw, wl["nsamps"], fourier)
 
@@ -820,7 +930,8 @@ def main() -> int:
                             "fused_irfft_score" if fourier else "fused"),
         },
         "roofline": roof,
-        "pipeline_alg_GBps": total_alg / (ms_per_step * 1e-3) / 1e9,
+        "step_floor": step_floor(plan, nbins, nw, wl["nsamps"], args, ms_per_step, fourier),
+        "unfused_alg_bytes": total_alg,
         "kernel_ms": kern_ms,
         "storage_compare": storage_cmp,
         "score_filter": score_filter,
@@ -835,7 +946,7 @@ def main() -> int:
         if not args.no_refine:
             result["refine"] = run_refine(wl, plan, scores, widths, d_e, d_v, injections)
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads or cpu_share()[0]
         result["cpu_baseline"] = cpu_baseline(wl, ts_e, ts_v, widths, threads)
         if "refine" in result:
             result["refine"]["cpu_baseline"] = refine_cpu_baseline(wl, ts_e, ts_v)

@@ -152,6 +152,43 @@ void orc_brute_fold_time_f64(const float* ts_e, const float* ts_v, size_t nsamps
 }
 
 /*
+ * o[j] = t[(j - St) mod n] + h[(j - Sh) mod n] for j < n, one FP32 add per
+ * element in the order of kernels.cpp:71-146; written as up to three
+ * contiguous runs (split at St and Sh) so that the compiler vectorises them.
+ */
+static inline void add_rot_f32(float* restrict o, const float* restrict t, size_t St,
+                               const float* restrict h, size_t Sh, size_t n) {
+    const size_t a = St < Sh ? St : Sh;
+    const size_t b = St < Sh ? Sh : St;
+    const size_t cut[4] = {0, a, b, n};
+    for (int k = 0; k < 3; ++k) {
+        const size_t j0 = cut[k], j1 = cut[k + 1];
+        if (j0 >= j1) continue;
+        const float* tp = t + (j0 < St ? j0 + n - St : j0 - St);
+        const float* hp = h + (j0 < Sh ? j0 + n - Sh : j0 - Sh);
+        float* op = o + j0;
+        const size_t m = j1 - j0;
+        for (size_t j = 0; j < m; ++j) op[j] = tp[j] + hp[j];
+    }
+}
+
+static inline void add_rot_f64(double* restrict o, const double* restrict t, size_t St,
+                               const double* restrict h, size_t Sh, size_t n) {
+    const size_t a = St < Sh ? St : Sh;
+    const size_t b = St < Sh ? Sh : St;
+    const size_t cut[4] = {0, a, b, n};
+    for (int k = 0; k < 3; ++k) {
+        const size_t j0 = cut[k], j1 = cut[k + 1];
+        if (j0 >= j1) continue;
+        const double* tp = t + (j0 < St ? j0 + n - St : j0 - St);
+        const double* hp = h + (j0 < Sh ? j0 + n - Sh : j0 - Sh);
+        double* op = o + j0;
+        const size_t m = j1 - j0;
+        for (size_t j = 0; j < m; ++j) op[j] = tp[j] + hp[j];
+    }
+}
+
+/*
  * Freq-only time merge, kernels.cpp:118-146 (shift_add_linear_with_buffer)
  * and 722-827: out[s,c,q,j] = in[2s,idx,q,j] + in[2s+1,idx,q,(j-S) mod n].
  */
@@ -168,10 +205,7 @@ void orc_ffa_iter_freq(const float* in, float* out, const orc_coord_freq* coords
             float* o = out + s * ncur * row + c * row;
             const size_t S = shift_bins(coords[c].shift, nbins);
             for (size_t q = 0; q < 2; ++q)
-                for (size_t j = 0; j < nbins; ++j) {
-                    const size_t jh = (j < S) ? (j + nbins - S) : (j - S);
-                    o[q * nbins + j] = t[q * nbins + j] + h[q * nbins + jh];
-                }
+                add_rot_f32(o + q * nbins, t + q * nbins, 0, h + q * nbins, S, nbins);
         }
     }
 }
@@ -194,11 +228,7 @@ void orc_ffa_iter_poly(const float* in, float* out, const orc_coord* coords,
             const size_t St = shift_bins(coords[c].shift_tail, nbins);
             const size_t Sh = shift_bins(coords[c].shift_head, nbins);
             for (size_t q = 0; q < 2; ++q)
-                for (size_t j = 0; j < nbins; ++j) {
-                    const size_t jt = (j < St) ? (j + nbins - St) : (j - St);
-                    const size_t jh = (j < Sh) ? (j + nbins - Sh) : (j - Sh);
-                    o[q * nbins + j] = t[q * nbins + jt] + h[q * nbins + jh];
-                }
+                add_rot_f32(o + q * nbins, t + q * nbins, St, h + q * nbins, Sh, nbins);
         }
     }
 }
@@ -217,10 +247,7 @@ void orc_ffa_iter_freq_f64(const double* in, double* out, const orc_coord_freq* 
             double* o = out + s * ncur * row + c * row;
             const size_t S = shift_bins(coords[c].shift, nbins);
             for (size_t q = 0; q < 2; ++q)
-                for (size_t j = 0; j < nbins; ++j) {
-                    const size_t jh = (j < S) ? (j + nbins - S) : (j - S);
-                    o[q * nbins + j] = t[q * nbins + j] + h[q * nbins + jh];
-                }
+                add_rot_f64(o + q * nbins, t + q * nbins, 0, h + q * nbins, S, nbins);
         }
 }
 
@@ -238,11 +265,7 @@ void orc_ffa_iter_poly_f64(const double* in, double* out, const orc_coord* coord
             const size_t St = shift_bins(coords[c].shift_tail, nbins);
             const size_t Sh = shift_bins(coords[c].shift_head, nbins);
             for (size_t q = 0; q < 2; ++q)
-                for (size_t j = 0; j < nbins; ++j) {
-                    const size_t jt = (j < St) ? (j + nbins - St) : (j - St);
-                    const size_t jh = (j < Sh) ? (j + nbins - Sh) : (j - Sh);
-                    o[q * nbins + j] = t[q * nbins + jt] + h[q * nbins + jh];
-                }
+                add_rot_f64(o + q * nbins, t + q * nbins, St, h + q * nbins, Sh, nbins);
         }
 }
 
