@@ -111,8 +111,9 @@ def parse() -> argparse.Namespace:
                     help="hardware rsqrt profile normalisation (S/N within 1e-4, not bit-exact)")
     ap.add_argument("--unfused", action="store_true",
                     help="store the final fold and score it in a separate kernel")
-    ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"],
-                    help="fold storage precision of levels 0..L-2 (adds stay FP32)")
+    ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16", "bfp16"],
+                    help="fold storage precision of levels 0..L-2 (adds stay FP32; bfp16: "
+                         "block floating point, level 0 in FP32)")
     ap.add_argument("--no-storage-compare", action="store_true",
                     help="skip timing the FP16 / BF16 storage variants beside the main run")
     ap.add_argument("--cpu-threads", type=int, default=0,
@@ -402,7 +403,7 @@ def storage_error_vs_fp64(wl, ts_e, ts_v, widths) -> dict:
     out = {"sample": f"param_limits {smp['param_limits']}: {snr64.size // len(widths)} trials "
                      f"x {len(widths)} widths (FP64 oracle fold in {cpu_s:.1f} s)",
            "max_fp64_snr": float(snr64.max())}
-    for st in ("f32", "f16", "bf16"):
+    for st in ("f32", "f16", "bf16", "bfp16"):
         ffa.set_fold_storage(st)
         got = ffa.scores(ts_e, ts_v, widths).astype(np.float64)
         d = np.abs(got - snr64)
@@ -658,10 +659,12 @@ def run_seeds(dev, stream, steps: int) -> dict:
 
 
 def stored_level_bytes(plan, nbins, level, storage):
-    """Bytes of fold level `level` as stored (time domain)."""
-    s = 4 if storage == "f32" else 2
-    shape = [int(x) for x in plan.fold_shapes[level]]
-    return int(np.prod(shape)) * s
+    """Bytes of fold level `level` as stored (time domain).  Block floating point: int16
+    mantissas plus one 16-bit exponent word per row pair, and level 0 in FP32."""
+    n = int(np.prod([int(x) for x in plan.fold_shapes[level]]))
+    if storage == "bfp16":
+        return 4 * n if level == 0 else 2 * n + n // nbins
+    return n * (4 if storage == "f32" else 2)
 
 
 def fused_roofline(plan, nbins, nw, args, wl, avg_ms, dom_name, variant, traffic):
@@ -812,7 +815,7 @@ def main() -> int:
     storage_cmp = None
     if not fourier and not args.no_storage_compare:
         storage_cmp = {args.storage: ms_per_step}
-        for st in ("f32", "f16", "bf16"):
+        for st in ("f32", "f16", "bf16", "bfp16"):
             if st == args.storage:
                 continue
             ffa.set_fold_storage(st)
@@ -833,8 +836,10 @@ def main() -> int:
             "ms_per_step": {k: round(v, 4) for k, v in storage_cmp.items()},
             "speedup_vs_f32": {k: round(storage_cmp["f32"] / v, 4)
                                for k, v in storage_cmp.items()},
-            "note": "levels 0..L-2 stored at that precision, every add in FP32; S/N error vs "
-                    "FP64 is stated by tests/test_gpu_parity.py::test_storage_snr_error_vs_fp64",
+            "note": "levels 0..L-2 stored at that precision (bfp16: block floating point, "
+                    "levels 1..L-2, level 0 in FP32), every add in FP32; S/N error vs FP64 in "
+                    "snr_error_vs_fp64 (jerk workload) and "
+                    "tests/test_gpu_parity.py::test_storage_snr_error_vs_fp64",
         }
 
     # Score-and-filter (the FFAFreqSweep step, ffa_freq_sweep_cuda.cu:219-260): the same FFA with

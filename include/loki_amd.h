@@ -231,15 +231,19 @@ int loki_ffa_set_exact_norm(loki_ffa_t ffa, int exact);
  * Fourier brute fold is FP32, kernels_cuda.cu:780-965). */
 int loki_ffa_set_dft_precision(loki_ffa_t ffa, int mode);
 /* Storage precision of the intermediate fold levels 0..L-2 of a time-domain FFA object:
- * LOKI_STORAGE_F32 (default), LOKI_STORAGE_F16 or LOKI_STORAGE_BF16.  Each add is done in FP32
- * and each stored value is rounded once (round to nearest even); level l is stored at the exact
- * scale 2^-(k0 + l), k0 = floor(log2(segment_len / nbins)); the final level is returned and
- * scored in FP32.  FP16 assumes inputs of order unity per sample (bucket sums beyond
- * 65504 * 2^k0 overflow).  No reference counterpart (the reference stores FP32 or complex64
- * folds, plans.hpp:151-201). */
+ * LOKI_STORAGE_F32 (default), LOKI_STORAGE_F16, LOKI_STORAGE_BF16 or LOKI_STORAGE_BFP16.  Each
+ * add is done in FP32 and each stored value is rounded once (round to nearest even).  F16/BF16:
+ * level l is stored at the exact scale 2^-(k0 + l), k0 = floor(log2(segment_len / nbins)); FP16
+ * assumes inputs of order unity per sample (bucket sums beyond 65504 * 2^k0 overflow).  BFP16
+ * (block floating point; power-of-two nbins in [4, 256], finite inputs): levels 1..L-2 store
+ * each component row as nbins int16 mantissas with one shared exponent chosen from the row's
+ * largest magnitude (15 significant bits relative to it), level 0 stays FP32.  The final level
+ * is returned and scored in FP32.  No reference counterpart (the reference stores FP32 or
+ * complex64 folds, plans.hpp:151-201). */
 #define LOKI_STORAGE_F32 0
 #define LOKI_STORAGE_F16 1
 #define LOKI_STORAGE_BF16 2
+#define LOKI_STORAGE_BFP16 3
 int loki_ffa_set_fold_storage(loki_ffa_t ffa, int storage);
 /* Waits for the recorded events; *n receives the number of timed launches. */
 int loki_ffa_timings(loki_ffa_t ffa, loki_kernel_timing* out, size_t cap, size_t* n);

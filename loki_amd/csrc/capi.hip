@@ -590,8 +590,8 @@ int loki_ffa_set_fold_storage(loki_ffa_t ffa, int storage) {
     return guarded([&] {
         REQ(ffa != nullptr, "null ffa");
         REQ(storage == LOKI_STORAGE_F32 || storage == LOKI_STORAGE_F16 ||
-                storage == LOKI_STORAGE_BF16,
-            "fold storage must be 0 (f32), 1 (f16) or 2 (bf16)");
+                storage == LOKI_STORAGE_BF16 || storage == LOKI_STORAGE_BFP16,
+            "fold storage must be 0 (f32), 1 (f16), 2 (bf16) or 3 (bfp16)");
         ffa->exec.set_fold_storage(static_cast<loki_amd::FoldStorage>(storage));
     });
 }

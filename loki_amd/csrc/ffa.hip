@@ -233,7 +233,33 @@ void FFAExecutor::prepare_brute_fold(hipStream_t s) {
 void FFAExecutor::set_fold_storage(FoldStorage st) {
     LOKI_REQUIRE(!m_fourier || st == FoldStorage::kF32,
                  "reduced-precision fold storage is available for the time-domain FFA only");
+    if (st == FoldStorage::kBFP16) {
+        const SizeType nb = m_plan.config().get_nbins();
+        LOKI_REQUIRE(nb >= 4 && nb <= 256 && (nb & (nb - 1)) == 0,
+                     "block floating-point storage needs a power-of-two nbins in [4, 256]");
+    }
     m_storage = st;
+}
+
+FoldStorage FFAExecutor::stored_as(SizeType l, SizeType last_level, bool last_fp32) const {
+    if (l == last_level && last_fp32) {
+        return FoldStorage::kF32;
+    }
+    // block floating point keeps the brute fold (level 0) in FP32
+    if (l == 0 && m_storage == FoldStorage::kBFP16) {
+        return FoldStorage::kF32;
+    }
+    return m_storage;
+}
+
+SizeType FFAExecutor::stored_bytes(SizeType last_level, bool last_fp32) const {
+    SizeType m = 0;
+    const SizeType nbins = m_plan.config().get_nbins();
+    for (SizeType l = 0; l <= last_level; ++l) {
+        m = std::max(m, static_cast<SizeType>(level_bytes(stored_as(l, last_level, last_fp32),
+                                                          m_plan.level_size(l), nbins)));
+    }
+    return m;
 }
 
 void FFAExecutor::init_level(const float* ts_e, const float* ts_v, void* out, void* scratch,
@@ -242,7 +268,9 @@ void FFAExecutor::init_level(const float* ts_e, const float* ts_v, void* out, vo
     const SizeType nseg  = m_plan.nsegments()[0];
     const int bslot      = m_timer.start(KernelKind::kBruteFold, 0, s);
     if (!m_fourier) {
-        const FoldStorage so = fp32_out ? FoldStorage::kF32 : m_storage;
+        // FP16/BF16 level 0 is stored at 2^-k0; FP32 (also under block
+        // floating point) at its true scale (the kernel ignores the scale)
+        const FoldStorage so = stored_as(0, 0, fp32_out);
         kern::brute_fold_time(ts_e, ts_v, nseg, m_bf, out, so, std::ldexp(1.0F, -m_k0), s);
         check_launch("brute fold launch");
         m_timer.stop(bslot, s);
@@ -306,15 +334,17 @@ void* FFAExecutor::run_levels(const float* ts_e, const float* ts_v, void* buf_ev
                                 m_plan.ncoords()[l], m_plan.ncoords()[l - 1],
                                 m_plan.nsegments()[l], static_cast<uint32_t>(nbins), s);
         } else {
-            // reduced precision: (tail + head) * 0.5 keeps level l at 2^-(k0 + l); a last
-            // level returned in FP32 is unscaled by 2^(k0 + l - 1) instead
+            // FP16/BF16: (tail + head) * 0.5 keeps level l at 2^-(k0 + l); a last
+            // level returned in FP32 is unscaled by 2^(k0 + l - 1) instead.
+            // FP32 and block floating point are unscaled (scale unused).
             const bool to32      = last_fp32 && l == last_level;
-            const FoldStorage so = to32 ? FoldStorage::kF32 : m_storage;
+            const FoldStorage si = stored_as(l - 1, last_level, last_fp32);
+            const FoldStorage so = stored_as(l, last_level, last_fp32);
             const float scale =
                 to32 ? std::ldexp(1.0F, m_k0 + static_cast<int>(l) - 1) : 0.5F;
             kern::merge_time(in, out, cp, cfp, m_plan.ncoords()[l], m_plan.ncoords()[l - 1],
                              m_plan.nsegments()[l], m_plan.param_counts()[l].back(),
-                             static_cast<uint32_t>(nbins), m_storage, so, scale, s);
+                             static_cast<uint32_t>(nbins), si, so, scale, s);
         }
         check_launch("merge launch");
         m_timer.stop(slot, s);
@@ -509,13 +539,17 @@ void FFAExecutor::execute_scores(const float* ts_e, const float* ts_v,
         // Neither the final fold cube nor (depth 2) the parent level is stored.
         const bool grand    = m_fuse_depth == 2 && L >= 3;
         const SizeType top  = grand ? L - 3 : L - 2;  // last level through HBM
-        const SizeType need = max_level_size(top) * storage_bytes(m_storage);
+        const SizeType need = stored_bytes(top, false);
         m_ws->internal.reserve(need);
         m_ws->result.reserve(need);
         const void* stored =
             run_levels(ts_e, ts_v, m_ws->result.get(), m_ws->internal.get(), top, false, s, L - 2);
         kern::Final2Args a{};
         a.in   = stored;
+        if (stored_as(top, top, false) == FoldStorage::kBFP16) {
+            a.ein = reinterpret_cast<const uint16_t*>(static_cast<const char*>(stored) +
+                                                      2 * m_plan.level_size(top));
+        }
         a.n_p1 = m_plan.ncoords()[L - 2];
         a.n_p2 = grand ? m_plan.ncoords()[L - 3] : 0;
         if (grand) {
@@ -545,7 +579,11 @@ void FFAExecutor::execute_scores(const float* ts_e, const float* ts_v,
                          "score filter: trial index exceeds 32 bits");
         }
         const int slot = m_timer.start(KernelKind::kMergeScore, static_cast<int>(L - 1), s);
-        kern::final2_merge_score(a, m_storage, m_f2_nblocks, find_max, m_exact_norm, s);
+        // the stored level may be FP32 (the brute fold) while the parent level
+        // merged on chip is block floating point
+        a.pbfp = grand && stored_as(L - 2, L - 2, false) == FoldStorage::kBFP16 ? 1U : 0U;
+        kern::final2_merge_score(a, stored_as(top, top, false), m_f2_nblocks, find_max,
+                                 m_exact_norm, s);
         check_launch("final-level merge+score launch");
         m_timer.stop(slot, s);
         return;
@@ -553,14 +591,15 @@ void FFAExecutor::execute_scores(const float* ts_e, const float* ts_v,
     if (m_fused && fused_scores_supported()) {
         // Levels 0..L-2 through HBM, the last level fused with its shift
         // tables and the S/N: the final fold cube is never stored.
-        const SizeType need = max_level_size(L - 2) * storage_bytes(m_storage);
+        const SizeType need = stored_bytes(L - 2, false);
         m_ws->internal.reserve(need);
         m_ws->result.reserve(need);
         const void* parent =
             run_levels(ts_e, ts_v, m_ws->result.get(), m_ws->internal.get(), L - 2, false, s);
         const auto lv  = resolve_level(m_plan, L - 1);
         const int slot = m_timer.start(KernelKind::kMergeScore, static_cast<int>(L - 1), s);
-        kern::final_merge_score(parent, m_storage, std::ldexp(1.0F, m_k0 + static_cast<int>(L) - 2),
+        kern::final_merge_score(parent, stored_as(L - 2, L - 2, false),
+                                std::ldexp(1.0F, m_k0 + static_cast<int>(L) - 2),
                                 ncoords, m_plan.ncoords()[L - 2], lv, wt, scores, find_max,
                                 m_exact_norm, s, fo);
         check_launch("final-level merge+score launch");

@@ -182,6 +182,11 @@ private:
     void init_level(const float* ts_e, const float* ts_v, void* out, void* scratch,
                     bool fp32_out, hipStream_t s);
     SizeType max_level_size(SizeType last_level) const;
+    /// Storage of level l in a run whose last level is last_level (returned
+    /// in FP32 when last_fp32).
+    FoldStorage stored_as(SizeType l, SizeType last_level, bool last_fp32) const;
+    /// Bytes of the largest stored level of such a run (ping-pong buffer size).
+    SizeType stored_bytes(SizeType last_level, bool last_fp32) const;
     SizeType elem_bytes() const noexcept { return m_fourier ? 2 * sizeof(float) : sizeof(float); }
 };
 

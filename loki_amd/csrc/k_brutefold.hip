@@ -647,6 +647,9 @@ void brute_fold_time(const float* ts_e, const float* ts_v, uint64_t nseg,
         return;
     }
     switch (so) {
+    case FoldStorage::kBFP16:
+        LOKI_REQUIRE(false, "brute fold: block floating point keeps level 0 in FP32");
+        return;
     case FoldStorage::kF32:
         return launch_fold(ts_e, ts_v, blocks, bps, L, static_cast<float*>(fold), scale, s);
     case FoldStorage::kF16:
@@ -686,6 +689,9 @@ void brute_fold_grouped(const FoldTask* tasks, uint32_t ntasks, uint32_t nblocks
         return;
     }
     switch (so) {
+    case FoldStorage::kBFP16:
+        LOKI_REQUIRE(false, "grouped brute fold: block floating point unsupported");
+        return;
     case FoldStorage::kF32:
         return grouped_fold<float>(tasks, ntasks, nblocks, runs, max_nbins, s);
     case FoldStorage::kF16:

@@ -21,7 +21,8 @@
 // Every FP32 operation matches the unfused path and the CPU oracle bit for
 // bit.  With FP16/BF16 parents (storage.hpp) the loads are 8-byte chunks
 // widened to FP32 and the merged sums are unscaled exactly before step 2's
-// normalisation.
+// normalisation; block floating-point parents are widened with their rows'
+// exponents and need no unscaling.
 #include <cfloat>
 #include <type_traits>
 #include <utility>
@@ -73,7 +74,7 @@ __device__ __forceinline__ void merge_store(const float4 te, const float4 tv, co
     v.y = tv.y + hv.y;
     v.z = tv.z + hv.z;
     v.w = tv.w + hv.w;
-    if constexpr (!std::is_same_v<TI, float>) {
+    if constexpr (kScaledHalf<TI>) {
         e.x *= unscale;
         e.y *= unscale;
         e.z *= unscale;
@@ -131,7 +132,8 @@ __device__ __forceinline__ void final_block(uint32_t bid, const TI* __restrict__
                                             uint64_t nprev, const ResolveLevel& lv,
                                             const FinalGrid& fg, const WidthTable& wt,
                                             float* __restrict__ scores, const TileSchedule& ts,
-                                            float unscale, const FilterOut& fo) {
+                                            float unscale, const FilterOut& fo,
+                                            const uint16_t* __restrict__ ein) {
     using CI = chunk4_t<TI>;
     constexpr uint32_t C4  = NB / 4;        // float4 chunks per component row
     constexpr uint32_t LPR = C4;            // lanes per row (4 bins of e and v each)
@@ -209,10 +211,17 @@ __device__ __forceinline__ void final_block(uint32_t bid, const TI* __restrict__
                     // of every add, half the rotations.
                     const uint32_t St = NP == 1 ? 0U : shift_int(cd.shift_tail, NB);
                     const uint32_t Sh = shift_int(cd.shift_head, NB);
-                    te[u] = widen4<TI>(trow[i4]);
-                    tv[u] = widen4<TI>(trow[C4 + i4]);
-                    he[u] = rotated4_st<NB, TI>(hrow, i4, (Sh - St) & (NB - 1));
-                    hv[u] = rotated4_st<NB, TI>(hrow + C4, i4, (Sh - St) & (NB - 1));
+                    float ste = 1.0F, stv = 1.0F, she = 1.0F, shv = 1.0F;
+                    if constexpr (kBlockFP<TI>) {
+                        ste = bfp_row_scale(ein, cd.i_tail, 0);
+                        stv = bfp_row_scale(ein, cd.i_tail, 1);
+                        she = bfp_row_scale(ein, nprev + cd.i_head, 0);
+                        shv = bfp_row_scale(ein, nprev + cd.i_head, 1);
+                    }
+                    te[u] = chunk4_st<TI>(trow, i4, ste);
+                    tv[u] = chunk4_st<TI>(trow + C4, i4, stv);
+                    he[u] = rotated4_st<NB, TI>(hrow, i4, (Sh - St) & (NB - 1), she);
+                    hv[u] = rotated4_st<NB, TI>(hrow + C4, i4, (Sh - St) & (NB - 1), shv);
                     b0[u] = 4 * i4 + St;
                 }
             }
@@ -295,9 +304,9 @@ template <int NB, int NP, bool FINDMAX, bool EXACT, typename TI>
 __global__ void __launch_bounds__(kRows)
 k_final_merge_score(const TI* __restrict__ in, uint64_t nprev, ResolveLevel lv, FinalGrid fg,
                     WidthTable wt, float* __restrict__ scores, TileSchedule ts, float unscale,
-                    FilterOut fo) {
+                    FilterOut fo, const uint16_t* __restrict__ ein) {
     final_block<NB, NP, FINDMAX, EXACT, TI>(blockIdx.x, in, nprev, lv, fg, wt, scores, ts,
-                                            unscale, fo);
+                                            unscale, fo, ein);
 }
 
 // Grouped launch: the last levels of many independent FFAs (one task each,
@@ -319,7 +328,7 @@ k_final_grouped(const FinalTask* __restrict__ tasks, uint32_t ntasks) {
     const FinalTask& t = tasks[lo];
     final_block<NB, NP, FINDMAX, EXACT, TI>(blockIdx.x - t.block0, static_cast<const TI*>(t.in),
                                             t.nprev, t.lv, t.fg, t.wt, t.scores, t.ts,
-                                            t.unscale, FilterOut{});
+                                            t.unscale, FilterOut{}, nullptr);
 }
 
 template <int NB, int NP, typename TI>
@@ -335,18 +344,23 @@ void launch_nb_np(const TI* in, float unscale, uint64_t ncur, uint64_t nprev,
     }
     fg.stride = final_lds_stride(NB, wt.wmax);
     const size_t lds = static_cast<size_t>(kRows) * fg.stride * sizeof(float);
+    // BFP16 parents: the exponent plane follows the level's 2 * nprev rows
+    const uint16_t* ein = nullptr;
+    if constexpr (kBlockFP<TI>) {
+        ein = reinterpret_cast<const uint16_t*>(in + 2 * nprev * (2 * NB));
+    }
     if (find_max && exact) {
         k_final_merge_score<NB, NP, true, true, TI>
-            <<<nb, kRows, lds, s>>>(in, nprev, lv, fg, wt, scores, ts, unscale, fo);
+            <<<nb, kRows, lds, s>>>(in, nprev, lv, fg, wt, scores, ts, unscale, fo, ein);
     } else if (find_max) {
         k_final_merge_score<NB, NP, true, false, TI>
-            <<<nb, kRows, lds, s>>>(in, nprev, lv, fg, wt, scores, ts, unscale, fo);
+            <<<nb, kRows, lds, s>>>(in, nprev, lv, fg, wt, scores, ts, unscale, fo, ein);
     } else if (exact) {
         k_final_merge_score<NB, NP, false, true, TI>
-            <<<nb, kRows, lds, s>>>(in, nprev, lv, fg, wt, scores, ts, unscale, fo);
+            <<<nb, kRows, lds, s>>>(in, nprev, lv, fg, wt, scores, ts, unscale, fo, ein);
     } else {
         k_final_merge_score<NB, NP, false, false, TI>
-            <<<nb, kRows, lds, s>>>(in, nprev, lv, fg, wt, scores, ts, unscale, fo);
+            <<<nb, kRows, lds, s>>>(in, nprev, lv, fg, wt, scores, ts, unscale, fo, ein);
     }
 }
 
@@ -418,6 +432,9 @@ void final_merge_score(const void* in, FoldStorage si, float unscale, uint64_t n
     case FoldStorage::kBF16:
         return launch_typed<__bf16>(in, unscale, ncur, nprev, lv, wt, scores, find_max, exact, f,
                                     s);
+    case FoldStorage::kBFP16:
+        return launch_typed<int16_t>(in, unscale, ncur, nprev, lv, wt, scores, find_max, exact,
+                                     f, s);
     }
 }
 

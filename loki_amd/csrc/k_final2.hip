@@ -16,8 +16,9 @@
 //      are either copied from the stored parent level or, with `grand` set,
 //      merged on chip from the grandparent level through the parent level's
 //      shift table (kernels.cpp:71-146: one FP32 add per element, then the
-//      reduced-precision rounding of a stored level), so the parent level
-//      never reaches HBM;
+//      reduced-precision rounding of a stored level -- for block floating
+//      point the row exponent comes from a max over the row's lanes), so the
+//      parent level never reaches HBM;
 //   2. the lane merges its tail and head rows (one FP32 add each), forms
 //      p = e * (1/sqrt(v)) (score.cpp:139-141) and the sequential circular
 //      prefix sum in registers;
@@ -40,9 +41,11 @@ namespace {
 constexpr uint32_t kLanes = 64;
 constexpr uint32_t kBlock = 4 * kLanes;  // largest block: 4 rows
 
-/// Repeated leading bins of a staged row: a rotated read of 4 bins starting
-/// at any bin stays inside the row.
-constexpr uint32_t kPad = 3;
+/// Staged row: the NB (e, v) pairs twice over, so the row rotated by any
+/// shift S is the contiguous run starting at pair NB - S, plus one pair of
+/// padding (rows start on alternating LDS banks).
+template <int NB>
+constexpr uint32_t kStride = 2 * NB + 1;
 
 __device__ __forceinline__ uint32_t wave_min(uint32_t x) {
 #pragma unroll
@@ -143,7 +146,7 @@ template <int NB, int NP, bool FINDMAX, bool EXACT, typename TI>
 __global__ void __launch_bounds__(kBlock)
 k_final2(Final2Args a) {
     constexpr uint32_t C4     = NB / 4;           // 4-bin chunks per row
-    constexpr uint32_t STRIDE = NB + kPad;        // (e, v) pairs per staged row
+    constexpr uint32_t STRIDE = kStride<NB>;      // (e, v) pairs per staged row
     extern __shared__ float2 s_rows[];            // [2 * cap][STRIDE]
     __shared__ uint32_t s_rng[4];                 // lo_t, hi_t, lo_h, hi_h
 
@@ -225,24 +228,37 @@ k_final2(Final2Args a) {
             float4 e, v;
             uint32_t b0 = 4 * q4;
             if (!a.grand) {
-                const CI* src = reinterpret_cast<const CI*>(in + ((side * a.n_p1) + k) * (2 * NB));
-                e = widen4<TI>(src[q4]);
-                v = widen4<TI>(src[C4 + q4]);
+                const uint64_t r = (side * a.n_p1) + k;
+                const CI* src    = reinterpret_cast<const CI*>(in + r * (2 * NB));
+                float se = 1.0F, sv = 1.0F;
+                if constexpr (kBlockFP<TI>) {
+                    se = bfp_row_scale(a.ein, r, 0);
+                    sv = bfp_row_scale(a.ein, r, 1);
+                }
+                e = chunk4_st<TI>(src, q4, se);
+                v = chunk4_st<TI>(src + C4, q4, sv);
             } else {
                 uint32_t it, st, ih, sh;
                 parent_record<NP>(a, k, it, st, ih, sh, NB);
-                const CI* trow = reinterpret_cast<const CI*>(
-                    in + (((2 * side) * a.n_p2) + it) * (2 * NB));
-                const CI* hrow = reinterpret_cast<const CI*>(
-                    in + (((2 * side + 1) * a.n_p2) + ih) * (2 * NB));
+                const uint64_t rt = ((2 * side) * a.n_p2) + it;
+                const uint64_t rh = ((2 * side + 1) * a.n_p2) + ih;
+                const CI* trow    = reinterpret_cast<const CI*>(in + rt * (2 * NB));
+                const CI* hrow    = reinterpret_cast<const CI*>(in + rh * (2 * NB));
+                float ste = 1.0F, stv = 1.0F, she = 1.0F, shv = 1.0F;
+                if constexpr (kBlockFP<TI>) {
+                    ste = bfp_row_scale(a.ein, rt, 0);
+                    stv = bfp_row_scale(a.ein, rt, 1);
+                    she = bfp_row_scale(a.ein, rh, 0);
+                    shv = bfp_row_scale(a.ein, rh, 1);
+                }
                 // parent[j] = tail[j - St] + head[j - Sh]: the tail is read
                 // unrotated and the sums placed at j = 4*q4 + i + St, so only
                 // the head is rotated (by Sh - St)
                 const uint32_t d  = wrap<NB>(sh + NB - st);
-                const float4 te   = widen4<TI>(trow[q4]);
-                const float4 tv   = widen4<TI>(trow[C4 + q4]);
-                const float4 he   = rotated4_st<NB, TI>(hrow, q4, d);
-                const float4 hv   = rotated4_st<NB, TI>(hrow + C4, q4, d);
+                const float4 te   = chunk4_st<TI>(trow, q4, ste);
+                const float4 tv   = chunk4_st<TI>(trow + C4, q4, stv);
+                const float4 he   = rotated4_st<NB, TI>(hrow, q4, d, she);
+                const float4 hv   = rotated4_st<NB, TI>(hrow + C4, q4, d, shv);
                 e.x = te.x + he.x;
                 e.y = te.y + he.y;
                 e.z = te.z + he.z;
@@ -251,7 +267,30 @@ k_final2(Final2Args a) {
                 v.y = tv.y + hv.y;
                 v.z = tv.z + hv.z;
                 v.w = tv.w + hv.w;
-                if constexpr (!std::is_same_v<TI, float>) {
+                bool bfp_parent = kBlockFP<TI>;
+                if constexpr (std::is_same_v<TI, float>) {
+                    bfp_parent = a.pbfp != 0;  // FP32 brute fold under a BFP16 parent
+                }
+                if (bfp_parent) {
+                    // the parent level as stored: row exponents from the row
+                    // maxima over the C4 consecutive tasks (lanes) of this row
+                    const int ee = bfp_exponent(group_absmax<C4>(__builtin_fmaxf(
+                        __builtin_fmaxf(__builtin_fabsf(e.x), __builtin_fabsf(e.y)),
+                        __builtin_fmaxf(__builtin_fabsf(e.z), __builtin_fabsf(e.w)))));
+                    const int ev = bfp_exponent(group_absmax<C4>(__builtin_fmaxf(
+                        __builtin_fmaxf(__builtin_fabsf(v.x), __builtin_fabsf(v.y)),
+                        __builtin_fmaxf(__builtin_fabsf(v.z), __builtin_fabsf(v.w)))));
+                    const float ie = pow2f(-ee), se = pow2f(ee);
+                    const float iv = pow2f(-ev), sv = pow2f(ev);
+                    e.x = round_bfp(e.x, ie, se);
+                    e.y = round_bfp(e.y, ie, se);
+                    e.z = round_bfp(e.z, ie, se);
+                    e.w = round_bfp(e.w, ie, se);
+                    v.x = round_bfp(v.x, iv, sv);
+                    v.y = round_bfp(v.y, iv, sv);
+                    v.z = round_bfp(v.z, iv, sv);
+                    v.w = round_bfp(v.w, iv, sv);
+                } else if constexpr (kScaledHalf<TI>) {
                     e.x = round_stored<TI>(e.x * a.scale1);
                     e.y = round_stored<TI>(e.y * a.scale1);
                     e.z = round_stored<TI>(e.z * a.scale1);
@@ -269,9 +308,7 @@ k_final2(Final2Args a) {
                 const uint32_t b = wrap<NB>(b0 + i);
                 const float2 pr  = make_float2(ev[i][0], ev[i][1]);
                 dst[b]           = pr;
-                if (b < kPad) {
-                    dst[NB + b] = pr;
-                }
+                dst[NB + b]      = pr;
             }
         }
     }
@@ -300,7 +337,7 @@ k_final2(Final2Args a) {
             v.y = tv.y + hv.y;
             v.z = tv.z + hv.z;
             v.w = tv.w + hv.w;
-            if constexpr (!std::is_same_v<TI, float>) {
+            if constexpr (kScaledHalf<TI>) {
                 e.x *= a.unscale;
                 e.y *= a.unscale;
                 e.z *= a.unscale;
@@ -319,14 +356,21 @@ k_final2(Final2Args a) {
             ps[4 * c + 2]  = acc;
             acc            = acc + p.w;
             ps[4 * c + 3]  = acc;
+            // materialise this chunk's prefix values before the next chunk's
+            // reads: left pending (the compiler sinks the products and adds),
+            // every chunk's operands stay live and the kernel needs 256 VGPRs
+            __asm__ volatile("" : "+v"(ps[4 * c]), "+v"(ps[4 * c + 1]), "+v"(ps[4 * c + 2]),
+                             "+v"(ps[4 * c + 3]), "+v"(acc));
         }
     };
     {
         const float2* trow = s_rows + (cd.i_tail - lo_t) * STRIDE;
         const float2* hrow = s_rows + (a.cap + (cd.i_head - lo_h)) * STRIDE;
+        const float2* tb = trow + (NB - St);  // rotated rows: contiguous runs
+        const float2* hb = hrow + (NB - Sh);
         prefix([&](uint32_t c, float4& te, float4& tv, float4& he, float4& hv) {
-            const float2* tp = trow + wrap<NB>(4 * c + NB - St);
-            const float2* hp = hrow + wrap<NB>(4 * c + NB - Sh);
+            const float2* tp = tb + 4 * c;
+            const float2* hp = hb + 4 * c;
             const float2 t0 = tp[0], t1 = tp[1], t2 = tp[2], t3 = tp[3];
             const float2 h0 = hp[0], h1 = hp[1], h2 = hp[2], h3 = hp[3];
             te = make_float4(t0.x, t1.x, t2.x, t3.x);
@@ -347,9 +391,14 @@ k_final2(Final2Args a) {
     const uint64_t c_idx = static_cast<uint64_t>(row) * n_f + jf;
     float best           = -FLT_MAX;
     for (uint32_t iw = 0; iw < a.wt.n; ++iw) {
-        // keep the windows inside the width loop (hoisted, all NB - 1 would be
-        // formed for every trial)
-        __asm__ volatile("" : "+v"(ps[0]));
+        // keep the windows inside the width loop: every prefix value is opaque
+        // per iteration, so no window difference of any width can be hoisted
+        // (hoisted, the windows of all NB - 1 widths would be formed for every
+        // trial and held in registers)
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            __asm__ volatile("" : "+v"(ps[k]));
+        }
         const float dmax =
             window_max_reg<NB>(std::make_integer_sequence<int, NB - 1>{}, ps, ext, a.wt.w[iw]);
         const float snr = ((a.wt.h[iw] + a.wt.b[iw]) * dmax) - (a.wt.b[iw] * sum);
@@ -402,7 +451,7 @@ k_final2_ranges(Final2Args a, uint32_t nblocks, uint32_t* out) {
 
 template <int NB, int NP, typename TI>
 void launch3(const Final2Args& a, uint32_t nblocks, bool find_max, bool exact, hipStream_t s) {
-    const size_t lds     = static_cast<size_t>(2) * a.cap * (NB + kPad) * sizeof(float2);
+    const size_t lds     = static_cast<size_t>(2) * a.cap * kStride<NB> * sizeof(float2);
     const uint32_t block = a.rows_per_block * kLanes;
     if (find_max && exact) {
         k_final2<NB, NP, true, true, TI><<<nblocks, block, lds, s>>>(a);
@@ -454,7 +503,7 @@ void final2_ranges(const Final2Args& a, uint32_t nblocks, uint32_t* out, hipStre
 bool final2_supported(uint32_t nbins) { return nbins == 8 || nbins == 16 || nbins == 32; }
 
 uint32_t final2_lds_bytes(uint32_t nbins, uint32_t cap) {
-    return 2U * cap * (nbins + kPad) * static_cast<uint32_t>(sizeof(float2));
+    return 2U * cap * (2 * nbins + 1) * static_cast<uint32_t>(sizeof(float2));
 }
 
 void final2_merge_score(const Final2Args& a, FoldStorage si, uint32_t nblocks, bool find_max,
@@ -478,6 +527,9 @@ void final2_merge_score(const Final2Args& a, FoldStorage si, uint32_t nblocks, b
     case FoldStorage::kF32: return launch1<float>(a, nblocks, find_max, exact, s);
     case FoldStorage::kF16: return launch1<_Float16>(a, nblocks, find_max, exact, s);
     case FoldStorage::kBF16: return launch1<__bf16>(a, nblocks, find_max, exact, s);
+    case FoldStorage::kBFP16:
+        LOKI_REQUIRE(a.ein != nullptr, "final-level kernel: exponent plane missing");
+        return launch1<int16_t>(a, nblocks, find_max, exact, s);
     }
 }
 

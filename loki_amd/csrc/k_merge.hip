@@ -15,7 +15,10 @@
 // remapped so each XCD works on a contiguous range of tiles (parent reuse
 // stays inside one L2).  All offsets are 64-bit.  With FP16/BF16 storage
 // (storage.hpp) a lane reads 4 elements as 8 bytes and rotates them with a
-// 16-bit funnel shift; the add stays in FP32.
+// 16-bit funnel shift; the add stays in FP32.  Block floating-point levels
+// (int16 mantissas) are widened with their row's exponent and stored with an
+// exponent from the row maximum of the FP32 sums (a reduction over the lanes
+// of each component row).
 #include <tuple>
 #include <type_traits>
 
@@ -45,9 +48,10 @@ template <int NB, bool POLY, int ITERS, typename TI, typename TO>
 __global__ void __launch_bounds__(kBlock)
 k_merge_v4(const TI* __restrict__ in, TO* __restrict__ out,
            const FFACoord* __restrict__ coords, const FFACoordFreq* __restrict__ cf,
-           uint64_t ncur, uint64_t nprev, TileSchedule ts, float scale) {
+           uint64_t ncur, uint64_t nprev, TileSchedule ts, float scale,
+           const uint16_t* __restrict__ ein, uint8_t* __restrict__ eout) {
     merge_v4_block<NB, POLY, ITERS, TI, TO>(blockIdx.x, in, out, coords, cf, ncur, nprev, ts,
-                                            scale);
+                                            scale, ein, eout);
 }
 
 // Any nbins: one lane per output element pair (e and v of bin j).
@@ -153,14 +157,32 @@ void launch_v4(const TI* in, TO* out, const FFACoord* coords, const FFACoordFreq
     // up to 64 rows per block, at most 16 chunk loads in flight per lane
     using G               = V4Geom<NB>;
     const TileSchedule ts = make_schedule(ncur, n_f, nseg, G::TILE, kRowGroup, kRun);
-    k_merge_v4<NB, POLY, G::ITERS, TI, TO><<<ts.nblocks, kBlock, 0, s>>>(in, out, coords, cf,
-                                                                         ncur, nprev, ts, scale);
+    // BFP16 exponent planes follow each level's mantissas (parent level:
+    // 2 * nseg segments of nprev rows)
+    const uint16_t* ein = nullptr;
+    uint8_t* eout       = nullptr;
+    if constexpr (kBlockFP<TI>) {
+        ein = reinterpret_cast<const uint16_t*>(in + (2 * nseg) * nprev * (2 * NB));
+    }
+    if constexpr (kBlockFP<TO>) {
+        eout = reinterpret_cast<uint8_t*>(out + nseg * ncur * (2 * NB));
+    }
+    if constexpr ((kBlockFP<TI> || kBlockFP<TO>) && NB > 256) {
+        LOKI_REQUIRE(false, "block floating-point storage needs nbins <= 256");
+    } else {
+        k_merge_v4<NB, POLY, G::ITERS, TI, TO><<<ts.nblocks, kBlock, 0, s>>>(
+            in, out, coords, cf, ncur, nprev, ts, scale, ein, eout);
+    }
 }
 
 template <bool POLY, typename TI, typename TO>
 void dispatch(const TI* in, TO* out, const FFACoord* coords, const FFACoordFreq* cf,
               uint64_t ncur, uint64_t nprev, uint64_t nseg, uint64_t n_f, uint32_t nbins,
               float scale, hipStream_t s) {
+    if constexpr (kBlockFP<TI> || kBlockFP<TO>) {
+        LOKI_REQUIRE(nbins >= 4 && nbins <= 256 && (nbins & (nbins - 1)) == 0,
+                     "block floating-point storage needs a power-of-two nbins in [4, 256]");
+    }
     switch (nbins) {
     case 4: return launch_v4<4, POLY>(in, out, coords, cf, ncur, nprev, nseg, n_f, scale, s);
     case 8: return launch_v4<8, POLY>(in, out, coords, cf, ncur, nprev, nseg, n_f, scale, s);
@@ -174,11 +196,13 @@ void dispatch(const TI* in, TO* out, const FFACoord* coords, const FFACoordFreq*
         return launch_v4<1024, POLY>(in, out, coords, cf, ncur, nprev, nseg, n_f, scale, s);
     default: break;
     }
-    const uint64_t total = nseg * ncur * nbins;
-    const uint64_t want  = (total + kBlock - 1) / kBlock;
-    const unsigned grid  = static_cast<unsigned>(want < 65536 * 8 ? want : 65536 * 8);
-    k_merge_scalar<POLY, TI, TO><<<grid, kBlock, 0, s>>>(in, out, coords, cf, ncur, nprev, nseg,
-                                                         nbins, scale);
+    if constexpr (!(kBlockFP<TI> || kBlockFP<TO>)) {
+        const uint64_t total = nseg * ncur * nbins;
+        const uint64_t want  = (total + kBlock - 1) / kBlock;
+        const unsigned grid  = static_cast<unsigned>(want < 65536 * 8 ? want : 65536 * 8);
+        k_merge_scalar<POLY, TI, TO><<<grid, kBlock, 0, s>>>(in, out, coords, cf, ncur, nprev,
+                                                             nseg, nbins, scale);
+    }
 }
 
 template <typename TI, typename TO>
@@ -203,14 +227,21 @@ void merge_time(const void* in, void* out, const FFACoord* coords,
     LOKI_REQUIRE(nbins >= 1, "merge: nbins must be positive");
     LOKI_REQUIRE((coords == nullptr) != (coords_freq == nullptr),
                  "merge: exactly one coordinate table");
-    LOKI_REQUIRE(si == so || so == FoldStorage::kF32, "merge: unsupported storage pair");
+    LOKI_REQUIRE(si == so || so == FoldStorage::kF32 ||
+                     (si == FoldStorage::kF32 && so == FoldStorage::kBFP16),
+                 "merge: unsupported storage pair");
     if (ncur == 0 || nseg == 0) {
         return;
     }
     const auto args = std::make_tuple(in, out, coords, coords_freq, ncur, nprev, nseg, n_f, nbins,
                                       scale, s);
     if (si == FoldStorage::kF32) {
-        return std::apply(merge_typed<float, float>, args);
+        return so == FoldStorage::kBFP16 ? std::apply(merge_typed<float, int16_t>, args)
+                                         : std::apply(merge_typed<float, float>, args);
+    }
+    if (si == FoldStorage::kBFP16) {
+        return so == FoldStorage::kBFP16 ? std::apply(merge_typed<int16_t, int16_t>, args)
+                                         : std::apply(merge_typed<int16_t, float>, args);
     }
     if (si == FoldStorage::kF16) {
         return so == FoldStorage::kF16 ? std::apply(merge_typed<_Float16, _Float16>, args)
@@ -238,6 +269,7 @@ void merge_grouped(const MergeTask* tasks, uint32_t ntasks, uint32_t nblocks, ui
     if (ntasks == 0 || nblocks == 0) {
         return;
     }
+    LOKI_REQUIRE(st != FoldStorage::kBFP16, "grouped merge: block floating point unsupported");
     if (st == FoldStorage::kF32) {
         return poly ? grouped_nb<true, float>(tasks, ntasks, nblocks, nbins, s)
                     : grouped_nb<false, float>(tasks, ntasks, nblocks, nbins, s);

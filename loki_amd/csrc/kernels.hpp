@@ -194,6 +194,8 @@ struct Final2Args {
     uint32_t grand;
     float unscale;            ///< reduced precision: final sums x 2^(k0 + F - 1)
     float scale1;             ///< reduced precision: parent merge scale (0.5)
+    const uint16_t* ein;      ///< block floating point: exponent plane of `in`
+    uint32_t pbfp;            ///< FP32 `in` whose on-chip parent level is block floating point
 };
 /// nbins supported by final2_merge_score (register windows).
 bool final2_supported(uint32_t nbins);

@@ -141,10 +141,11 @@ __device__ __forceinline__ float4 rotated4(const float4* __restrict__ row, uint3
 }
 
 /// rotated4 for any storage type, widened to FP32: row points at the first
-/// chunk of one component row of NB stored elements.
+/// chunk of one component row of NB stored elements.  BFP16 rows are widened
+/// with their scale s = 2^E (unused otherwise).
 template <int NB, typename T>
 __device__ __forceinline__ float4 rotated4_st(const chunk4_t<T>* __restrict__ row, uint32_t i4,
-                                              uint32_t S) {
+                                              uint32_t S, float s = 1.0F) {
     if constexpr (std::is_same_v<T, float>) {
         return rotated4<NB>(row, i4, S);
     } else {
@@ -152,8 +153,32 @@ __device__ __forceinline__ float4 rotated4_st(const chunk4_t<T>* __restrict__ ro
         const uint32_t e0 = (4U * i4 + NB - S) & (NB - 1);
         const uint32_t c0 = e0 >> 2;
         const uint32_t c1 = (c0 + 1) & (C4 - 1);
-        return widen4<T>(funnel4_16(row[c0], row[c1], e0 & 3U));
+        const uint2 r     = funnel4_16(row[c0], row[c1], e0 & 3U);
+        if constexpr (kBlockFP<T>) {
+            return widen4_bfp(r, s);
+        } else {
+            return widen4<T>(r);
+        }
     }
+}
+
+/// Aligned chunk i4 of a stored component row, widened (BFP16: scale s).
+template <typename T>
+__device__ __forceinline__ float4 chunk4_st(const chunk4_t<T>* __restrict__ row, uint32_t i4,
+                                            float s = 1.0F) {
+    if constexpr (kBlockFP<T>) {
+        return widen4_bfp(row[i4], s);
+    } else {
+        return widen4<T>(row[i4]);
+    }
+}
+
+/// Scale 2^E of component q (0: e, 1: v) of a BFP16 row pair from its
+/// exponent word (low byte e, high byte v).
+__device__ __forceinline__ float bfp_row_scale(const uint16_t* __restrict__ ew, uint64_t row,
+                                               uint32_t q) {
+    const uint32_t w = ew[row];
+    return pow2f(static_cast<int>(static_cast<int8_t>(q == 0 ? (w & 0xFFU) : (w >> 8))));
 }
 
 /// Lanes per block of the time-domain merge (single-level and grouped launches).
@@ -161,15 +186,20 @@ constexpr int kMergeBlock = 256;
 
 /// One 256-lane block of the time-domain merge (k_merge.hip): block `bid` of
 /// the level's tile schedule.  TI/TO: stored types of parent and child
-/// levels; unless both are float the FP32 sum is multiplied by `scale` before
-/// it is rounded to TO.
+/// levels; when either is FP16/BF16 the FP32 sum is multiplied by `scale`
+/// before it is rounded to TO.  BFP16 levels (int16_t) carry exponent planes:
+/// `ein` of the parent level, `eout` of the child level (16-bit word per row
+/// pair); the child's row exponents come from the row maxima of the FP32
+/// sums, reduced over the C4 lanes of each component row.
 template <int NB, bool POLY, int ITERS, typename TI, typename TO>
 __device__ __forceinline__ void merge_v4_block(uint32_t bid, const TI* __restrict__ in,
                                                TO* __restrict__ out,
                                                const FFACoord* __restrict__ coords,
                                                const FFACoordFreq* __restrict__ cf,
                                                uint64_t ncur, uint64_t nprev,
-                                               const TileSchedule& ts, float scale) {
+                                               const TileSchedule& ts, float scale,
+                                               const uint16_t* __restrict__ ein = nullptr,
+                                               uint8_t* __restrict__ eout = nullptr) {
     constexpr uint32_t kBlock = kMergeBlock;
     using CI = chunk4_t<TI>;
     using CO = chunk4_t<TO>;
@@ -178,6 +208,8 @@ __device__ __forceinline__ void merge_v4_block(uint32_t bid, const TI* __restric
     constexpr uint32_t LANES = LPR < kBlock ? LPR : kBlock;
     constexpr uint32_t ROWS = kBlock / LANES;
     constexpr uint32_t TILE = ROWS * ITERS;
+    static_assert(!(kBlockFP<TI> || kBlockFP<TO>) || (C4 <= 64 && LPR <= kBlock),
+                  "block floating point: a component row must fit one wave");
     const Tile tl = tile_of(ts, xcd_remap(bid, ts.nblocks), TILE);
     if (tl.count == 0) {
         return;
@@ -218,12 +250,17 @@ __device__ __forceinline__ void merge_v4_block(uint32_t bid, const TI* __restric
                 const uint32_t ch = lch + k * LANES;
                 const uint32_t q  = ch / C4;
                 const uint32_t i4 = ch % C4;
-                if constexpr (POLY) {
-                    tv[it][k] = rotated4_st<NB, TI>(trow + q * C4, i4, St);
-                } else {
-                    tv[it][k] = widen4<TI>(trow[ch]);
+                float st = 1.0F, sh = 1.0F;
+                if constexpr (kBlockFP<TI>) {
+                    st = bfp_row_scale(ein, it_row, q);
+                    sh = bfp_row_scale(ein, ih_row, q);
                 }
-                hv[it][k] = rotated4_st<NB, TI>(hrow + q * C4, i4, Sh);
+                if constexpr (POLY) {
+                    tv[it][k] = rotated4_st<NB, TI>(trow + q * C4, i4, St, st);
+                } else {
+                    tv[it][k] = chunk4_st<TI>(trow, ch, st);
+                }
+                hv[it][k] = rotated4_st<NB, TI>(hrow + q * C4, i4, Sh, sh);
             }
         }
     }
@@ -240,13 +277,26 @@ __device__ __forceinline__ void merge_v4_block(uint32_t bid, const TI* __restric
                 o.y = tv[it][k].y + hv[it][k].y;
                 o.z = tv[it][k].z + hv[it][k].z;
                 o.w = tv[it][k].w + hv[it][k].w;
-                if constexpr (!(std::is_same_v<TI, float> && std::is_same_v<TO, float>)) {
+                if constexpr (kScaledHalf<TI> || kScaledHalf<TO>) {
                     o.x *= scale;
                     o.y *= scale;
                     o.z *= scale;
                     o.w *= scale;
                 }
-                store_stream(&orow[lch + k * LANES], narrow4<TO>(o));
+                if constexpr (kBlockFP<TO>) {
+                    const float mx = group_absmax<C4>(__builtin_fmaxf(
+                        __builtin_fmaxf(__builtin_fabsf(o.x), __builtin_fabsf(o.y)),
+                        __builtin_fmaxf(__builtin_fabsf(o.z), __builtin_fabsf(o.w))));
+                    const int e = bfp_exponent(mx);
+                    store_stream(&orow[lch + k * LANES], narrow4_bfp(o, pow2f(-e)));
+                    const uint32_t ch = lch + k * LANES;
+                    if (ch % C4 == 0) {
+                        eout[(seg * ncur + c) * 2 + ch / C4] =
+                            static_cast<uint8_t>(static_cast<int8_t>(e));
+                    }
+                } else {
+                    store_stream(&orow[lch + k * LANES], narrow4<TO>(o));
+                }
             }
         }
     }

@@ -352,8 +352,10 @@ public:
             k = LOKI_STORAGE_F16;
         } else if (st == "bf16" || st == "bfloat16") {
             k = LOKI_STORAGE_BF16;
+        } else if (st == "bfp16") {
+            k = LOKI_STORAGE_BFP16;
         } else {
-            throw std::runtime_error("fold_storage must be 'f32', 'f16' or 'bf16'");
+            throw std::runtime_error("fold_storage must be 'f32', 'f16', 'bf16' or 'bfp16'");
         }
         check(loki_ffa_set_fold_storage(m_h.get(), k));
     }

@@ -563,3 +563,45 @@ void orc_round_storage(float* x, size_t n, float scale, int storage, int nthread
         for (size_t i = 0; i < n; ++i) x[i] = x[i] * scale;
     }
 }
+
+/* Block floating-point storage (bfp16): each component row of nbins FP32 values is stored as
+ * nbins int16 mantissas m and one exponent E, value = m * 2^E.  E = max(ex(M) - 14, -126) with
+ * M = max |x| over the row and ex(M) the unbiased exponent field of M (-127 for zero or
+ * subnormal M), so M * 2^-E < 2^15; m = rint(x * 2^-E) (ties to even) clamped to +-32767.
+ * x * 2^-E and m * 2^E are exact in FP32, so the single rounding is the rint.  In place: x
+ * becomes the decoded value m * 2^E.  Finite inputs only. */
+static inline float orc_pow2f(int e) {
+    uint32_t b = (uint32_t)(e + 127) << 23;
+    float f;
+    memcpy(&f, &b, sizeof f);
+    return f;
+}
+
+int orc_bfp16_exponent(float mx) {
+    uint32_t b;
+    memcpy(&b, &mx, sizeof b);
+    const int ex = (int)((b >> 23) & 0xFFu) - 127;
+    const int e = ex - 14;
+    return e < -126 ? -126 : e;
+}
+
+void orc_round_bfp16(float* x, size_t nrows, size_t nbins, int nthreads) {
+    const int nt = clamp_threads(nthreads);
+#pragma omp parallel for num_threads(nt) schedule(static)
+    for (size_t r = 0; r < nrows; ++r) {
+        float* row = x + r * nbins;
+        float mx = 0.0f;
+        for (size_t j = 0; j < nbins; ++j) {
+            const float a = fabsf(row[j]);
+            mx = a > mx ? a : mx;
+        }
+        const int e = orc_bfp16_exponent(mx);
+        const float inv = orc_pow2f(-e);
+        const float s = orc_pow2f(e);
+        for (size_t j = 0; j < nbins; ++j) {
+            float m = rintf(row[j] * inv);
+            m = m > 32767.0f ? 32767.0f : (m < -32767.0f ? -32767.0f : m);
+            row[j] = m * s;
+        }
+    }
+}

@@ -60,6 +60,9 @@ def lib() -> C.CDLL:
         L.orc_snr_boxcar_3d.argtypes = [_f32p, _sz, _sz, _u64p, _sz, _f32p, C.c_int, C.c_int]
         L.orc_snr_boxcar_3d_f64.argtypes = [_f64p, _sz, _sz, _u64p, _sz, _f64p, C.c_int]
         L.orc_round_storage.argtypes = [_f32p, _sz, C.c_float, C.c_int, C.c_int]
+        L.orc_round_bfp16.argtypes = [_f32p, _sz, _sz, C.c_int]
+        L.orc_bfp16_exponent.restype = C.c_int
+        L.orc_bfp16_exponent.argtypes = [C.c_float]
         _LIB = L
     return _LIB
 
@@ -194,6 +197,26 @@ def round_storage(x, storage: str):
 
 
 _STORAGE_CODE = {"f32": 0, "f16": 1, "bf16": 2}
+STORAGES = ("f32", "f16", "bf16", "bfp16")
+
+
+def bfp16_exponent(mx):
+    """Shared exponent E of a block floating-point row whose largest magnitude is mx (float32):
+    max(ex(mx) - 14, -126), ex the unbiased exponent field (-127 for zero and subnormals)."""
+    bits = np.asarray(mx, np.float32).view(np.uint32).astype(np.int64)
+    return np.maximum(((bits >> 23) & 0xFF) - 127 - 14, -126)
+
+
+def round_bfp16(x, nbins: int):
+    """Block floating-point storage of rows of nbins float32 values (numpy): each row becomes
+    int16 mantissas m times 2^E with E = bfp16_exponent(max |row|), m = rint(x 2^-E) (ties to
+    even) clamped to +-32767; returns the decoded float32 values m 2^E (exact)."""
+    x = np.ascontiguousarray(x, np.float32)
+    rows = x.reshape(-1, nbins)
+    e = bfp16_exponent(np.abs(rows).max(axis=1))[:, None]
+    inv = np.ldexp(np.float32(1.0), -e).astype(np.float32)
+    m = np.clip(np.rint(rows * inv), -32767, 32767).astype(np.float32)
+    return (m * np.ldexp(np.float32(1.0), e).astype(np.float32)).reshape(x.shape)
 
 
 def scale_round_storage(x, scale: float, storage: str, nthreads=None):
@@ -202,8 +225,19 @@ def scale_round_storage(x, scale: float, storage: str, nthreads=None):
     if x.dtype != np.float32 or not x.flags.c_contiguous:
         raise ValueError("scale_round_storage needs a C-contiguous float32 array")
     flat = x.reshape(-1)
+    if storage == "bfp16":
+        raise ValueError("bfp16 rounds whole rows: use round_bfp16_rows")
     lib().orc_round_storage(flat, flat.size, float(scale), _STORAGE_CODE[storage],
                             _nthreads(nthreads))
+    return x
+
+
+def round_bfp16_rows(x, nbins: int, nthreads=None):
+    """round_bfp16 in place on a float32 array of whole rows, in C with OpenMP."""
+    if x.dtype != np.float32 or not x.flags.c_contiguous or x.size % nbins:
+        raise ValueError("round_bfp16_rows needs a C-contiguous float32 array of whole rows")
+    flat = x.reshape(-1)
+    lib().orc_round_bfp16(flat, flat.size // nbins, nbins, _nthreads(nthreads))
     return x
 
 
@@ -223,6 +257,9 @@ def compute_ffa(ts_e, ts_v, cfg: SearchConfig, fourier: bool = False, nthreads=N
     storage 'f16' / 'bf16' (time domain): levels 0..L-2 are stored at that precision, each
     value rounded once from its FP32 sum at the exact scale 2^-(k0 + l) (merge = (t + h) * 0.5);
     the last level is returned in FP32 at its true scale, (t + h) * 2^(k0 + L - 2).
+    storage 'bfp16': level 0 (the brute fold) stays FP32, levels 1..L-2 are stored as block
+    floating point (round_bfp16 of each component row of the FP32 sum t + h, no scaling) and
+    the last level is the FP32 sum.
     """
     if storage != "f32":
         if fourier or f64:
@@ -258,6 +295,16 @@ def _compute_ffa_storage(ts_e, ts_v, cfg: SearchConfig, nthreads, storage: str):
     coords = plan.resolve_coordinates()
     freqs = plan.brute_fold_freqs()
     t_ref = plan.brute_fold_tref()
+    if storage == "bfp16":
+        fold = brute_fold_time(ts_e, ts_v, freqs, plan.segment_lens[0], cfg.nbins, cfg.tsamp,
+                               t_ref, nthreads)
+        last = plan.n_levels - 1
+        for lvl in range(1, plan.n_levels):
+            out = ffa_iter(fold, coords[lvl], plan.ncoords[lvl], plan.ncoords[lvl - 1],
+                           plan.nsegments[lvl], cfg.nbins, nthreads)
+            del fold
+            fold = round_bfp16_rows(out, cfg.nbins, nthreads) if lvl < last else out
+        return fold, plan
     k0 = storage_k0(cfg, plan)
     fold = brute_fold_time(ts_e, ts_v, freqs, plan.segment_lens[0], cfg.nbins, cfg.tsamp, t_ref,
                            nthreads)
@@ -293,7 +340,8 @@ def compute_ffa_scores(ts_e, ts_v, cfg: SearchConfig, widths=None, nthreads=None
 
 __all__ = [
     "FFA_COORD_DTYPE", "FFA_COORD_FREQ_DTYPE", "Plan", "SearchConfig", "brute_fold_fourier",
-    "brute_fold_time", "build", "circular_prefix_sum", "compute_ffa", "compute_ffa_scores",
+    "STORAGES", "bfp16_exponent", "brute_fold_time", "build", "circular_prefix_sum",
+    "compute_ffa", "compute_ffa_scores", "round_bfp16", "round_bfp16_rows",
     "diff_max", "ffa_iter", "irfft", "lib", "rfft", "round_storage", "snr_boxcar_3d",
     "scale_round_storage", "snr_boxcar_3d_f64", "storage_k0",
 ]

@@ -304,24 +304,30 @@ def test_errors_raise_runtime_error():
 # ---------------------------------------------------------------------------
 # Reduced-precision fold storage: bit-exact against the oracle's emulation
 # (same FP32 sums, each stored value rounded once to FP16 / BF16 at the scale
-# 2^-(k0 + l)); the S/N error against FP64 is stated in test_storage_snr_error.
+# 2^-(k0 + l), or each component row to block floating point with a shared
+# exponent); the S/N error against FP64 is stated in test_storage_snr_error.
 # ---------------------------------------------------------------------------
-@pytest.mark.parametrize("storage", ["f16", "bf16"])
+@pytest.mark.parametrize("storage", ["f16", "bf16", "bfp16"])
 @pytest.mark.parametrize("name", ["freq16", "acc17w", "acc20", "jerk17w", "freq18_nb100",
-                                  "acc_eta2_bseg"])
+                                  "acc_eta2_bseg", "acc17_nb128"])
 def test_reduced_storage_fold_bit_exact(oracle, name, storage):
     from loki_amd import libculoki
 
     cfg, ocfg = _cfgs(name)
     ts_e, ts_v = _series(cfg.nsamps, seed=31)
+    if storage == "bfp16" and cfg.nbins & (cfg.nbins - 1):
+        with pytest.raises(RuntimeError, match="power-of-two nbins"):
+            libculoki.ffa.compute_ffa_time_cuda(ts_e, ts_v, cfg, fold_storage=storage)
+        return
     got, _ = libculoki.ffa.compute_ffa_time_cuda(ts_e, ts_v, cfg, fold_storage=storage)
     want, _ = oracle.compute_ffa(ts_e, ts_v, ocfg, storage=storage)
     np.testing.assert_array_equal(got, want)
 
 
-@pytest.mark.parametrize("storage", ["f16", "bf16"])
+@pytest.mark.parametrize("storage", ["f16", "bf16", "bfp16"])
 @pytest.mark.parametrize("name", ["freq16", "acc17w", "acc20", "jerk17w", "jerk20",
-                                  "acc_eta2_bseg", "freq17_nb32", "jerk18_nb16", "acc_3lvl"])
+                                  "acc_eta2_bseg", "freq17_nb32", "jerk18_nb16", "acc_3lvl",
+                                  "acc16_nb8", "acc17_nb128"])
 @pytest.mark.parametrize("mode", ["unfused", "depth1", "depth2"])
 def test_reduced_storage_scores_bit_exact(oracle, name, storage, mode):
     from loki_amd import libculoki
@@ -342,8 +348,9 @@ def test_reduced_storage_scores_bit_exact(oracle, name, storage, mode):
 # Stated S/N error of each storage precision against the FP64 oracle (FP64 folds and FP64
 # scoring), on series with an injected pulsar (S/N 15 at the best width).  FP32 storage is
 # ~1e-6 (FP32 rounding only); FP16 keeps 11 significand bits, so a strong candidate carries
-# about S/N * 2^-12 per rounding (~5e-3 at S/N 15); BF16 (8 bits) about 8x more.
-STORAGE_SNR_BOUND = {"f32": 2e-5, "f16": 1e-2, "bf16": 8e-2}
+# about S/N * 2^-12 per rounding (~5e-3 at S/N 15); BF16 (8 bits) about 8x more.  Block
+# floating point keeps 15 bits relative to each row's largest value: within the 1e-3 bar.
+STORAGE_SNR_BOUND = {"f32": 2e-5, "f16": 1e-2, "bf16": 8e-2, "bfp16": 1e-3}
 
 
 @pytest.mark.parametrize("name", ["acc20", "jerk20"])

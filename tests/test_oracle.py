@@ -527,3 +527,99 @@ def test_reduced_storage_merge_is_scaled_sum(oracle):
     # FP16 storage perturbs the FP32 result by about 2^-12 relative per level
     assert np.max(np.abs(f16 - f32)) <= 2.0**-12 * plan.n_levels * np.max(np.abs(f32))
     assert not np.array_equal(f16, f32)
+
+
+# ---------------------------------------------------------------------------
+# Block floating-point storage (bfp16) emulation
+# ---------------------------------------------------------------------------
+def _bfp16_exact(row):
+    """Independent restatement in exact rationals: E from math.frexp of the row maximum,
+    m = round-half-even(x / 2^E) clamped to +-32767, value m * 2^E."""
+    import math
+    from fractions import Fraction
+
+    mx = max(abs(float(v)) for v in row)
+    if mx == 0.0 or mx < 2.0**-126:
+        e = -126
+    else:
+        _, k = math.frexp(mx)           # mx = f 2^k, f in [0.5, 1): ex = k - 1
+        e = max(k - 1 - 14, -126)
+    out = []
+    for v in row:
+        m = round(Fraction(float(v)) / Fraction(2) ** e)   # ties to even
+        m = max(-32767, min(32767, m))
+        out.append(float(Fraction(m) * Fraction(2) ** e))
+    return np.array(out, np.float32)
+
+
+def test_bfp16_known_values(oracle):
+    """Hand-worked rows: exponent from the row maximum, 15-bit mantissas, ties to even,
+    clamping at +-32767, zero and subnormal rows."""
+    r = np.array([3.0, 1e-5, -2.5, 0.0], np.float32)        # max 3: E = 1 - 14 = -13
+    np.testing.assert_array_equal(oracle.round_bfp16(r, 4), [3.0, 0.0, -2.5, 0.0])
+    assert int(oracle.bfp16_exponent(np.float32(3.0))) == -13
+    u = 2.0**-14                                             # max 1: E = -14
+    r = np.array([1.0, 0.5 * u, 1.5 * u, 2.5 * u], np.float32)
+    np.testing.assert_array_equal(oracle.round_bfp16(r, 4), [1.0, 0.0, 2 * u, 2 * u])
+    r = np.array([65535.5, -65535.5, 1.0, 3.0], np.float32)  # E = 1: 32767.75 -> 32767
+    np.testing.assert_array_equal(oracle.round_bfp16(r, 4), [65534.0, -65534.0, 0.0, 4.0])
+    np.testing.assert_array_equal(oracle.round_bfp16(np.zeros(4, np.float32), 4), np.zeros(4))
+    sub = np.full(4, 1e-40, np.float32)                      # E = -126: flushes to zero
+    np.testing.assert_array_equal(oracle.round_bfp16(sub, 4), np.zeros(4))
+    # rows are independent: a large row does not change its neighbour's exponent
+    two = np.array([1000.0, 1.0, 0.001, 0.5], np.float32)
+    np.testing.assert_array_equal(oracle.round_bfp16(two, 2)[2:],
+                                  oracle.round_bfp16(two[2:], 2))
+
+
+def test_bfp16_equals_exact_rationals_and_c(oracle):
+    """numpy and the OpenMP C rounding both equal the exact-rational restatement on random
+    rows spanning 60 binades (including rows of mixed magnitudes and exact ties)."""
+    rng = np.random.default_rng(17)
+    nbins = 16
+    rows = []
+    for k in range(300):
+        scale = 2.0 ** rng.integers(-30, 30)
+        r = (rng.standard_normal(nbins) * scale).astype(np.float32)
+        if k % 3 == 0:
+            r[rng.integers(0, nbins)] *= np.float32(2.0**12)   # one dominant bin
+        if k % 5 == 0:  # exact ties at the row's mantissa step
+            e = int(oracle.bfp16_exponent(np.abs(r).max()))
+            r[1] = np.float32((rng.integers(-1000, 1000) + 0.5) * 2.0**e)
+        rows.append(r)
+    x = np.concatenate(rows)
+    want = np.concatenate([_bfp16_exact(r) for r in rows])
+    np.testing.assert_array_equal(oracle.round_bfp16(x, nbins), want)
+    got = oracle.round_bfp16_rows(x.copy(), nbins)
+    np.testing.assert_array_equal(got, want)
+    # 15 significant bits relative to the row maximum
+    err = np.abs(want - x).reshape(-1, nbins).max(axis=1)
+    assert np.all(err <= np.abs(x).reshape(-1, nbins).max(axis=1) * 2.0**-15)
+
+
+def test_bfp16_tree_is_rounded_sums(oracle):
+    """The bfp16 tree: level 0 unrounded FP32, each stored level the row-rounded FP32 sum of
+    the level below, the last level the unrounded sum; much closer to FP32 than FP16 is."""
+    from oracle.plan import Plan, SearchConfig
+
+    cfg = SearchConfig(2**14, 64e-6, 16, 1.0, [(8.0, 12.0)], use_fourier=False)
+    rng = np.random.default_rng(5)
+    ts_e = rng.standard_normal(2**14, dtype=np.float32)
+    ts_v = np.ones(2**14, np.float32)
+    plan = Plan(cfg)
+    bfp, _ = oracle.compute_ffa(ts_e, ts_v, cfg, storage="bfp16")
+    # the same tree by hand from the oracle's pinned kernels
+    coords = plan.resolve_coordinates()
+    fold = oracle.brute_fold_time(ts_e, ts_v, plan.brute_fold_freqs(), plan.segment_lens[0],
+                                  cfg.nbins, cfg.tsamp, plan.brute_fold_tref())
+    for lvl in range(1, plan.n_levels):
+        fold = oracle.ffa_iter(fold, coords[lvl], plan.ncoords[lvl], plan.ncoords[lvl - 1],
+                               plan.nsegments[lvl], cfg.nbins)
+        if lvl < plan.n_levels - 1:
+            fold = oracle.round_bfp16(fold, cfg.nbins)
+    np.testing.assert_array_equal(bfp, fold)
+    f32, _ = oracle.compute_ffa(ts_e, ts_v, cfg)
+    f16, _ = oracle.compute_ffa(ts_e, ts_v, cfg, storage="f16")
+    d_bfp = np.max(np.abs(bfp - f32))
+    assert 0 < d_bfp <= 2.0**-15 * plan.n_levels * np.max(np.abs(f32))
+    assert d_bfp < np.max(np.abs(f16 - f32)) / 4
