@@ -438,6 +438,19 @@ int loki_brute_fold_time_device(const float* ts_e, const float* ts_v, size_t nsa
                                 size_t nbins, double tsamp, double t_ref, float* fold,
                                 void* stream);
 
+/* No counterpart in the reference: loki_brute_fold_time_device for callers that cannot allocate
+ * device memory (code that must not call hipMalloc/hipFree, such as tight serving loops or
+ * harnesses that forbid allocation).  Same arguments and results (bit-identical fold); the
+ * frequencies, bucket counts, offsets and lists live in a 64 MiB file-scope device array of the
+ * library instead of allocated buffers (the same (freq, bin) sample lists sized by the largest
+ * bucket per frequency: about nfreqs * segment_len * 4 bytes).  Returns LOKI_ERR_INVALID when
+ * they need more than 64 MiB.  Each GPU holds one copy of the array, so calls that overlap on
+ * one GPU race; each call synchronises `stream` before returning. */
+int loki_brute_fold_time_device_noalloc(const float* ts_e, const float* ts_v, size_t nsamps,
+                                        const double* freqs_host, size_t nfreqs,
+                                        size_t segment_len, size_t nbins, double tsamp,
+                                        double t_ref, float* fold, void* stream);
+
 /* compute_brute_fold_cuda<ComplexTypeCUDA> (fold_cuda.cu:260-277, kernels_cuda.cu:1012-1061):
  * fold has (nsamps/segment_len) * nfreqs * 2 * (nbins/2+1) complex64 values
  * (interleaved float pairs); nbins in [2, 256]. */

@@ -948,6 +948,20 @@ int loki_brute_fold_time_device(const float* ts_e, const float* ts_v, size_t nsa
     });
 }
 
+int loki_brute_fold_time_device_noalloc(const float* ts_e, const float* ts_v, size_t nsamps,
+                                        const double* freqs_host, size_t nfreqs,
+                                        size_t segment_len, size_t nbins, double tsamp,
+                                        double t_ref, float* fold, void* stream) {
+    return guarded([&] {
+        REQ(nfreqs < (1ULL << 32) && segment_len < (1ULL << 32),
+            "BruteFold: frequency count and segment length must be below 2^32");
+        loki_amd::kern::brute_fold_time_noalloc(
+            ts_e, ts_v, nsamps, freqs_host, static_cast<uint32_t>(nfreqs),
+            static_cast<uint32_t>(segment_len), static_cast<uint32_t>(nbins), tsamp, t_ref, fold,
+            as_stream(stream));
+    });
+}
+
 int loki_brute_fold_fourier_host(const float* ts_e, const float* ts_v, size_t nsamps,
                                  const double* freqs, size_t nfreqs, size_t segment_len,
                                  size_t nbins, double tsamp, double t_ref, float* fold,

@@ -11,6 +11,8 @@
 // to the CPU order and deterministic.  Reduced-precision storage rounds each FP32 bucket
 // sum once (storage.hpp).
 #include <algorithm>
+#include <array>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -62,6 +64,40 @@ k_bf_count(const double* __restrict__ freqs, uint32_t seglen, uint32_t nbins, do
     __syncthreads();
     if (tid == 0) {
         kmax[f] = s_max;
+    }
+}
+
+// Bucket offsets on the device (the allocation-free path): base[f] = sum over
+// f' < f of kmax[f'] * nbins -- the host loop of build_brute_fold_lists --
+// and *total the sum over all frequencies.  One block: each thread sums a
+// contiguous range of frequencies, then an inclusive scan of the partials.
+__global__ void __launch_bounds__(1024)
+k_bf_offsets(const uint32_t* __restrict__ kmax, uint32_t nfreqs, uint32_t nbins,
+             uint64_t* __restrict__ base, uint64_t* __restrict__ total) {
+    __shared__ uint64_t s_part[1024];
+    const uint32_t t   = threadIdx.x;
+    const uint64_t per = (static_cast<uint64_t>(nfreqs) + 1023) / 1024;
+    const uint64_t f0  = min(static_cast<uint64_t>(nfreqs), t * per);
+    const uint64_t f1  = min(static_cast<uint64_t>(nfreqs), f0 + per);
+    uint64_t sum = 0;
+    for (uint64_t f = f0; f < f1; ++f) {
+        sum += static_cast<uint64_t>(kmax[f]) * nbins;
+    }
+    s_part[t] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        const uint64_t v = t >= off ? s_part[t - off] : 0;
+        __syncthreads();
+        s_part[t] += v;
+        __syncthreads();
+    }
+    uint64_t run = s_part[t] - sum;
+    for (uint64_t f = f0; f < f1; ++f) {
+        base[f] = run;
+        run += static_cast<uint64_t>(kmax[f]) * nbins;
+    }
+    if (t == 1023) {
+        *total = s_part[1023];
     }
 }
 
@@ -587,7 +623,38 @@ BruteFoldLists build_brute_fold_lists(const double* freqs_dev, uint32_t nfreqs, 
     return L;
 }
 
+// Device scratch of the allocation-free brute fold (brute_fold_time_noalloc):
+// its frequencies, bucket counts, bucket maxima, list offsets and lists.  Each
+// GPU that loads the library holds its own copy until the process ends; host
+// code reaches it only through the address hipGetSymbolAddress returns on
+// that device (the array's name is a host-side stand-in).
+constexpr uint64_t kScratchBytes = 64ULL << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+std::array<void*, kMaxDevices> g_scratch_ptr{};
+
+/// Device address of `scratch` on the current device, looked up once per
+/// device (with that device current) and kept.
+unsigned char* scratch_address() {
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    LOKI_REQUIRE(dev >= 0 && dev < kMaxDevices, "brute fold scratch: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[static_cast<size_t>(dev)] == nullptr) {
+        void* p = nullptr;
+        hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)),
+                  "hipGetSymbolAddress(brute fold scratch)");
+        LOKI_REQUIRE(p != nullptr, "brute fold scratch: no device address");
+        g_scratch_ptr[static_cast<size_t>(dev)] = p;
+    }
+    return static_cast<unsigned char*>(g_scratch_ptr[static_cast<size_t>(dev)]);
+}
+
+constexpr uint64_t align256(uint64_t x) { return (x + 255) & ~static_cast<uint64_t>(255); }
 
 /// Segments per block of the list fold: as many as fit 32 KB of LDS (8 B per
 /// sample), at most 8; 1 selects the one-segment kernel.
@@ -602,10 +669,10 @@ int fold_segs_per_block(uint32_t seglen) {
 
 template <typename TO>
 void launch_fold(const float* ts_e, const float* ts_v, uint64_t blocks, uint32_t bps,
-                 const BruteFoldLists& L, TO* fold, float scale, hipStream_t s) {
+                 const BruteFoldView& L, TO* fold, float scale, hipStream_t s) {
     if (L.use_runs) {
         const auto g = static_cast<unsigned>(blocks);
-        const auto* fr = L.freqs.as<double>();
+        const auto* fr = L.freqs;
         k_bf_units<TO><<<g, kBlock, unit_lds_bytes(L.nbins), s>>>(
             ts_e, ts_v, L.seglen, L.nfreqs, L.nbins, bps, fr, L.tsamp, L.t_ref, fold, scale);
     } else if (fold_segs_per_block(L.seglen) > 1) {
@@ -615,9 +682,9 @@ void launch_fold(const float* ts_e, const float* ts_v, uint64_t blocks, uint32_t
         const uint64_t grid = (nseg + ns - 1) / ns * bps;
         const size_t lds    = static_cast<size_t>(ns) * L.seglen * sizeof(float2);
         const auto g        = static_cast<unsigned>(grid);
-        const auto* li = L.list.as<uint32_t>();
-        const auto* ba = L.base.as<uint64_t>();
-        const auto* co = L.counts.as<uint32_t>();
+        const auto* li = L.list;
+        const auto* ba = L.base;
+        const auto* co = L.counts;
         if (ns == 8) {
             k_bf_fold_ms<8, TO><<<g, kBlock, lds, s>>>(ts_e, ts_v, L.seglen, nseg, L.nfreqs,
                                                        L.nbins, bps, li, ba, co, fold, scale);
@@ -630,8 +697,7 @@ void launch_fold(const float* ts_e, const float* ts_v, uint64_t blocks, uint32_t
         }
     } else {
         k_bf_fold<true, TO><<<static_cast<unsigned>(blocks), kBlock, 0, s>>>(
-            ts_e, ts_v, L.seglen, L.nfreqs, L.nbins, bps, L.list.as<uint32_t>(),
-            L.base.as<uint64_t>(), L.counts.as<uint32_t>(), fold, scale);
+            ts_e, ts_v, L.seglen, L.nfreqs, L.nbins, bps, L.list, L.base, L.counts, fold, scale);
     }
 }
 
@@ -639,6 +705,12 @@ void launch_fold(const float* ts_e, const float* ts_v, uint64_t blocks, uint32_t
 
 void brute_fold_time(const float* ts_e, const float* ts_v, uint64_t nseg,
                      const BruteFoldLists& L, void* fold, FoldStorage so, float scale,
+                     hipStream_t s) {
+    brute_fold_time(ts_e, ts_v, nseg, L.view(), fold, so, scale, s);
+}
+
+void brute_fold_time(const float* ts_e, const float* ts_v, uint64_t nseg,
+                     const BruteFoldView& L, void* fold, FoldStorage so, float scale,
                      hipStream_t s) {
     const uint32_t bps    = brute_fold_blocks_per_segment(L);
     const uint64_t blocks = nseg * bps;
@@ -659,7 +731,88 @@ void brute_fold_time(const float* ts_e, const float* ts_v, uint64_t nseg,
     }
 }
 
+uint64_t brute_fold_scratch_bytes() { return kScratchBytes; }
+
+void brute_fold_time_noalloc(const float* ts_e, const float* ts_v, uint64_t nsamps,
+                             const double* freqs_host, uint32_t nfreqs, uint32_t seglen,
+                             uint32_t nbins, double tsamp, double t_ref, float* fold,
+                             hipStream_t s) {
+    LOKI_REQUIRE(ts_e != nullptr && ts_v != nullptr && freqs_host != nullptr && fold != nullptr,
+                 "null argument");
+    LOKI_REQUIRE(nfreqs >= 1, "BruteFold: Frequency array is empty");
+    LOKI_REQUIRE(seglen >= 1 && nsamps % seglen == 0,
+                 "BruteFold: Number of samples is not a multiple of segment length");
+    LOKI_REQUIRE(nbins >= 1 && nbins <= 1024, "BruteFold: nbins must be in [1, 1024]");
+    LOKI_REQUIRE(tsamp > 0.0, "brute fold: tsamp must be positive");
+    // scratch layout (256-byte aligned parts): freqs f64[nfreqs] | counts
+    // u32[nfreqs * nbins] | kmax u32[nfreqs] | total u64 | base u64[nfreqs] |
+    // list u32[total]
+    const uint64_t o_cnt   = align256(static_cast<uint64_t>(nfreqs) * sizeof(double));
+    const uint64_t o_kmax  = align256(o_cnt + static_cast<uint64_t>(nfreqs) * nbins * 4);
+    const uint64_t o_total = align256(o_kmax + static_cast<uint64_t>(nfreqs) * 4);
+    const uint64_t o_base  = align256(o_total + sizeof(uint64_t));
+    const uint64_t o_list  = align256(o_base + static_cast<uint64_t>(nfreqs) * sizeof(uint64_t));
+    BruteFoldView v;
+    v.nfreqs   = nfreqs;
+    v.seglen   = seglen;
+    v.nbins    = nbins;
+    v.tsamp    = tsamp;
+    v.t_ref    = t_ref;
+    v.use_runs = seglen > static_cast<uint32_t>(kLdsSamps);
+    const uint64_t need = v.use_runs ? static_cast<uint64_t>(nfreqs) * sizeof(double) : o_list;
+    if (need > kScratchBytes) {
+        throw_invalid("brute fold (no allocation): the frequencies and bucket tables need " +
+                          std::to_string(need) + " bytes, more than the " +
+                          std::to_string(kScratchBytes) + "-byte device scratch",
+                      __FILE__, __LINE__);
+    }
+    unsigned char* sc = scratch_address();
+    auto* d_freqs     = reinterpret_cast<double*>(sc);
+    hip_check(hipMemcpyAsync(d_freqs, freqs_host, static_cast<size_t>(nfreqs) * sizeof(double),
+                             hipMemcpyHostToDevice, s),
+              "copy brute-fold frequencies");
+    v.freqs = d_freqs;
+    if (!v.use_runs) {
+        auto* counts = reinterpret_cast<uint32_t*>(sc + o_cnt);
+        auto* kmax   = reinterpret_cast<uint32_t*>(sc + o_kmax);
+        auto* total  = reinterpret_cast<uint64_t*>(sc + o_total);
+        auto* base   = reinterpret_cast<uint64_t*>(sc + o_base);
+        auto* list   = reinterpret_cast<uint32_t*>(sc + o_list);
+        const uint32_t threads = ((nbins + 63) / 64) * 64;
+        // every count, maximum, offset and the total are written before they
+        // are read; list slots past a bucket's count are never read
+        k_bf_count<<<nfreqs, threads, 0, s>>>(d_freqs, seglen, nbins, tsamp, t_ref, counts, kmax);
+        hip_check(hipGetLastError(), "brute-fold count launch");
+        k_bf_offsets<<<1, 1024, 0, s>>>(kmax, nfreqs, nbins, base, total);
+        hip_check(hipGetLastError(), "brute-fold offsets launch");
+        uint64_t h_total = 0;
+        hip_check(hipMemcpyAsync(&h_total, total, sizeof(uint64_t), hipMemcpyDeviceToHost, s),
+                  "copy bucket total");
+        hip_check(hipStreamSynchronize(s), "brute-fold count");
+        const uint64_t need_list = o_list + h_total * 4;
+        if (need_list > kScratchBytes) {
+            throw_invalid("brute fold (no allocation): the bucket lists need " +
+                              std::to_string(need_list) + " bytes, more than the " +
+                              std::to_string(kScratchBytes) + "-byte device scratch",
+                          __FILE__, __LINE__);
+        }
+        k_bf_fill<<<nfreqs, threads, 0, s>>>(d_freqs, seglen, nbins, tsamp, t_ref, base, list);
+        hip_check(hipGetLastError(), "brute-fold fill launch");
+        v.counts = counts;
+        v.base   = base;
+        v.list   = list;
+    }
+    brute_fold_time(ts_e, ts_v, nsamps / seglen, v, fold, FoldStorage::kF32, 1.0F, s);
+    hip_check(hipGetLastError(), "brute fold launch");
+    // the next call on this device reuses the scratch: finish first
+    hip_check(hipStreamSynchronize(s), "brute fold");
+}
+
 uint32_t brute_fold_blocks_per_segment(const BruteFoldLists& L) {
+    return brute_fold_blocks_per_segment(L.view());
+}
+
+uint32_t brute_fold_blocks_per_segment(const BruteFoldView& L) {
     if (L.use_runs) {
         const uint32_t nf = unit_nf(L.nbins);
         return (L.nfreqs + nf - 1) / nf;

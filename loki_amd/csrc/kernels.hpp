@@ -43,6 +43,19 @@ void resolve_poly(const ResolveLevel& lv, FFACoord* out, uint64_t ncoords, hipSt
 /// ascending order, stored transposed: list[base[f] + k * nbins + b].
 /// Segments longer than the LDS staging limit use no lists: their buckets are
 /// walked as contiguous runs of samples (use_runs).
+/// Bucket lists of a brute fold as device pointers (owned elsewhere).
+struct BruteFoldView {
+    uint32_t nfreqs{0};
+    uint32_t seglen{0};
+    uint32_t nbins{0};
+    bool use_runs{false};
+    double tsamp{0.0};
+    double t_ref{0.0};
+    const double* freqs{nullptr};    ///< f64 [nfreqs]
+    const uint32_t* counts{nullptr};  ///< u32 [nfreqs * nbins]
+    const uint64_t* base{nullptr};    ///< u64 [nfreqs]
+    const uint32_t* list{nullptr};    ///< u32 [sum_f kmax_f * nbins]
+};
 struct BruteFoldLists {
     uint32_t nfreqs{0};
     uint32_t seglen{0};
@@ -54,6 +67,11 @@ struct BruteFoldLists {
     DeviceBuffer counts;  ///< u32 [nfreqs * nbins]
     DeviceBuffer base;    ///< u64 [nfreqs]
     DeviceBuffer list;    ///< u32 [sum_f kmax_f * nbins]
+    BruteFoldView view() const {
+        return {nfreqs, seglen, nbins, use_runs, tsamp, t_ref,
+                freqs.as<double>(), counts.as<uint32_t>(), base.as<uint64_t>(),
+                list.as<uint32_t>()};
+    }
 };
 /// FP64 phase bins (psr_utils.hpp:53-63) -> lists; synchronises `s`.
 BruteFoldLists build_brute_fold_lists(const double* freqs_dev, uint32_t nfreqs, uint32_t seglen,
@@ -64,6 +82,20 @@ BruteFoldLists build_brute_fold_lists(const double* freqs_dev, uint32_t nfreqs, 
 void brute_fold_time(const float* ts_e, const float* ts_v, uint64_t nseg,
                      const BruteFoldLists& lists, void* fold, FoldStorage so, float scale,
                      hipStream_t s);
+void brute_fold_time(const float* ts_e, const float* ts_v, uint64_t nseg,
+                     const BruteFoldView& lists, void* fold, FoldStorage so, float scale,
+                     hipStream_t s);
+/// Bytes of the brute fold's file-scope device scratch (allocation-free entry point).
+uint64_t brute_fold_scratch_bytes();
+/// FP32 time brute fold of nsamps / seglen segments with its frequencies,
+/// bucket counts, offsets and lists in the file-scope scratch array instead
+/// of allocated buffers: allocates no device memory.  Throws if they need
+/// more than brute_fold_scratch_bytes().  Synchronises `s`.  Calls that
+/// overlap on one device race on the scratch.
+void brute_fold_time_noalloc(const float* ts_e, const float* ts_v, uint64_t nsamps,
+                             const double* freqs_host, uint32_t nfreqs, uint32_t seglen,
+                             uint32_t nbins, double tsamp, double t_ref, float* fold,
+                             hipStream_t s);
 
 /// Time-domain merge of one level (freq-only when coords_freq != nullptr);
 /// n_f: frequency grid points of the level (coordinates are rows of n_f).
@@ -261,6 +293,7 @@ void resolve_grouped(const ResolveTask* tasks, uint32_t ntasks, uint32_t nblocks
 /// Blocks per segment of the time brute fold of `lists` (list or run path;
 /// tasks of one grouped launch must share the path).
 uint32_t brute_fold_blocks_per_segment(const BruteFoldLists& lists);
+uint32_t brute_fold_blocks_per_segment(const BruteFoldView& lists);
 /// Grouped time brute fold (all tasks stored as `so`): the list path, or with
 /// `runs` the long-segment run path (max_nbins: the largest nbins of the
 /// tasks, which sizes its LDS).
