@@ -41,11 +41,22 @@ namespace {
 constexpr uint32_t kLanes = 64;
 constexpr uint32_t kBlock = 4 * kLanes;  // largest block: 4 rows
 
-/// Staged row: the NB (e, v) pairs twice over, so the row rotated by any
-/// shift S is the contiguous run starting at pair NB - S, plus one pair of
-/// padding (rows start on alternating LDS banks).
+/// Repeated leading bins of a staged row: a rotated read of 4 bins starting
+/// at any bin stays inside the row.
+constexpr uint32_t kPad = 3;
+
+/// (e, v) pairs per staged row: at least NB + kPad, and 5 mod 16.  Lane l of
+/// a wave reads bin k of its rotated parent row, at pair r_l * stride +
+/// (k - S_l) mod NB; along a grid row the parent index r_l and the shift S_l
+/// both advance by about half a step per trial at the FFA-native spacing
+/// (more shift per trial for coarser spacings), so lanes whose rows differ
+/// by d and shifts by about d, 2d, 3d or 4d must land on different LDS bank
+/// pairs: stride * d - m * d != 0 mod 16 for m = 0..4.  (A stride of 1 mod 16
+/// makes every lane of such a wave collide: 45 % of the kernel's cycles were
+/// bank conflicts.)
 template <int NB>
-constexpr uint32_t kStride = 2 * NB + 1;
+constexpr uint32_t kStride = ((NB + kPad + 10) / 16) * 16 + 5;
+static_assert(kStride<32> == 37 && kStride<16> == 21 && kStride<8> == 21);
 
 __device__ __forceinline__ uint32_t wave_min(uint32_t x) {
 #pragma unroll
@@ -308,7 +319,9 @@ k_final2(Final2Args a) {
                 const uint32_t b = wrap<NB>(b0 + i);
                 const float2 pr  = make_float2(ev[i][0], ev[i][1]);
                 dst[b]           = pr;
-                dst[NB + b]      = pr;
+                if (b < kPad) {
+                    dst[NB + b] = pr;
+                }
             }
         }
     }
@@ -366,11 +379,9 @@ k_final2(Final2Args a) {
     {
         const float2* trow = s_rows + (cd.i_tail - lo_t) * STRIDE;
         const float2* hrow = s_rows + (a.cap + (cd.i_head - lo_h)) * STRIDE;
-        const float2* tb = trow + (NB - St);  // rotated rows: contiguous runs
-        const float2* hb = hrow + (NB - Sh);
         prefix([&](uint32_t c, float4& te, float4& tv, float4& he, float4& hv) {
-            const float2* tp = tb + 4 * c;
-            const float2* hp = hb + 4 * c;
+            const float2* tp = trow + wrap<NB>(4 * c + NB - St);
+            const float2* hp = hrow + wrap<NB>(4 * c + NB - Sh);
             const float2 t0 = tp[0], t1 = tp[1], t2 = tp[2], t3 = tp[3];
             const float2 h0 = hp[0], h1 = hp[1], h2 = hp[2], h3 = hp[3];
             te = make_float4(t0.x, t1.x, t2.x, t3.x);
@@ -503,7 +514,8 @@ void final2_ranges(const Final2Args& a, uint32_t nblocks, uint32_t* out, hipStre
 bool final2_supported(uint32_t nbins) { return nbins == 8 || nbins == 16 || nbins == 32; }
 
 uint32_t final2_lds_bytes(uint32_t nbins, uint32_t cap) {
-    return 2U * cap * (2 * nbins + 1) * static_cast<uint32_t>(sizeof(float2));
+    const uint32_t stride = nbins == 32 ? kStride<32> : (nbins == 16 ? kStride<16> : kStride<8>);
+    return 2U * cap * stride * static_cast<uint32_t>(sizeof(float2));
 }
 
 void final2_merge_score(const Final2Args& a, FoldStorage si, uint32_t nblocks, bool find_max,
