@@ -28,6 +28,7 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
+METRIC = "FFA param trials/s + % HBM roofline; FP16-store vs FP32 at |\u0394S/N|<1e-3"
 HBM_PEAK_GBPS = 8000.0
 MFMA_F32_PEAK_TFLOPS = 157.3
 # VALU issue: 256 CUs x 4 SIMDs, a wave64 instruction every 4 cycles (16 lanes per cycle) at
@@ -45,6 +46,9 @@ WORKLOADS = {
         # CPU baseline on the whole workload: ~10 s on 16 threads, ~75 GB of host memory
         # (two fold levels, the shift tables and the scores).
         cpu_sample=dict(param_limits=[(-50.0, 50.0), (100.0, 400.0)], bseg_brute=1024),
+        # S/N error of each fold storage against an FP64 oracle fold: the full accel range
+        # over a 2 Hz band around the injection
+        storage_sample=dict(param_limits=[(-50.0, 50.0), (199.0, 201.0)]),
     ),
     # Fourier-domain FFA over (fddot, fdot, f) expressed as (jerk, accel, freq) at f_min = 50 Hz:
     # accel = c * fdot / f_min, jerk = c * fddot / f_min for |fdot| <= 1e-8 Hz/s,
@@ -842,6 +846,31 @@ def main() -> int:
                     "tests/test_gpu_parity.py::test_storage_snr_error_vs_fp64",
         }
 
+    # The same step with the hardware reciprocal square root in the profile normalisation
+    # (|dS/N| <= 1e-4 against the exact scores, tests/test_gpu_parity.py and
+    # tests/test_gpu_configs.py), beside the bit-exact default.
+    norm_cmp = None
+    if not fourier and not args.fast_norm:
+        ffa.set_exact_norm(False)
+        with torch.cuda.stream(stream):
+            step()
+        torch.cuda.synchronize()
+        tn0 = time.perf_counter()
+        with torch.cuda.stream(stream):
+            for _ in range(args.steps):
+                step()
+        torch.cuda.synchronize()
+        n_ms = (time.perf_counter() - tn0) * 1e3 / args.steps
+        ffa.set_exact_norm(True)
+        with torch.cuda.stream(stream):
+            step()  # exact scores again for the recovery check
+        torch.cuda.synchronize()
+        norm_cmp = {"exact_ms_per_step": round(ms_per_step, 4),
+                    "hw_rsqrt_ms_per_step": round(n_ms, 4),
+                    "hw_rsqrt_param_trials_per_s": nc / (n_ms * 1e-3),
+                    "note": "exact: correctly rounded e * (1/sqrt(v)), bit-identical to the CPU "
+                            "oracle; hw_rsqrt: e * rsqrt(v), |dS/N| <= 1e-4"}
+
     # Score-and-filter (the FFAFreqSweep step, ffa_freq_sweep_cuda.cu:219-260): the same FFA with
     # only the (trial, width) pairs above the threshold leaving the last-level kernel.
     score_filter = None
@@ -910,7 +939,7 @@ def main() -> int:
     total_alg = pipeline_bytes(plan, nbins, nw, wl["nsamps"], fourier)
 
     result = {
-        "metric": "FFA param trials/s + % HBM roofline",
+        "metric": METRIC,
         "value": nc / (ms_per_step * 1e-3),
         "unit": "param trials/s",
         "n_gpus": 1,
@@ -939,6 +968,7 @@ def main() -> int:
         "unfused_alg_bytes": total_alg,
         "kernel_ms": kern_ms,
         "storage_compare": storage_cmp,
+        "norm_compare": norm_cmp,
         "score_filter": score_filter,
         "pruning_seeds": seeds,
         "found": found,

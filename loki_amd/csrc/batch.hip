@@ -299,6 +299,9 @@ void FFABatch::build_tasks() {
                                   static_cast<uint32_t>(block0), append(blob, tasks), stride});
         }
     }
+    // a previous execute() on any stream may still read the old task tables:
+    // let the device finish before they are replaced
+    hip_check(hipDeviceSynchronize(), "batch: wait before rewriting the task tables");
     m_tasks.reserve(std::max<SizeType>(blob.size(), 1));
     hip_check(hipMemcpy(m_tasks.get(), blob.data(), blob.size(), hipMemcpyHostToDevice),
               "copy batch task tables");

@@ -56,6 +56,27 @@ void DeviceBuffer::reserve(SizeType bytes) {
     m_bytes = bytes;
 }
 
+FFAWorkspace::~FFAWorkspace() {
+    if (m_done != nullptr) {
+        (void)hipEventDestroy(m_done);
+    }
+}
+
+void FFAWorkspace::begin_use(hipStream_t s) {
+    if (m_recorded && m_last != s) {
+        hip_check(hipStreamWaitEvent(s, m_done, 0), "workspace: wait for the previous stream");
+    }
+}
+
+void FFAWorkspace::end_use(hipStream_t s) {
+    if (m_done == nullptr) {
+        hip_check(hipEventCreateWithFlags(&m_done, hipEventDisableTiming), "hipEventCreate");
+    }
+    hip_check(hipEventRecord(m_done, s), "workspace: record use");
+    m_last     = s;
+    m_recorded = true;
+}
+
 DeviceGuard::DeviceGuard(int device) {
     int count = 0;
     hip_check(hipGetDeviceCount(&count), "hipGetDeviceCount");
@@ -368,12 +389,14 @@ void FFAExecutor::execute(const float* ts_e, const float* ts_v, void* fold, Size
     LOKI_REQUIRE(fold_len == m_plan.buffer_size(),
                  "FFA::execute: fold must have size buffer_size");
     const SizeType L = m_plan.n_levels();
+    m_ws->begin_use(s);
     m_ws->internal.reserve(m_plan.buffer_size() * elem_bytes());
     if ((L - 1) % 2 == 0) {
         run_levels(ts_e, ts_v, fold, m_ws->internal.get(), L - 1, true, s);
     } else {
         run_levels(ts_e, ts_v, m_ws->internal.get(), fold, L - 1, true, s);
     }
+    m_ws->end_use(s);
 }
 
 void FFAExecutor::execute_time(const float* ts_e, const float* ts_v, float* fold,
@@ -388,6 +411,7 @@ void FFAExecutor::execute_time(const float* ts_e, const float* ts_v, float* fold
     // The caller's buffer (buffer_size complex elements) is the ping-pong
     // partner; the last level lands in the internal buffer and is returned to
     // the time domain into the caller's buffer (ffa_cuda.cu:248-290).
+    m_ws->begin_use(s);
     m_ws->internal.reserve(m_plan.buffer_size() * elem_bytes());
     const void* fin = ((L - 1) % 2 == 0)
                           ? run_levels(ts_e, ts_v, m_ws->internal.get(), fold, L - 1, true, s)
@@ -398,6 +422,7 @@ void FFAExecutor::execute_time(const float* ts_e, const float* ts_v, float* fold
                      static_cast<uint32_t>(nbins), s);
     check_launch("irfft launch");
     m_timer.stop(slot, s);
+    m_ws->end_use(s);
 }
 
 bool FFAExecutor::fused_scores_supported() const {
@@ -513,6 +538,15 @@ void FFAExecutor::execute_scores(const float* ts_e, const float* ts_v,
                                  SizeType scores_len, bool find_max, hipStream_t s,
                                  const kern::FilterOut* fo) {
     DeviceGuard g(m_device);
+    m_ws->begin_use(s);
+    scores_levels(ts_e, ts_v, widths, scores, scores_len, find_max, s, fo);
+    m_ws->end_use(s);
+}
+
+void FFAExecutor::scores_levels(const float* ts_e, const float* ts_v,
+                                std::span<const SizeType> widths, float* scores,
+                                SizeType scores_len, bool find_max, hipStream_t s,
+                                const kern::FilterOut* fo) {
     LOKI_REQUIRE(m_plan.nsegments().back() == 1,
                  "compute_ffa_scores: nsegments must be 1 for scores");
     LOKI_REQUIRE(ts_e != nullptr && ts_v != nullptr, "FFA::execute: null buffer");

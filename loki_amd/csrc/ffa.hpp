@@ -66,10 +66,27 @@ private:
 /// Device buffers an FFA borrows (workspace.hpp:169-206 FFAWorkspaceCUDA): the
 /// ping-pong partner, the second scores buffer and the shift tables.  FFAs
 /// that share one run one at a time; each grows the buffers to its plan.
+/// Uses on different streams are ordered: a use on stream s first waits for
+/// an event recorded at the end of the previous use on another stream.
 struct FFAWorkspace {
+    FFAWorkspace() = default;
+    ~FFAWorkspace();
+    FFAWorkspace(const FFAWorkspace&)            = delete;
+    FFAWorkspace& operator=(const FFAWorkspace&) = delete;
+
     DeviceBuffer internal;
     DeviceBuffer result;
     DeviceBuffer coords;
+
+    /// Before enqueueing work that touches the buffers on `s`.
+    void begin_use(hipStream_t s);
+    /// After enqueueing it.
+    void end_use(hipStream_t s);
+
+private:
+    hipEvent_t m_done{nullptr};
+    hipStream_t m_last{nullptr};
+    bool m_recorded{false};
 };
 
 class FFAExecutor {
@@ -182,6 +199,10 @@ private:
     void init_level(const float* ts_e, const float* ts_v, void* out, void* scratch,
                     bool fp32_out, hipStream_t s);
     SizeType max_level_size(SizeType last_level) const;
+    /// execute_scores without the workspace ordering.
+    void scores_levels(const float* ts_e, const float* ts_v, std::span<const SizeType> widths,
+                       float* scores, SizeType scores_len, bool find_max, hipStream_t s,
+                       const kern::FilterOut* fo);
     /// Storage of level l in a run whose last level is last_level (returned
     /// in FP32 when last_fp32).
     FoldStorage stored_as(SizeType l, SizeType last_level, bool last_fp32) const;

@@ -58,3 +58,33 @@ def test_shared_workspace_fourier():
     f2 = np.zeros(out["freq16"].plan.buffer_size, np.complex64)
     out["freq16"].execute(*ts, f2)
     assert np.isfinite(f2).all()
+
+
+def test_shared_workspace_two_streams_ordered():
+    """Two FFAs sharing one workspace, enqueued back to back on two different streams with no
+    host synchronisation between: the second use waits for the first (an event recorded at
+    the end of each use), so both score arrays equal those of private buffers."""
+    import torch
+
+    from loki_amd import libculoki
+
+    ws = libculoki.ffa.FFAWorkspaceCUDA(device_id=0)
+    dev = torch.device("cuda:0")
+    runs = []
+    for n, seed in (("acc17w", 51), ("jerk17w", 52)):
+        cfg, _ = _cfgs(n)
+        e, v = _series(cfg.nsamps, seed=seed)
+        widths = np.array([w for w in (1, 2, 4, 8) if w < cfg.nbins], np.uint64)
+        want = libculoki.ffa.FFACUDA(cfg, 0).scores(e, v, widths)
+        ffa = libculoki.ffa.FFACUDA(cfg, 0, workspace=ws)
+        out = torch.empty(want.size, dtype=torch.float32, device=dev)
+        runs.append((ffa, torch.from_numpy(e).to(dev), torch.from_numpy(v).to(dev), widths,
+                     out, want, torch.cuda.Stream(device=dev)))
+    torch.cuda.synchronize()
+    for _ in range(3):
+        for ffa, de, dv, widths, out, want, st in runs:
+            ffa.scores_device(de.data_ptr(), dv.data_ptr(), widths, out.data_ptr(), out.numel(),
+                              False, st.cuda_stream)
+        torch.cuda.synchronize()
+        for ffa, de, dv, widths, out, want, st in runs:
+            np.testing.assert_array_equal(out.cpu().numpy(), want)
