@@ -220,6 +220,11 @@ int loki_ffa_set_fused(loki_ffa_t ffa, int enable);
  * chip from the level below it and never stored.  Same bits either way; other nbins keep the
  * one-level fused kernel.  No reference counterpart (ffa_cuda.cu:650-692 stores every level). */
 int loki_ffa_set_fuse_depth(loki_ffa_t ffa, int depth);
+/* Time-domain tree levels merged two at a time where the storage allows (default 1): one pass
+ * reads four rows of level l - 1 per output row of level l + 1 and never stores level l.  Same
+ * bits as one pass per level (0).  No reference counterpart (kernels_cuda.cu:280-383 merges one
+ * level per launch). */
+int loki_ffa_set_two_level_merges(loki_ffa_t ffa, int enable);
 /* Profile normalisation in the fused scores path: 1 (default) the CPU's correctly rounded
  * e * (1/sqrt(v)), bit-identical to the reference CPU; 0 the hardware reciprocal square root
  * (|dS/N| ~ 1e-6, the analogue of the reference CUDA path's fast-math e/sqrtf(v),

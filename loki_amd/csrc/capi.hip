@@ -571,6 +571,13 @@ int loki_ffa_set_fuse_depth(loki_ffa_t ffa, int depth) {
     });
 }
 
+int loki_ffa_set_two_level_merges(loki_ffa_t ffa, int enable) {
+    return guarded([&] {
+        REQ(ffa != nullptr, "null ffa");
+        ffa->exec.set_two_level_merges(enable != 0);
+    });
+}
+
 int loki_ffa_set_dft_precision(loki_ffa_t ffa, int mode) {
     return guarded([&] {
         REQ(ffa != nullptr, "null argument");

@@ -338,39 +338,97 @@ void* FFAExecutor::run_levels(const float* ts_e, const float* ts_v, void* buf_ev
     const SizeType nbins = m_plan.config().get_nbins();
     init_level(ts_e, ts_v, even, odd, last_fp32 && last_level == 0, s);
 
-    for (SizeType l = 1; l <= last_level; ++l) {
+    auto tables = [&](SizeType l, const FFACoord*& cp, const FFACoordFreq*& cfp) {
         const SizeType off = m_plan.ncoords_offsets64()[l];
-        const FFACoord* cp = nullptr;
-        const FFACoordFreq* cfp = nullptr;
+        cp                 = nullptr;
+        cfp                = nullptr;
         if (m_freq_only) {
             cfp = m_ws->coords.as<FFACoordFreq>() + off;
         } else {
             cp = m_ws->coords.as<FFACoord>() + off;
         }
-        void* in       = (l % 2 == 1) ? even : odd;
-        void* out      = (l % 2 == 1) ? odd : even;
-        const int slot = m_timer.start(KernelKind::kMerge, static_cast<int>(l), s);
-        if (m_fourier) {
-            kern::merge_fourier(static_cast<const float2*>(in), static_cast<float2*>(out), cp, cfp,
-                                m_plan.ncoords()[l], m_plan.ncoords()[l - 1],
-                                m_plan.nsegments()[l], static_cast<uint32_t>(nbins), s);
-        } else {
-            // FP16/BF16: (tail + head) * 0.5 keeps level l at 2^-(k0 + l); a last
-            // level returned in FP32 is unscaled by 2^(k0 + l - 1) instead.
-            // FP32 and block floating point are unscaled (scale unused).
-            const bool to32      = last_fp32 && l == last_level;
+    };
+    // Pass plan: levels l and l + 1 merged in one two-level pass where allowed
+    // (time domain, both stored in the tree's storage), pairing from the top
+    // down; the number of two-level passes is kept even, so that level k
+    // still lands in buf_even for even k and in buf_odd for odd k.
+    std::vector<uint8_t> two(last_level + 2, 0);  // two[l]: levels l, l+1 in one pass
+    if (!m_fourier && m_two_level && kern::merge2_supported(static_cast<uint32_t>(nbins))) {
+        std::vector<SizeType> starts;
+        for (SizeType top = last_level; top >= 2;) {
+            const SizeType l     = top - 1;
             const FoldStorage si = stored_as(l - 1, last_level, last_fp32);
-            const FoldStorage so = stored_as(l, last_level, last_fp32);
-            const float scale =
-                to32 ? std::ldexp(1.0F, m_k0 + static_cast<int>(l) - 1) : 0.5F;
-            kern::merge_time(in, out, cp, cfp, m_plan.ncoords()[l], m_plan.ncoords()[l - 1],
-                             m_plan.nsegments()[l], m_plan.param_counts()[l].back(),
-                             static_cast<uint32_t>(nbins), si, so, scale, s);
+            const FoldStorage sb = stored_as(l, last_level, last_fp32);
+            const FoldStorage so = stored_as(top, last_level, last_fp32);
+            const bool to32      = last_fp32 && top == last_level;
+            // only where the tree does not grow: a pass whose output level is
+            // larger than its input level (the accel/jerk grid refining) gathers
+            // its four input rows from a level half its size and measured slower
+            // than two single passes (2^23-sample accel search, levels 10-11:
+            // 1.97 ms against 0.47 + 1.28 ms)
+            const bool grows = m_plan.level_size(top) > m_plan.level_size(l - 1) + m_plan.level_size(l - 1) / 4;
+            if (to32 || grows || sb != so ||
+                !(si == so || (si == FoldStorage::kF32 && so == FoldStorage::kBFP16))) {
+                top -= 1;  // this pair cannot be merged in one pass; try one lower
+                continue;
+            }
+            starts.push_back(l);
+            top -= 2;
         }
-        check_launch("merge launch");
-        m_timer.stop(slot, s);
+        if (starts.size() % 2 == 1) {
+            starts.pop_back();  // the lowest pair
+        }
+        for (const SizeType l : starts) {
+            two[l] = 1;
+        }
     }
-    return (last_level % 2 == 0) ? even : odd;
+    void* cur   = even;
+    void* other = odd;
+    for (SizeType l = 1; l <= last_level;) {
+        const FFACoord* cp      = nullptr;
+        const FFACoordFreq* cfp = nullptr;
+        tables(l, cp, cfp);
+        if (two[l] != 0) {
+            const FFACoord* cp2      = nullptr;
+            const FFACoordFreq* cfp2 = nullptr;
+            tables(l + 1, cp2, cfp2);
+            const int slot = m_timer.start(KernelKind::kMerge, static_cast<int>(l + 1), s);
+            kern::merge_time2(cur, other, cp, cfp, cp2, cfp2, m_plan.ncoords()[l + 1],
+                              m_plan.ncoords()[l], m_plan.ncoords()[l - 1],
+                              m_plan.nsegments()[l + 1], m_plan.param_counts()[l + 1].back(),
+                              static_cast<uint32_t>(nbins), stored_as(l - 1, last_level, last_fp32),
+                              stored_as(l + 1, last_level, last_fp32), 0.5F, s);
+            check_launch("two-level merge launch");
+            m_timer.stop(slot, s);
+            l += 2;
+        } else {
+            const int slot = m_timer.start(KernelKind::kMerge, static_cast<int>(l), s);
+            if (m_fourier) {
+                kern::merge_fourier(static_cast<const float2*>(cur), static_cast<float2*>(other),
+                                    cp, cfp, m_plan.ncoords()[l], m_plan.ncoords()[l - 1],
+                                    m_plan.nsegments()[l], static_cast<uint32_t>(nbins), s);
+            } else {
+                // FP16/BF16: (tail + head) * 0.5 keeps level l at 2^-(k0 + l); a last
+                // level returned in FP32 is unscaled by 2^(k0 + l - 1) instead.
+                // FP32 and block floating point are unscaled (scale unused).
+                const bool to32      = last_fp32 && l == last_level;
+                const FoldStorage si = stored_as(l - 1, last_level, last_fp32);
+                const FoldStorage so = stored_as(l, last_level, last_fp32);
+                const float scale =
+                    to32 ? std::ldexp(1.0F, m_k0 + static_cast<int>(l) - 1) : 0.5F;
+                kern::merge_time(cur, other, cp, cfp, m_plan.ncoords()[l],
+                                 m_plan.ncoords()[l - 1], m_plan.nsegments()[l],
+                                 m_plan.param_counts()[l].back(), static_cast<uint32_t>(nbins),
+                                 si, so, scale, s);
+            }
+            check_launch("merge launch");
+            m_timer.stop(slot, s);
+            l += 1;
+        }
+        std::swap(cur, other);
+    }
+    LOKI_REQUIRE(cur == ((last_level % 2 == 0) ? even : odd), "merge passes: wrong landing buffer");
+    return cur;
 }
 
 SizeType FFAExecutor::max_level_size(SizeType last_level) const {
@@ -487,18 +545,16 @@ void FFAExecutor::prepare_final2(hipStream_t s) {
         groups.emplace_back(static_cast<uint32_t>(r), static_cast<uint32_t>(e - r));
         r = e;
     }
-    std::vector<uint32_t> sizes;
-    for (const auto& g : groups) {
-        sizes.push_back(g.second);
-    }
-    std::nth_element(sizes.begin(), sizes.begin() + sizes.size() / 2, sizes.end());
-    const uint32_t med = sizes[sizes.size() / 2];
-    m_f2_rows_per_block = med >= 4 ? 4U : (med >= 2 ? 2U : 1U);
+    // chunks of up to 16 rows of one group: a block stages the group's parent
+    // rows once for all its rows (its four waves take 4 rows each)
+    constexpr uint32_t kChunkRows = 16;
+    m_f2_rows_per_block = 1;
     m_f2_chunks_host.clear();
     for (const auto& g : groups) {
-        for (uint32_t o = 0; o < g.second; o += m_f2_rows_per_block) {
-            m_f2_chunks_host.push_back(
-                {g.first + o, std::min(m_f2_rows_per_block, g.second - o)});
+        for (uint32_t o = 0; o < g.second; o += kChunkRows) {
+            const uint32_t n = std::min(kChunkRows, g.second - o);
+            m_f2_chunks_host.push_back({g.first + o, n});
+            m_f2_rows_per_block = std::max(m_f2_rows_per_block, n);
         }
     }
     const uint32_t n_ft    = (n_f + 63) / 64;

@@ -149,6 +149,10 @@ public:
     /// Storage precision of fold levels 0..L-2 (time domain only); the final
     /// level is always returned and scored in FP32.  Default FP32.
     void set_fold_storage(FoldStorage st);
+    /// Merge pairs of tree levels in one two-level pass where allowed
+    /// (default on); off: one pass per level (same bits).
+    void set_two_level_merges(bool on) noexcept { m_two_level = on; }
+    bool two_level_merges() const noexcept { return m_two_level; }
     FoldStorage fold_storage() const noexcept { return m_storage; }
     bool fused_scores_supported() const;
 
@@ -170,6 +174,7 @@ private:
     bool m_exact_norm{true};
     bool m_dft_split{false};
     FoldStorage m_storage{FoldStorage::kF32};
+    bool m_two_level{true};  ///< merge pairs of levels in one pass where allowed
     int m_k0{0};  // level l of a reduced-precision tree is stored at scale 2^-(k0 + l)
     int m_fuse_depth{2};
     LaunchTimer m_timer;

@@ -39,7 +39,13 @@ namespace loki_amd::kern {
 namespace {
 
 constexpr uint32_t kLanes = 64;
-constexpr uint32_t kBlock = 4 * kLanes;  // largest block: 4 rows
+constexpr uint32_t kWaves = 4;
+constexpr uint32_t kBlock = kWaves * kLanes;
+/// Grid rows per block (rows sharing their parent rows): wave w takes rows
+/// w, w + 4, w + 8 and w + 12 of its chunk, so the staged parent rows serve
+/// up to 16 rows of trials.
+constexpr uint32_t kRowsPerWave = 4;
+constexpr uint32_t kMaxRows     = kWaves * kRowsPerWave;
 
 /// Repeated leading bins of a staged row: a rotated read of 4 bins starting
 /// at any bin stays inside the row.
@@ -110,14 +116,16 @@ __device__ __forceinline__ void parent_record(const Final2Args& a, uint32_t k, u
 }
 
 /// max_j (hi_j - ps[j]) for the fixed width W, hi_j = ps[j + W] or, past the
-/// end of the row, ext[j + W - NB] = ps[j + W - NB] + sum.
+/// end of the row, ps[j + W - NB] + sum (the circular extension of
+/// score.cpp:143-156), formed where it is used: holding the NB - 1 extended
+/// values across the widths costs 31 VGPRs (96 instead of 71, 5 waves per
+/// SIMD instead of 7) for 45 fewer adds per trial.
 template <int NB, int W>
-__device__ __forceinline__ float window_max_fixed(const float (&ps)[NB],
-                                                  const float (&ext)[NB - 1]) {
+__device__ __forceinline__ float window_max_fixed(const float (&ps)[NB]) {
     float ma = -FLT_MAX, mb = -FLT_MAX;
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
-        const float hi = (j + W < NB) ? ps[(j + W) % NB] : ext[(j + W - NB) % (NB - 1)];
+        const float hi = (j + W < NB) ? ps[(j + W) % NB] : ps[(j + W - NB) % NB] + ps[NB - 1];
         const float d  = hi - ps[j];
         if ((j & 2) == 0) {
             ma = __builtin_fmaxf(ma, d);
@@ -130,10 +138,9 @@ __device__ __forceinline__ float window_max_fixed(const float (&ps)[NB],
 
 template <int NB, int... Ws>
 __device__ __forceinline__ float window_max_reg(std::integer_sequence<int, Ws...>,
-                                                const float (&ps)[NB],
-                                                const float (&ext)[NB - 1], uint32_t w) {
+                                                const float (&ps)[NB], uint32_t w) {
     float r = 0.0F;
-    (void)((w == static_cast<uint32_t>(Ws + 1) ? (r = window_max_fixed<NB, Ws + 1>(ps, ext), true)
+    (void)((w == static_cast<uint32_t>(Ws + 1) ? (r = window_max_fixed<NB, Ws + 1>(ps), true)
                                                : false) ||
            ...);
     return r;
@@ -161,7 +168,7 @@ k_final2(Final2Args a) {
     extern __shared__ float2 s_rows[];            // [2 * cap][STRIDE]
     __shared__ uint32_t s_rng[4];                 // lo_t, hi_t, lo_h, hi_h
 
-    const uint32_t wave = threadIdx.x / kLanes;  // row of the chunk
+    const uint32_t wave = threadIdx.x / kLanes;  // rows wave, wave + 4, ... of the chunk
     const uint32_t lane = threadIdx.x % kLanes;  // frequency of the tile
     const uint32_t nthr = blockDim.x;
     const uint32_t id   = xcd_remap(blockIdx.x, gridDim.x);
@@ -174,24 +181,33 @@ k_final2(Final2Args a) {
     const uint32_t nrows = a.chunks[chunk].nrows;
     const uint64_t jf64  = static_cast<uint64_t>(tile) * kLanes + lane;
     const uint32_t n_f   = a.lv.n_cur[2];
-    const bool active    = wave < nrows && jf64 < n_f;
-    const uint32_t row   = row0 + wave;
+    const bool in_tile   = jf64 < n_f;
     const uint32_t jf    = static_cast<uint32_t>(jf64);
+    auto has_row = [&](uint32_t k) { return in_tile && wave + k * kWaves < nrows; };
 
-    // 0. this lane's trial
-    FFACoord cd{0, 0.0F, 0, 0.0F};
-    if (active) {
-        cd = resolve_pair_rc<NP>(a.lv, a.fg.dv, row, jf);
+    // 0. this lane's trials (one per row of the wave)
+    FFACoord cds[kRowsPerWave];
+    uint32_t lt = 0xFFFFFFFFU, ht = 0, lh = 0xFFFFFFFFU, hh = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kRowsPerWave; ++k) {
+        cds[k] = FFACoord{0, 0.0F, 0, 0.0F};
+        if (has_row(k)) {
+            cds[k] = resolve_pair_rc<NP>(a.lv, a.fg.dv, row0 + wave + k * kWaves, jf);
+            lt     = min(lt, cds[k].i_tail);
+            ht     = max(ht, cds[k].i_tail);
+            lh     = min(lh, cds[k].i_head);
+            hh     = max(hh, cds[k].i_head);
+        }
     }
     if (threadIdx.x < 4) {
         s_rng[threadIdx.x] = (threadIdx.x % 2 == 0) ? 0xFFFFFFFFU : 0U;
     }
     __syncthreads();
     {
-        const uint32_t lt = wave_min(active ? cd.i_tail : 0xFFFFFFFFU);
-        const uint32_t ht = wave_max(active ? cd.i_tail : 0U);
-        const uint32_t lh = wave_min(active ? cd.i_head : 0xFFFFFFFFU);
-        const uint32_t hh = wave_max(active ? cd.i_head : 0U);
+        lt = wave_min(lt);
+        ht = wave_max(ht);
+        lh = wave_min(lh);
+        hh = wave_max(hh);
         if (lane == 0 && lt != 0xFFFFFFFFU) {
             atomicMin(&s_rng[0], lt);
             atomicMax(&s_rng[1], ht);
@@ -210,19 +226,27 @@ k_final2(Final2Args a) {
     if (n_t > a.cap || n_h > a.cap) {
         // cannot happen: cap is the exact maximum over all blocks
         // (final2_ranges); never index past the staged rows
-        if (active) {
-            const uint64_t c_idx = static_cast<uint64_t>(row) * n_f + jf;
-            if (a.fo.count == nullptr) {
-                if constexpr (FINDMAX) {
-                    a.scores[c_idx] = __builtin_nanf("");
-                } else {
-                    for (uint32_t iw = 0; iw < a.wt.n; ++iw) {
-                        a.scores[c_idx * a.wt.n + iw] = __builtin_nanf("");
-                    }
+        for (uint32_t k = 0; k < kRowsPerWave; ++k) {
+            if (has_row(k) && a.fo.count == nullptr) {
+                const uint64_t c_idx =
+                    static_cast<uint64_t>(row0 + wave + k * kWaves) * n_f + jf;
+                const uint32_t nw = FINDMAX ? 1U : a.wt.n;
+                for (uint32_t iw = 0; iw < nw; ++iw) {
+                    a.scores[c_idx * nw + iw] = __builtin_nanf("");
                 }
             }
         }
         return;
+    }
+    // each trial's parent rows as staged slots and integer shifts, one word
+    // per row (slots < 256: the LDS budget bounds cap; shifts < NB)
+    uint32_t trial[kRowsPerWave];
+#pragma unroll
+    for (uint32_t k = 0; k < kRowsPerWave; ++k) {
+        const uint32_t St = NP == 1 ? 0U : shift_int(cds[k].shift_tail, NB);
+        const uint32_t Sh = shift_int(cds[k].shift_head, NB);
+        trial[k] = ((cds[k].i_tail - lo_t) & 0xFFU) | (St << 8) |
+                   (((cds[k].i_head - lo_h) & 0xFFU) << 16) | (Sh << 24);
     }
 
     // 1. stage the parent rows: task (row slot, 4-bin chunk)
@@ -326,107 +350,97 @@ k_final2(Final2Args a) {
         }
     }
     __syncthreads();
-    if (!active) {
-        return;
-    }
 
-    // 2. merge + normalise + prefix sum of this lane's trial.  `get(c, te,
-    //    tv, he, hv)` returns bins 4c..4c+3 of the rotated tail and head rows.
-    const uint32_t St = NP == 1 ? 0U : shift_int(cd.shift_tail, NB);
-    const uint32_t Sh = shift_int(cd.shift_head, NB);
-    float ps[NB];
-    auto prefix = [&](auto&& get) {
-        float acc = 0.0F;
+#pragma unroll 1
+    for (uint32_t k = 0; k < kRowsPerWave; ++k) {
+        if (!has_row(k)) {
+            continue;
+        }
+        const uint32_t row = row0 + wave + k * kWaves;
+        const uint32_t tw  = trial[k];
+        // 2. merge + normalise + prefix sum of this lane's trial: bins 4c..4c+3
+        //    of the rotated tail and head rows, one FP32 add each, then
+        //    p = e * (1/sqrt(v)) (score.cpp:139-141)
+        const uint32_t St = (tw >> 8) & 0xFFU;
+        const uint32_t Sh = tw >> 24;
+        const float2* trow = s_rows + (tw & 0xFFU) * STRIDE;
+        const float2* hrow = s_rows + (a.cap + ((tw >> 16) & 0xFFU)) * STRIDE;
+        float ps[NB];
+        {
+            float acc = 0.0F;
 #pragma unroll
-        for (uint32_t c = 0; c < C4; ++c) {
-            float4 te, tv, he, hv;
-            get(c, te, tv, he, hv);
-            float4 e, v;
-            e.x = te.x + he.x;
-            e.y = te.y + he.y;
-            e.z = te.z + he.z;
-            e.w = te.w + he.w;
-            v.x = tv.x + hv.x;
-            v.y = tv.y + hv.y;
-            v.z = tv.z + hv.z;
-            v.w = tv.w + hv.w;
-            if constexpr (kScaledHalf<TI>) {
-                e.x *= a.unscale;
-                e.y *= a.unscale;
-                e.z *= a.unscale;
-                e.w *= a.unscale;
-                v.x *= a.unscale;
-                v.y *= a.unscale;
-                v.z *= a.unscale;
-                v.w *= a.unscale;
+            for (uint32_t c = 0; c < C4; ++c) {
+                const float2* tp = trow + wrap<NB>(4 * c + NB - St);
+                const float2* hp = hrow + wrap<NB>(4 * c + NB - Sh);
+                const float2 t0 = tp[0], t1 = tp[1], t2 = tp[2], t3 = tp[3];
+                const float2 h0 = hp[0], h1 = hp[1], h2 = hp[2], h3 = hp[3];
+                float4 e, v;
+                e.x = t0.x + h0.x;
+                e.y = t1.x + h1.x;
+                e.z = t2.x + h2.x;
+                e.w = t3.x + h3.x;
+                v.x = t0.y + h0.y;
+                v.y = t1.y + h1.y;
+                v.z = t2.y + h2.y;
+                v.w = t3.y + h3.y;
+                if constexpr (kScaledHalf<TI>) {
+                    e.x *= a.unscale;
+                    e.y *= a.unscale;
+                    e.z *= a.unscale;
+                    e.w *= a.unscale;
+                    v.x *= a.unscale;
+                    v.y *= a.unscale;
+                    v.z *= a.unscale;
+                    v.w *= a.unscale;
+                }
+                const float4 p = normalise4v<EXACT>(e, v);
+                acc            = (c == 0) ? p.x : acc + p.x;
+                ps[4 * c]      = acc;
+                acc            = acc + p.y;
+                ps[4 * c + 1]  = acc;
+                acc            = acc + p.z;
+                ps[4 * c + 2]  = acc;
+                acc            = acc + p.w;
+                ps[4 * c + 3]  = acc;
+                // materialise this chunk's prefix values before the next chunk's
+                // reads: left pending (the compiler sinks the products and adds),
+                // every chunk's operands stay live and the kernel needs 256 VGPRs
+                __asm__ volatile("" : "+v"(ps[4 * c]), "+v"(ps[4 * c + 1]), "+v"(ps[4 * c + 2]),
+                                 "+v"(ps[4 * c + 3]), "+v"(acc));
             }
-            const float4 p = normalise4v<EXACT>(e, v);
-            acc            = (c == 0) ? p.x : acc + p.x;
-            ps[4 * c]      = acc;
-            acc            = acc + p.y;
-            ps[4 * c + 1]  = acc;
-            acc            = acc + p.z;
-            ps[4 * c + 2]  = acc;
-            acc            = acc + p.w;
-            ps[4 * c + 3]  = acc;
-            // materialise this chunk's prefix values before the next chunk's
-            // reads: left pending (the compiler sinks the products and adds),
-            // every chunk's operands stay live and the kernel needs 256 VGPRs
-            __asm__ volatile("" : "+v"(ps[4 * c]), "+v"(ps[4 * c + 1]), "+v"(ps[4 * c + 2]),
-                             "+v"(ps[4 * c + 3]), "+v"(acc));
         }
-    };
-    {
-        const float2* trow = s_rows + (cd.i_tail - lo_t) * STRIDE;
-        const float2* hrow = s_rows + (a.cap + (cd.i_head - lo_h)) * STRIDE;
-        prefix([&](uint32_t c, float4& te, float4& tv, float4& he, float4& hv) {
-            const float2* tp = trow + wrap<NB>(4 * c + NB - St);
-            const float2* hp = hrow + wrap<NB>(4 * c + NB - Sh);
-            const float2 t0 = tp[0], t1 = tp[1], t2 = tp[2], t3 = tp[3];
-            const float2 h0 = hp[0], h1 = hp[1], h2 = hp[2], h3 = hp[3];
-            te = make_float4(t0.x, t1.x, t2.x, t3.x);
-            tv = make_float4(t0.y, t1.y, t2.y, t3.y);
-            he = make_float4(h0.x, h1.x, h2.x, h3.x);
-            hv = make_float4(h0.y, h1.y, h2.y, h3.y);
-        });
-    }
 
-    // 3. boxcar S/N over the widths: circular wrap ext[k] = ps[k] + sum
-    //    (score.cpp:143-156), windows in registers
-    const float sum = ps[NB - 1];
-    float ext[NB - 1];
+        // 3. boxcar S/N over the widths (score.cpp:95-172), windows in registers
+        const float sum = ps[NB - 1];
+        const uint64_t c_idx = static_cast<uint64_t>(row) * n_f + jf;
+        float best           = -FLT_MAX;
+        for (uint32_t iw = 0; iw < a.wt.n; ++iw) {
+            // keep the windows inside the width loop: every prefix value is opaque
+            // per iteration, so no window difference of any width can be hoisted
+            // (hoisted, the windows of all NB - 1 widths would be formed for every
+            // trial and held in registers)
 #pragma unroll
-    for (int k = 0; k < NB - 1; ++k) {
-        ext[k] = ps[k] + sum;
-    }
-    const uint64_t c_idx = static_cast<uint64_t>(row) * n_f + jf;
-    float best           = -FLT_MAX;
-    for (uint32_t iw = 0; iw < a.wt.n; ++iw) {
-        // keep the windows inside the width loop: every prefix value is opaque
-        // per iteration, so no window difference of any width can be hoisted
-        // (hoisted, the windows of all NB - 1 widths would be formed for every
-        // trial and held in registers)
-#pragma unroll
-        for (int k = 0; k < NB; ++k) {
-            __asm__ volatile("" : "+v"(ps[k]));
+            for (int q = 0; q < NB; ++q) {
+                __asm__ volatile("" : "+v"(ps[q]));
+            }
+            const float dmax =
+                window_max_reg<NB>(std::make_integer_sequence<int, NB - 1>{}, ps, a.wt.w[iw]);
+            const float snr = ((a.wt.h[iw] + a.wt.b[iw]) * dmax) - (a.wt.b[iw] * sum);
+            if constexpr (FINDMAX) {
+                best = (best < snr) ? snr : best;
+            } else if (a.fo.count != nullptr) {
+                filter_emit(a.fo, snr > a.fo.threshold,
+                            static_cast<uint32_t>(c_idx * a.wt.n + iw), snr);
+            } else {
+                a.scores[c_idx * a.wt.n + iw] = snr;
+            }
         }
-        const float dmax =
-            window_max_reg<NB>(std::make_integer_sequence<int, NB - 1>{}, ps, ext, a.wt.w[iw]);
-        const float snr = ((a.wt.h[iw] + a.wt.b[iw]) * dmax) - (a.wt.b[iw] * sum);
         if constexpr (FINDMAX) {
-            best = (best < snr) ? snr : best;
-        } else if (a.fo.count != nullptr) {
-            filter_emit(a.fo, snr > a.fo.threshold, static_cast<uint32_t>(c_idx * a.wt.n + iw),
-                        snr);
-        } else {
-            a.scores[c_idx * a.wt.n + iw] = snr;
-        }
-    }
-    if constexpr (FINDMAX) {
-        if (a.fo.count != nullptr) {
-            filter_emit(a.fo, best > a.fo.threshold, static_cast<uint32_t>(c_idx), best);
-        } else {
-            a.scores[c_idx] = best;
+            if (a.fo.count != nullptr) {
+                filter_emit(a.fo, best > a.fo.threshold, static_cast<uint32_t>(c_idx), best);
+            } else {
+                a.scores[c_idx] = best;
+            }
         }
     }
 }
@@ -463,7 +477,7 @@ k_final2_ranges(Final2Args a, uint32_t nblocks, uint32_t* out) {
 template <int NB, int NP, typename TI>
 void launch3(const Final2Args& a, uint32_t nblocks, bool find_max, bool exact, hipStream_t s) {
     const size_t lds     = static_cast<size_t>(2) * a.cap * kStride<NB> * sizeof(float2);
-    const uint32_t block = a.rows_per_block * kLanes;
+    const uint32_t block = kBlock;
     if (find_max && exact) {
         k_final2<NB, NP, true, true, TI><<<nblocks, block, lds, s>>>(a);
     } else if (find_max) {
@@ -522,8 +536,9 @@ void final2_merge_score(const Final2Args& a, FoldStorage si, uint32_t nblocks, b
                         bool exact, hipStream_t s) {
     LOKI_REQUIRE(final2_supported(a.lv.nbins), "final-level kernel: unsupported nbins");
     LOKI_REQUIRE(a.wt.wmax < a.lv.nbins, "boxcar widths must be < nbins");
-    LOKI_REQUIRE(a.rows_per_block == 1 || a.rows_per_block == 2 || a.rows_per_block == 4,
-                 "final-level kernel: rows per block must be 1, 2 or 4");
+    LOKI_REQUIRE(a.rows_per_block >= 1 && a.rows_per_block <= kMaxRows,
+                 "final-level kernel: at most 16 grid rows per block");
+    LOKI_REQUIRE(a.cap <= 256, "final-level kernel: parent ranges above 256 rows");
     LOKI_REQUIRE(a.n_chunks > 0 && a.chunks != nullptr && a.cap > 0,
                  "final-level kernel: no row chunks");
     if (a.grand) {

@@ -98,6 +98,168 @@ k_merge_scalar(const TI* __restrict__ in, TO* __restrict__ out,
     }
 }
 
+// Two merge levels in one pass (B = A merged, C = B merged): output row c of
+// segment s at level C needs B's tail row T = rec_C(c).tail of segment 2s and
+// head row H of segment 2s + 1, and each of those is the merge of two rows of
+// A (segments 4s..4s+3) through B's shift table.  Per output bin j:
+//   C[j] = round_C( round_B(A0[j - S_T - s_0] + A1[j - S_T - s_1])
+//                 + round_B(A2[j - S_H - s_2] + A3[j - S_H - s_3]) )
+// -- the adds and roundings of two single-level merges, in their order, so the
+// bits are those of level C merged through B (kernels.cpp:71-146 twice), but
+// level B never reaches HBM.  round_B / round_C are the stored type's
+// rounding (identity for FP32; FP16/BF16 at scale 0.5 per level; block
+// floating point with row exponents from the max over the C4 lanes of each
+// component row, which together hold all NB bins of the B row).
+template <typename TB>
+__device__ __forceinline__ float4 round_level(float4 x, float scale) {
+    if constexpr (kScaledHalf<TB>) {
+        x.x = widen1(narrow1<TB>(x.x * scale));
+        x.y = widen1(narrow1<TB>(x.y * scale));
+        x.z = widen1(narrow1<TB>(x.z * scale));
+        x.w = widen1(narrow1<TB>(x.w * scale));
+    }
+    return x;
+}
+
+template <int C4>
+__device__ __forceinline__ float4 round_bfp_row(float4 x) {
+    const float mx = group_absmax<C4>(
+        __builtin_fmaxf(__builtin_fmaxf(__builtin_fabsf(x.x), __builtin_fabsf(x.y)),
+                        __builtin_fmaxf(__builtin_fabsf(x.z), __builtin_fabsf(x.w))));
+    const int e     = bfp_exponent(mx);
+    const float inv = pow2f(-e), sc = pow2f(e);
+    x.x = round_bfp(x.x, inv, sc);
+    x.y = round_bfp(x.y, inv, sc);
+    x.z = round_bfp(x.z, inv, sc);
+    x.w = round_bfp(x.w, inv, sc);
+    return x;
+}
+
+template <int NB, bool POLY, int ITERS, typename TI, typename TO>
+__global__ void __launch_bounds__(kBlock)
+k_merge2(const TI* __restrict__ in, TO* __restrict__ out, const FFACoord* __restrict__ cb,
+         const FFACoordFreq* __restrict__ cbf, const FFACoord* __restrict__ cc,
+         const FFACoordFreq* __restrict__ ccf, uint64_t n_c, uint64_t n_b, uint64_t n_a,
+         TileSchedule ts, float scale, const uint16_t* __restrict__ ein,
+         uint8_t* __restrict__ eout) {
+    using CI = chunk4_t<TI>;
+    using CO = chunk4_t<TO>;
+    constexpr uint32_t C4    = NB / 4;
+    constexpr uint32_t LPR   = 2 * C4;
+    static_assert(LPR <= static_cast<uint32_t>(kBlock) && C4 <= 64, "one lane per output chunk");
+    constexpr uint32_t LANES = LPR;
+    constexpr uint32_t ROWS  = kBlock / LANES;
+    constexpr uint32_t TILE  = ROWS * ITERS;
+    const Tile tl = tile_of(ts, xcd_remap(blockIdx.x, ts.nblocks), TILE);
+    if (tl.count == 0) {
+        return;
+    }
+    const uint64_t seg  = tl.seg;
+    const uint32_t rsub = threadIdx.x / LANES;
+    const uint32_t ch   = threadIdx.x % LANES;
+    const uint32_t q    = ch / C4;
+    const uint32_t i4   = ch % C4;
+    float4 r0[ITERS], r1[ITERS], r2[ITERS], r3[ITERS];
+#pragma unroll
+    for (int it = 0; it < ITERS; ++it) {
+        const uint32_t local = static_cast<uint32_t>(it) * ROWS + rsub;
+        if (local < tl.count) {
+            const uint64_t c = tl.c0 + local;
+            uint32_t bt, bh, st, sh;  // B rows and their shifts
+            uint32_t a[4], sa[4];     // A rows and their shifts within B
+            if constexpr (POLY) {
+                const FFACoord rc = cc[c];
+                bt = rc.i_tail;
+                bh = rc.i_head;
+                st = shift_int(rc.shift_tail, NB);
+                sh = shift_int(rc.shift_head, NB);
+                const FFACoord rt = cb[bt];
+                const FFACoord rh = cb[bh];
+                a[0]  = rt.i_tail;
+                sa[0] = shift_int(rt.shift_tail, NB);
+                a[1]  = rt.i_head;
+                sa[1] = shift_int(rt.shift_head, NB);
+                a[2]  = rh.i_tail;
+                sa[2] = shift_int(rh.shift_tail, NB);
+                a[3]  = rh.i_head;
+                sa[3] = shift_int(rh.shift_head, NB);
+            } else {
+                const FFACoordFreq rc = ccf[c];
+                bt = rc.idx;
+                bh = rc.idx;
+                st = 0;
+                sh = shift_int(rc.shift, NB);
+                const FFACoordFreq rb = cbf[bt];
+                a[0] = a[1] = a[2] = a[3] = rb.idx;
+                sa[0] = sa[2] = 0;
+                sa[1] = sa[3] = shift_int(rb.shift, NB);
+            }
+            float4 v[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint64_t row = (4 * seg + static_cast<uint64_t>(k)) * n_a + a[k];
+                const CI* __restrict__ src = reinterpret_cast<const CI*>(in + row * (2 * NB));
+                float sc = 1.0F;
+                if constexpr (kBlockFP<TI>) {
+                    sc = bfp_row_scale(ein, row, q);
+                }
+                const uint32_t rot = wrap_nb<NB>((k < 2 ? st : sh) + sa[k]);
+                v[k] = rotated4_st<NB, TI>(src + q * C4, i4, rot, sc);
+            }
+            r0[it] = v[0];
+            r1[it] = v[1];
+            r2[it] = v[2];
+            r3[it] = v[3];
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < ITERS; ++it) {
+        const uint32_t local = static_cast<uint32_t>(it) * ROWS + rsub;
+        if (local < tl.count) {
+            const uint64_t c = tl.c0 + local;
+            float4 bt, bh, o;
+            bt.x = r0[it].x + r1[it].x;
+            bt.y = r0[it].y + r1[it].y;
+            bt.z = r0[it].z + r1[it].z;
+            bt.w = r0[it].w + r1[it].w;
+            bh.x = r2[it].x + r3[it].x;
+            bh.y = r2[it].y + r3[it].y;
+            bh.z = r2[it].z + r3[it].z;
+            bh.w = r2[it].w + r3[it].w;
+            if constexpr (kBlockFP<TO>) {
+                bt = round_bfp_row<C4>(bt);
+                bh = round_bfp_row<C4>(bh);
+            } else {
+                bt = round_level<TO>(bt, scale);
+                bh = round_level<TO>(bh, scale);
+            }
+            o.x = bt.x + bh.x;
+            o.y = bt.y + bh.y;
+            o.z = bt.z + bh.z;
+            o.w = bt.w + bh.w;
+            CO* __restrict__ orow = reinterpret_cast<CO*>(out + (seg * n_c + c) * (2 * NB));
+            if constexpr (kBlockFP<TO>) {
+                const float mx = group_absmax<C4>(__builtin_fmaxf(
+                    __builtin_fmaxf(__builtin_fabsf(o.x), __builtin_fabsf(o.y)),
+                    __builtin_fmaxf(__builtin_fabsf(o.z), __builtin_fabsf(o.w))));
+                const int e = bfp_exponent(mx);
+                store_stream(&orow[ch], narrow4_bfp(o, pow2f(-e)));
+                if (i4 == 0) {
+                    eout[(seg * n_c + c) * 2 + q] = static_cast<uint8_t>(static_cast<int8_t>(e));
+                }
+            } else {
+                if constexpr (kScaledHalf<TO>) {
+                    o.x *= scale;
+                    o.y *= scale;
+                    o.z *= scale;
+                    o.w *= scale;
+                }
+                store_stream(&orow[ch], narrow4<TO>(o));
+            }
+        }
+    }
+}
+
 // Grouped merges of many series (one task each, sorted by block0).
 template <int NB, bool POLY, int ITERS, typename T>
 __global__ void __launch_bounds__(kBlock)
@@ -249,6 +411,102 @@ void merge_time(const void* in, void* out, const FFACoord* coords,
     }
     return so == FoldStorage::kBF16 ? std::apply(merge_typed<__bf16, __bf16>, args)
                                     : std::apply(merge_typed<__bf16, float>, args);
+}
+
+namespace {
+
+template <int NB, bool POLY, typename TI, typename TO>
+void launch_merge2(const TI* in, TO* out, const FFACoord* cb, const FFACoordFreq* cbf,
+                   const FFACoord* cc, const FFACoordFreq* ccf, uint64_t n_c, uint64_t n_b,
+                   uint64_t n_a, uint64_t nseg_c, uint64_t n_f, float scale, hipStream_t s) {
+    constexpr int ITERS = 2;
+    using G             = V4Geom<NB>;
+    constexpr uint32_t TILE = G::ROWS * ITERS;
+    const TileSchedule ts = make_schedule(n_c, n_f, nseg_c, TILE, kRowGroup, kRun);
+    const uint16_t* ein = nullptr;
+    uint8_t* eout       = nullptr;
+    if constexpr (kBlockFP<TI>) {
+        ein = reinterpret_cast<const uint16_t*>(in + (4 * nseg_c) * n_a * (2 * NB));
+    }
+    if constexpr (kBlockFP<TO>) {
+        eout = reinterpret_cast<uint8_t*>(out + nseg_c * n_c * (2 * NB));
+    }
+    k_merge2<NB, POLY, ITERS, TI, TO><<<ts.nblocks, kBlock, 0, s>>>(
+        in, out, cb, cbf, cc, ccf, n_c, n_b, n_a, ts, scale, ein, eout);
+}
+
+template <bool POLY, typename TI, typename TO>
+void merge2_nb(const void* in, void* out, const FFACoord* cb, const FFACoordFreq* cbf,
+               const FFACoord* cc, const FFACoordFreq* ccf, uint64_t n_c, uint64_t n_b,
+               uint64_t n_a, uint64_t nseg_c, uint64_t n_f, uint32_t nbins, float scale,
+               hipStream_t s) {
+    const auto* pi = static_cast<const TI*>(in);
+    auto* po       = static_cast<TO*>(out);
+    switch (nbins) {
+    case 8:
+        return launch_merge2<8, POLY>(pi, po, cb, cbf, cc, ccf, n_c, n_b, n_a, nseg_c, n_f,
+                                      scale, s);
+    case 16:
+        return launch_merge2<16, POLY>(pi, po, cb, cbf, cc, ccf, n_c, n_b, n_a, nseg_c, n_f,
+                                       scale, s);
+    case 32:
+        return launch_merge2<32, POLY>(pi, po, cb, cbf, cc, ccf, n_c, n_b, n_a, nseg_c, n_f,
+                                       scale, s);
+    case 64:
+        return launch_merge2<64, POLY>(pi, po, cb, cbf, cc, ccf, n_c, n_b, n_a, nseg_c, n_f,
+                                       scale, s);
+    case 128:
+        return launch_merge2<128, POLY>(pi, po, cb, cbf, cc, ccf, n_c, n_b, n_a, nseg_c, n_f,
+                                        scale, s);
+    default: LOKI_REQUIRE(false, "two-level merge: nbins must be 8, 16, 32, 64 or 128");
+    }
+}
+
+template <typename TI, typename TO>
+void merge2_typed(const void* in, void* out, const FFACoord* cb, const FFACoordFreq* cbf,
+                  const FFACoord* cc, const FFACoordFreq* ccf, uint64_t n_c, uint64_t n_b,
+                  uint64_t n_a, uint64_t nseg_c, uint64_t n_f, uint32_t nbins, float scale,
+                  hipStream_t s) {
+    if (cc != nullptr) {
+        merge2_nb<true, TI, TO>(in, out, cb, nullptr, cc, nullptr, n_c, n_b, n_a, nseg_c, n_f,
+                                nbins, scale, s);
+    } else {
+        merge2_nb<false, TI, TO>(in, out, nullptr, cbf, nullptr, ccf, n_c, n_b, n_a, nseg_c,
+                                 n_f, nbins, scale, s);
+    }
+}
+
+}  // namespace
+
+bool merge2_supported(uint32_t nbins) {
+    return nbins == 8 || nbins == 16 || nbins == 32 || nbins == 64 || nbins == 128;
+}
+
+void merge_time2(const void* in, void* out, const FFACoord* coords_b,
+                 const FFACoordFreq* coords_freq_b, const FFACoord* coords_c,
+                 const FFACoordFreq* coords_freq_c, uint64_t n_c, uint64_t n_b, uint64_t n_a,
+                 uint64_t nseg_c, uint64_t n_f, uint32_t nbins, FoldStorage si, FoldStorage so,
+                 float scale, hipStream_t s) {
+    LOKI_REQUIRE(merge2_supported(nbins), "two-level merge: unsupported nbins");
+    LOKI_REQUIRE((coords_c == nullptr) == (coords_b == nullptr) &&
+                     (coords_c == nullptr) != (coords_freq_c == nullptr) &&
+                     (coords_freq_c == nullptr) == (coords_freq_b == nullptr),
+                 "two-level merge: one kind of coordinate table for both levels");
+    LOKI_REQUIRE(si == so || (si == FoldStorage::kF32 && so == FoldStorage::kBFP16),
+                 "two-level merge: unsupported storage pair");
+    if (n_c == 0 || nseg_c == 0) {
+        return;
+    }
+    const auto args = std::make_tuple(in, out, coords_b, coords_freq_b, coords_c, coords_freq_c,
+                                      n_c, n_b, n_a, nseg_c, n_f, nbins, scale, s);
+    switch (so) {
+    case FoldStorage::kF32: return std::apply(merge2_typed<float, float>, args);
+    case FoldStorage::kF16: return std::apply(merge2_typed<_Float16, _Float16>, args);
+    case FoldStorage::kBF16: return std::apply(merge2_typed<__bf16, __bf16>, args);
+    case FoldStorage::kBFP16:
+        return si == FoldStorage::kF32 ? std::apply(merge2_typed<float, int16_t>, args)
+                                       : std::apply(merge2_typed<int16_t, int16_t>, args);
+    }
 }
 
 uint32_t merge_tile(uint32_t nbins) {

@@ -221,7 +221,7 @@ struct Final2Args {
     FilterOut fo;
     const RowChunk* chunks;  ///< device
     uint32_t n_chunks;
-    uint32_t rows_per_block;  ///< 1, 2 or 4 waves per block
+    uint32_t rows_per_block;  ///< most rows in a chunk (<= 16; 4 waves, 4 rows each)
     uint32_t cap;             ///< staged parent rows per side
     uint32_t grand;
     float unscale;            ///< reduced precision: final sums x 2^(k0 + F - 1)
@@ -304,6 +304,20 @@ void merge_grouped(const MergeTask* tasks, uint32_t ntasks, uint32_t nblocks, ui
                    bool poly, FoldStorage st, hipStream_t s);
 /// Tile geometry of merge_grouped for a given nbins: coordinates per block.
 uint32_t merge_tile(uint32_t nbins);
+/// nbins handled by merge_time2.
+bool merge2_supported(uint32_t nbins);
+/// Two time-domain merge levels in one pass: level C (n_c coordinates per
+/// segment, nseg_c segments) from level A (n_a coordinates, 4 * nseg_c
+/// segments) through level B (n_b coordinates) -- the bits of two merge_time
+/// calls, without storing B.  coords_b/_c: B's and C's shift tables (poly, or
+/// the freq-only tables); n_f: frequencies per row of C.  si: A's storage,
+/// so: B's and C's (equal, or FP32 -> block floating point); scale: 0.5 per
+/// level for FP16/BF16.
+void merge_time2(const void* in, void* out, const FFACoord* coords_b,
+                 const FFACoordFreq* coords_freq_b, const FFACoord* coords_c,
+                 const FFACoordFreq* coords_freq_c, uint64_t n_c, uint64_t n_b, uint64_t n_a,
+                 uint64_t nseg_c, uint64_t n_f, uint32_t nbins, FoldStorage si, FoldStorage so,
+                 float scale, hipStream_t s);
 /// Grouped fused last levels (freq-only) of one nbins and parent storage.
 void final_grouped(const FinalTask* tasks, uint32_t ntasks, uint32_t nblocks, uint32_t nbins,
                    FoldStorage si, bool find_max, bool exact, uint32_t lds_stride,

@@ -119,6 +119,12 @@ __device__ __forceinline__ float4 funnel4(const float4 a, const float4 b, uint32
     return o;
 }
 
+/// x mod NB for x < 2 * NB (NB a power of two).
+template <int NB>
+__device__ __forceinline__ uint32_t wrap_nb(uint32_t x) {
+    return x & static_cast<uint32_t>(NB - 1);
+}
+
 /// kernels.cpp:78-85: integer shift u32(shift + 0.5f), nbins -> 0.
 __device__ __forceinline__ uint32_t shift_int(float s, uint32_t nbins) {
     const uint32_t k = static_cast<uint32_t>(s + 0.5F);

@@ -337,6 +337,9 @@ public:
     void set_timing(bool on) { check(loki_ffa_set_timing(m_h.get(), on ? 1 : 0)); }
     void set_fused(bool on) { check(loki_ffa_set_fused(m_h.get(), on ? 1 : 0)); }
     void set_fuse_depth(int depth) { check(loki_ffa_set_fuse_depth(m_h.get(), depth)); }
+    void set_two_level_merges(bool on) {
+        check(loki_ffa_set_two_level_merges(m_h.get(), on ? 1 : 0));
+    }
     void set_exact_norm(bool on) { check(loki_ffa_set_exact_norm(m_h.get(), on ? 1 : 0)); }
     void set_dft_precision(const std::string& p) {
         if (p != "f32" && p != "bf16x3") {
@@ -1045,6 +1048,7 @@ PYBIND11_MODULE(_loki_amd, m) {
         .def("set_timing", &FFA::set_timing, py::arg("enable"))
         .def("set_fused", &FFA::set_fused, py::arg("enable"))
         .def("set_fuse_depth", &FFA::set_fuse_depth, py::arg("depth"))
+        .def("set_two_level_merges", &FFA::set_two_level_merges, py::arg("enable"))
         .def("set_exact_norm", &FFA::set_exact_norm, py::arg("exact"))
         .def("set_fold_storage", &FFA::set_fold_storage, py::arg("storage"))
         .def("scores_filter", &FFA::scores_filter_host, py::arg("ts_e"), py::arg("ts_v"),

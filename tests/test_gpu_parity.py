@@ -376,3 +376,31 @@ def test_storage_snr_error_vs_fp64(oracle, name):
               f"median {np.median(d):.2e}; top-100 candidates max {d[top].max():.2e} "
               f"(S/N up to {snr64.max():.1f})")
         assert d.max() <= bound
+
+
+# ---------------------------------------------------------------------------
+# Two-level merge passes: the same bits as one pass per level
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16", "bfp16"])
+@pytest.mark.parametrize("name", ["freq16", "acc17w", "jerk17w", "acc20", "acc16_nb8",
+                                  "jerk18_nb16", "acc17_nb128", "acc_eta2_bseg"])
+def test_two_level_merges_equal_single_levels(oracle, name, storage):
+    """Folds (every level returned through execute) and fused scores with two-level merge
+    passes equal one pass per level, and the oracle."""
+    from loki_amd import libculoki
+
+    cfg, ocfg = _cfgs(name)
+    ts_e, ts_v = _series(cfg.nsamps, seed=71)
+    widths = np.array([w for w in (1, 2, 3, 4, 6, 8, 12, 16, 24) if w < cfg.nbins], np.uint64)
+    out = {}
+    for two in (True, False):
+        ffa = libculoki.ffa.FFACUDA(cfg, device_id=0)
+        ffa.set_fold_storage(storage)
+        ffa.set_two_level_merges(two)
+        fold = np.zeros(ffa.plan.buffer_size, np.float32)
+        ffa.execute(ts_e, ts_v, fold)
+        out[two] = (fold[: ffa.plan.fold_size].copy(), ffa.scores(ts_e, ts_v, widths))
+    np.testing.assert_array_equal(out[True][0], out[False][0])
+    np.testing.assert_array_equal(out[True][1], out[False][1])
+    want, _ = oracle.compute_ffa_scores(ts_e, ts_v, ocfg, widths=widths, storage=storage)
+    np.testing.assert_array_equal(out[True][1], want)
