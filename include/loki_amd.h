@@ -281,10 +281,16 @@ int loki_ffa_scores_filter_host(loki_ffa_t ffa, const float* ts_e, const float* 
  * to loki_ffa_scores_* on its own config.  Inputs are copied into the batch's own device slots
  * (loki_batch_series) before loki_batch_execute. */
 typedef struct {
-    uint32_t merge_group; /* child rows of a merge scheduled back to back */
-    uint32_t merge_run;   /* frequency tiles per sequential run */
-    uint32_t final_group; /* the same for the fused last levels */
+    uint32_t merge_group;    /* child rows of a merge scheduled back to back */
+    uint32_t merge_run;      /* frequency tiles per sequential run */
+    uint32_t final_group;    /* the same for the fused last levels */
     uint32_t final_run;
+    uint32_t persistent;     /* must be 0: the one-launch persistent mode is disabled */
+    uint32_t score_lanes[5]; /* boxcar-width schedule of the last levels, nbins 8, 16, 32, 64,
+                                128: 0 = one lane per trial (64-lane workgroups); 1, 2 or 4 =
+                                that many lanes per trial in 256-lane workgroups, each taking
+                                every score_lanes-th width (nbins >= 4 * lanes; above 32 bins
+                                nbins <= 32 * lanes) */
 } loki_batch_schedule;
 /* widths: boxcar widths for every series (those >= nbins dropped); NULL/0: each config's own
  * scoring widths. */
@@ -311,9 +317,13 @@ int loki_batch_set_input_device(loki_batch_t batch, size_t i, const float* ts_e,
 int loki_batch_execute(loki_batch_t batch, void* stream);
 int loki_batch_set_graph(loki_batch_t batch, int enable);
 int loki_batch_set_schedule(loki_batch_t batch, const loki_batch_schedule* schedule);
-/* Times candidate tile schedules on the batch itself (synchronous) and keeps the fastest. */
+/* Times candidate schedules (tile orders, then lanes per trial of the last levels per nbins)
+ * on the batch itself (synchronous) and keeps the fastest. */
 int loki_batch_autotune(loki_batch_t batch, void* stream, int repeats,
                         loki_batch_schedule* chosen);
+/* Waits for the device; *flags = nonzero if a launch since the last call flagged an error
+ * (its scores are then invalid).  Clears the flag. */
+int loki_batch_errors(loki_batch_t batch, uint32_t* flags);
 
 /* --- Pruning seeds from final-level FFA folds ---------------------------- */
 /* Rows are [e: W][v: W]: W = nbins floats (time) or nbins_f = nbins/2+1 complex64 (Fourier,

@@ -25,7 +25,7 @@ ARCH = os.environ.get("LOKI_AMD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_SOURCES = ["k_resolve.hip", "k_brutefold.hip", "k_merge.hip", "k_score.hip", "k_final.hip", "k_final2.hip",
-               "k_fourier.hip", "k_fourier_score.hip", "k_refine.hip", "k_seed.hip",
+               "k_fourier.hip", "k_fourier_score.hip", "k_refine.hip", "k_seed.hip", "k_tree.hip",
                "ffa.hip", "batch.hip", "sweep.hip", "capi.hip"]
 HOST_SOURCES = ["config.cpp", "plan.cpp", "regions.cpp"]
 

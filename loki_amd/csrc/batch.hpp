@@ -1,8 +1,10 @@
 // Batched FFA over many independent time series of uneven length and
 // frequency range: every stage of every series is advanced by grouped launches
-// (one per stage and nbins class) and the whole sequence is captured once in
-// a HIP graph.  The reference runs series one FFACUDA object at a time with
-// host synchronisations per level (ffa_freq_sweep_cuda.cu:122-136,
+// (one per stage and nbins class), captured once in a HIP graph.  A
+// one-launch persistent mode (k_tree.hip: tickets in dependency order,
+// per-item completion counters) is built but disabled: its launches stalled
+// on MI355X.  The reference runs series one FFACUDA object at a time
+// with host synchronisations per level (ffa_freq_sweep_cuda.cu:122-136,
 // kernels_cuda.cu:1081-1082); it has no multi-series batching.
 #pragma once
 
@@ -18,13 +20,20 @@
 
 namespace loki_amd {
 
-/// Tile-schedule knobs of the grouped merges and last levels (rows of child
-/// coordinates scheduled back to back, frequency tiles per run).
+/// Schedule knobs: tile order of the merges and last levels (rows of child
+/// coordinates scheduled back to back, frequency tiles per run), the launch
+/// mode, and the boxcar-width schedule of the last levels per nbins (index
+/// log2(nbins) - 3): 0 is one lane per trial in 64-lane workgroups, 1, 2 or 4
+/// that many lanes per trial in 256-lane workgroups, each lane taking every
+/// lanes-th width.  persistent = 1 (one launch for every stage) is disabled:
+/// set_schedule rejects it.
 struct BatchSchedule {
     uint32_t merge_group{16};
     uint32_t merge_run{16};
     uint32_t final_group{16};
     uint32_t final_run{16};
+    uint32_t persistent{0};
+    uint32_t score_lanes[5]{0, 0, 0, 0, 0};
 };
 
 class FFABatch {
@@ -61,8 +70,14 @@ public:
     BatchSchedule autotune(hipStream_t s, int repeats);
     const BatchSchedule& schedule() const noexcept { return m_sched; }
     void set_schedule(const BatchSchedule& sc);
-    /// Grouped launches per execute().
-    SizeType launches() const noexcept { return m_launches.size(); }
+    /// Kernel launches per execute().
+    SizeType launches() const noexcept { return m_sched.persistent != 0 ? 1 : m_launches.size(); }
+    /// Persistent launch: polls a dependency wait may take before it gives up
+    /// and flags an error.
+    void set_spin_limit(uint32_t polls);
+    /// Waits for the device; nonzero if a launch since the last call flagged
+    /// an error (its results are then invalid).  Clears the flag.
+    uint32_t errors();
 
 private:
     struct Series {
@@ -82,18 +97,27 @@ private:
         uint32_t ntasks, nblocks;
         SizeType task_off;  ///< bytes into the task arena
         uint32_t stride;    ///< final: LDS row stride; fold: the largest nbins
+        uint32_t lanes;     ///< final: lanes per trial (0: 64-lane workgroups)
+        SizeType lds;       ///< final with lanes: dynamic LDS bytes
     };
 
     int m_device;
     std::vector<Series> m_series;
     DeviceBuffer m_in, m_fold, m_coords, m_scores, m_freqs, m_tasks;
     std::vector<Launch> m_launches;
+    // persistent launch: item table and task tables in m_tasks, counters in m_sync
+    kern::TreeArgs m_tree{};
+    uint32_t m_tree_blocks{0};
+    SizeType m_tree_lds{0};
+    uint32_t m_spin_limit{1U << 21};
+    DeviceBuffer m_sync;
     BatchSchedule m_sched;
     bool m_use_graph{true};
     hipGraph_t m_graph{nullptr};
     hipGraphExec_t m_exec{nullptr};
 
     void build_tasks();
+    void build_tree(std::vector<uint8_t>& blob, std::vector<SizeType>& offs);
     void enqueue(hipStream_t s) const;
     void drop_graph();
 };

@@ -763,8 +763,23 @@ int loki_batch_set_graph(loki_batch_t batch, int enable) {
 int loki_batch_set_schedule(loki_batch_t batch, const loki_batch_schedule* sc) {
     return guarded([&] {
         REQ(batch != nullptr && sc != nullptr, "null argument");
-        batch->batch->set_schedule({sc->merge_group, sc->merge_run, sc->final_group,
-                                    sc->final_run});
+        loki_amd::BatchSchedule b;
+        b.merge_group = sc->merge_group;
+        b.merge_run   = sc->merge_run;
+        b.final_group = sc->final_group;
+        b.final_run   = sc->final_run;
+        b.persistent  = sc->persistent;
+        for (int k = 0; k < 5; ++k) {
+            b.score_lanes[k] = sc->score_lanes[k];
+        }
+        batch->batch->set_schedule(b);
+    });
+}
+
+int loki_batch_errors(loki_batch_t batch, uint32_t* flags) {
+    return guarded([&] {
+        REQ(batch != nullptr && flags != nullptr, "null argument");
+        *flags = batch->batch->errors();
     });
 }
 
@@ -774,7 +789,14 @@ int loki_batch_autotune(loki_batch_t batch, void* stream, int repeats,
         REQ(batch != nullptr, "null batch");
         const auto sc = batch->batch->autotune(as_stream(stream), repeats);
         if (chosen != nullptr) {
-            *chosen = {sc.merge_group, sc.merge_run, sc.final_group, sc.final_run};
+            chosen->merge_group = sc.merge_group;
+            chosen->merge_run   = sc.merge_run;
+            chosen->final_group = sc.final_group;
+            chosen->final_run   = sc.final_run;
+            chosen->persistent  = sc.persistent;
+            for (int k = 0; k < 5; ++k) {
+                chosen->score_lanes[k] = sc.score_lanes[k];
+            }
         }
     });
 }

@@ -23,10 +23,12 @@
 // widened to FP32 and the merged sums are unscaled exactly before step 2's
 // normalisation; block floating-point parents are widened with their rows'
 // exponents and need no unscaling.
+#include <algorithm>
 #include <cfloat>
 #include <type_traits>
 #include <utility>
 
+#include "final_device.hpp"
 #include "merge_device.hpp"
 #include "resolve_device.hpp"
 #include "score_device.hpp"
@@ -42,90 +44,6 @@ constexpr uint32_t kRows     = 64;
 // rows: 10.69 ms at 16 x 16, 10.2 ms at 77 x 4).
 constexpr uint32_t kRowGroup = 128;
 constexpr uint32_t kRun      = 4;
-
-template <bool EXACT>
-__device__ __forceinline__ float4 normalise4(const float4 e, const float4 v) {
-    if constexpr (EXACT) {
-        return normalise4_exact(e, v);
-    } else {
-        float4 p;
-        p.x = normalise<false>(e.x, v.x);
-        p.y = normalise<false>(e.y, v.y);
-        p.z = normalise<false>(e.z, v.z);
-        p.w = normalise<false>(e.w, v.w);
-        return p;
-    }
-}
-
-/// e = tail + head, v = tail + head (one FP32 add each), unscaled for
-/// reduced-precision parents, normalised and stored as 4 profile bins at
-/// row[(b0 + k) mod NB], k < 4 (b0 = 4*i4 for aligned output, or rotated by
-/// the tail shift when the tail was loaded unrotated).
-template <int NB, bool EXACT, typename TI>
-__device__ __forceinline__ void merge_store(const float4 te, const float4 tv, const float4 he,
-                                            const float4 hv, float unscale,
-                                            float* __restrict__ row, uint32_t b0) {
-    float4 e, v;
-    e.x = te.x + he.x;
-    e.y = te.y + he.y;
-    e.z = te.z + he.z;
-    e.w = te.w + he.w;
-    v.x = tv.x + hv.x;
-    v.y = tv.y + hv.y;
-    v.z = tv.z + hv.z;
-    v.w = tv.w + hv.w;
-    if constexpr (kScaledHalf<TI>) {
-        e.x *= unscale;
-        e.y *= unscale;
-        e.z *= unscale;
-        e.w *= unscale;
-        v.x *= unscale;
-        v.y *= unscale;
-        v.z *= unscale;
-        v.w *= unscale;
-    }
-    const float4 p = normalise4<EXACT>(e, v);
-    row[(b0 + 0) & (NB - 1)] = p.x;
-    row[(b0 + 1) & (NB - 1)] = p.y;
-    row[(b0 + 2) & (NB - 1)] = p.z;
-    row[(b0 + 3) & (NB - 1)] = p.w;
-}
-
-/// Widest rows whose boxcar windows are formed in registers (compile-time
-/// window offsets, no prefix-sum row in LDS); wider rows use the LDS row.
-constexpr int kRegWindowMaxNB = 32;
-
-/// max_j (hi_j - ps[j]) for the fixed width W, hi_j = ps[j + W] or, past the
-/// end of the row, ext[j + W - NB] = ps[j + W - NB] + sum: the same operands
-/// as the LDS prefix-sum row of window_max.
-template <int NB, int W>
-__device__ __forceinline__ float window_max_fixed(const float (&ps)[NB],
-                                                  const float (&ext)[NB - 1]) {
-    float ma = -FLT_MAX, mb = -FLT_MAX;
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-        const float hi = (j + W < NB) ? ps[(j + W) % NB] : ext[(j + W - NB) % (NB - 1)];
-        const float d  = hi - ps[j];
-        if ((j & 2) == 0) {
-            ma = __builtin_fmaxf(ma, d);
-        } else {
-            mb = __builtin_fmaxf(mb, d);
-        }
-    }
-    return __builtin_fmaxf(ma, mb);
-}
-
-/// window_max_fixed<NB, w> for the runtime, wave-uniform width w in [1, NB).
-template <int NB, int... Ws>
-__device__ __forceinline__ float window_max_reg(std::integer_sequence<int, Ws...>,
-                                                const float (&ps)[NB],
-                                                const float (&ext)[NB - 1], uint32_t w) {
-    float r = 0.0F;
-    (void)((w == static_cast<uint32_t>(Ws + 1) ? (r = window_max_fixed<NB, Ws + 1>(ps, ext), true)
-                                               : false) ||
-           ...);
-    return r;
-}
 
 template <int NB, int NP, bool FINDMAX, bool EXACT, typename TI>
 __device__ __forceinline__ void final_block(uint32_t bid, const TI* __restrict__ in,
@@ -455,6 +373,86 @@ void grouped_launch(const FinalTask* tasks, uint32_t ntasks, uint32_t nblocks, b
 }
 
 }  // namespace
+
+namespace {
+
+// Grouped launch of final_tile: 256-lane workgroups, WS lanes per trial; the
+// tiles of each task in XCD-remapped order (as final_block's).
+template <int NB, int WS>
+__global__ void __launch_bounds__(kFinalWideBlock)
+k_final_grouped_wide(const FinalTask* __restrict__ tasks, uint32_t ntasks) {
+    extern __shared__ __align__(16) unsigned char s_wide[];
+    uint32_t lo = 0;
+    uint32_t hi = ntasks;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (tasks[mid].block0 <= blockIdx.x) {
+            lo = mid;
+        } else {
+            hi = mid;
+        }
+    }
+    const FinalTask& t = tasks[lo];
+    final_tile<NB, WS>(xcd_remap(blockIdx.x - t.block0, t.ts.nblocks), t, s_wide);
+}
+
+template <int NB>
+void wide_launch(const FinalTask* tasks, uint32_t ntasks, uint32_t nblocks, uint32_t lanes,
+                 size_t lds, hipStream_t s) {
+    if constexpr (NB >= 16) {
+        if (lanes == 4) {
+            k_final_grouped_wide<NB, 4><<<nblocks, kFinalWideBlock, lds, s>>>(tasks, ntasks);
+            return;
+        }
+    }
+    if constexpr (NB <= 64) {
+        if (lanes == 2) {
+            k_final_grouped_wide<NB, 2><<<nblocks, kFinalWideBlock, lds, s>>>(tasks, ntasks);
+            return;
+        }
+    }
+    if constexpr (NB <= 32) {
+        if (lanes == 1) {
+            k_final_grouped_wide<NB, 1><<<nblocks, kFinalWideBlock, lds, s>>>(tasks, ntasks);
+            return;
+        }
+    }
+    LOKI_REQUIRE(false, "grouped last level: unsupported lanes per trial for this nbins");
+}
+
+}  // namespace
+
+uint32_t final_wide_rows(uint32_t lanes) { return kFinalWideBlock / std::max(lanes, 1U); }
+
+bool final_wide_supported(uint32_t nbins, uint32_t lanes) {
+    const bool nb_ok = nbins == 8 || nbins == 16 || nbins == 32 || nbins == 64 || nbins == 128;
+    const bool ws_ok = lanes == 1 || lanes == 2 || lanes == 4;
+    // whole merge steps per tile (256 / lanes rows >= 1024 / nbins rows per
+    // step) and at most 64 KiB of profile rows (wider rows need more lanes)
+    return nb_ok && ws_ok && nbins >= 4 * lanes && (nbins <= 32 || nbins <= 32 * lanes);
+}
+
+size_t final_wide_lds_bytes(uint32_t lanes, uint32_t stride) {
+    const size_t rows = final_wide_rows(lanes);
+    return rows * stride * sizeof(float) + rows * sizeof(FFACoord);
+}
+
+void final_grouped_wide(const FinalTask* tasks, uint32_t ntasks, uint32_t nblocks,
+                        uint32_t nbins, uint32_t lanes, size_t lds, hipStream_t s) {
+    LOKI_REQUIRE(final_wide_supported(nbins, lanes), "grouped last level: unsupported lanes");
+    LOKI_REQUIRE(lds <= 64 * 1024, "grouped last level: more than 64 KiB of LDS");
+    if (ntasks == 0 || nblocks == 0) {
+        return;
+    }
+    switch (nbins) {
+    case 8: return wide_launch<8>(tasks, ntasks, nblocks, lanes, lds, s);
+    case 16: return wide_launch<16>(tasks, ntasks, nblocks, lanes, lds, s);
+    case 32: return wide_launch<32>(tasks, ntasks, nblocks, lanes, lds, s);
+    case 64: return wide_launch<64>(tasks, ntasks, nblocks, lanes, lds, s);
+    case 128: return wide_launch<128>(tasks, ntasks, nblocks, lanes, lds, s);
+    default: LOKI_REQUIRE(false, "grouped last level: nbins must be 8, 16, 32, 64 or 128");
+    }
+}
 
 void final_grouped(const FinalTask* tasks, uint32_t ntasks, uint32_t nblocks, uint32_t nbins,
                    FoldStorage si, bool find_max, bool exact, uint32_t lds_stride,

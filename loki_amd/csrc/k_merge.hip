@@ -280,19 +280,6 @@ k_merge_grouped(const MergeTask* __restrict__ tasks, uint32_t ntasks) {
                                           t.ncur, t.nprev, t.ts, t.scale);
 }
 
-/// Rows per lane and coordinates per block of the v4 merge for NB.
-template <int NB>
-struct V4Geom {
-    static constexpr uint32_t LPR     = NB / 2;
-    static constexpr uint32_t LANES   = LPR < kBlock ? LPR : kBlock;
-    static constexpr uint32_t ROWS    = kBlock / LANES;
-    static constexpr uint32_t CPL     = LPR / LANES;
-    static constexpr uint32_t IT_ROWS = ROWS >= 64 ? 1U : 64U / ROWS;
-    static constexpr uint32_t IT_REGS = 8U / CPL;
-    static constexpr int ITERS = static_cast<int>(IT_ROWS < IT_REGS ? IT_ROWS : IT_REGS);
-    static constexpr uint32_t TILE = ROWS * ITERS;
-};
-
 template <int NB, bool POLY, typename T>
 void launch_grouped(const MergeTask* tasks, uint32_t ntasks, uint32_t nblocks, hipStream_t s) {
     k_merge_grouped<NB, POLY, V4Geom<NB>::ITERS, T><<<nblocks, kBlock, 0, s>>>(tasks, ntasks);

@@ -325,6 +325,66 @@ void final_grouped(const FinalTask* tasks, uint32_t ntasks, uint32_t nblocks, ui
 /// Coordinates per block of the fused last level, and its LDS row stride.
 constexpr uint32_t kFinalRows = 64;
 uint32_t final_lds_stride(uint32_t nbins, uint32_t wmax);
+/// Grouped fused last levels with 256-lane workgroups and `lanes` lanes per
+/// trial (1, 2 or 4), each lane taking every lanes-th width: the boxcar-width
+/// schedule the batch autotuner picks per nbins.  Tiles hold
+/// final_wide_rows(lanes) trials (the tasks' TileSchedules are built with that
+/// tile); FP32 parents, exact normalisation, one score per width; the bits of
+/// final_grouped.  lds: final_wide_lds_bytes of the launch's widest stride.
+void final_grouped_wide(const FinalTask* tasks, uint32_t ntasks, uint32_t nblocks,
+                        uint32_t nbins, uint32_t lanes, size_t lds, hipStream_t s);
+uint32_t final_wide_rows(uint32_t lanes);
+/// (nbins, lanes) built: lanes 1, 2 or 4, nbins >= 4 * lanes, and above 32
+/// bins nbins <= 32 * lanes (at most 64 KiB of profile rows per workgroup).
+bool final_wide_supported(uint32_t nbins, uint32_t lanes);
+size_t final_wide_lds_bytes(uint32_t lanes, uint32_t stride);
+
+// ---------------------------------------------------------------------------
+// Persistent batch kernel (k_tree.hip): one launch runs every stage of every
+// series of a batch.  The work is a list of items (a brute fold, a level's
+// shift table, a merge level or a fused last level of one series), each a
+// range of tickets, one ticket per 256-lane tile.  Resident workgroups take
+// tickets in order from a counter; a tile first waits until every tile of the
+// items it depends on is done (per-item counters, agent-scope acquire), and
+// signals its own item when its results are written (agent-scope release).
+// Items are ordered so that every dependency's tickets precede the item's:
+// the smallest unfinished ticket never waits on an unfinished tile, so the
+// launch always drains.
+// ---------------------------------------------------------------------------
+enum class TreeKind : uint32_t { kResolve = 0, kFoldList = 1, kFoldUnits = 2, kMerge = 3, kFinal = 4 };
+constexpr uint32_t kTreeBlock  = 256;
+constexpr uint32_t kTreeNoDep  = 0xFFFFFFFFU;
+constexpr uint32_t kTreeSyncHeader = 3;  ///< ticket counter, finished workgroups, error flags
+struct TreeItem {
+    uint32_t kind;     ///< TreeKind
+    uint32_t nbins;    ///< merge / final: bins of the series
+    uint32_t task;     ///< index into the kind's task table
+    uint32_t ticket0;  ///< first ticket (items sorted by ticket0)
+    uint32_t ntiles;   ///< tickets of the item
+    uint32_t dep[2];   ///< items whose tiles must all be done first (kTreeNoDep: none)
+    uint32_t lanes;    ///< final: threads per trial in the S/N stage (1, 2, 4 or 8)
+};
+struct TreeArgs {
+    const TreeItem* items;  ///< device
+    uint32_t nitems;
+    uint32_t ntickets;
+    const ResolveTask* resolve;  ///< device task tables (block0 unused)
+    const FoldTask* fold;
+    const MergeTask* merge;
+    const FinalTask* final_;
+    /// device, kTreeSyncHeader + nitems words, zero before the first launch;
+    /// the launch leaves [0, 2) and the item counters zero again, and ORs 1
+    /// into [2] if a wait exceeded spin_limit polls (then results are invalid)
+    uint32_t* sync;
+    uint32_t spin_limit;
+};
+/// Dynamic LDS bytes of one item kind (fold: its nbins; final: nbins, lanes
+/// and the LDS row stride of final_lds_stride).
+size_t tree_lds_bytes(TreeKind kind, uint32_t nbins, uint32_t lanes, uint32_t stride);
+/// Workgroups of 256 lanes resident at once with `lds` bytes each.
+uint32_t tree_resident_blocks(size_t lds);
+/// One persistent launch of nblocks workgroups over every ticket.
+void tree_launch(const TreeArgs& a, uint32_t nblocks, size_t lds, hipStream_t s);
 
 /// snr_boxcar_3d[_max]: folds [nprof, 2, nbins] -> scores [nprof, nw] or [nprof].
 /// With `fo`, the scores go to the compacted candidate list instead (`scores`

@@ -197,6 +197,21 @@ constexpr int kMergeBlock = 256;
 /// `ein` of the parent level, `eout` of the child level (16-bit word per row
 /// pair); the child's row exponents come from the row maxima of the FP32
 /// sums, reduced over the C4 lanes of each component row.
+/// Geometry of merge_v4_block for NB bins: lanes per row, rows per pass and
+/// the passes (ITERS) whose loads are in flight together; TILE coordinates
+/// per block.
+template <int NB>
+struct V4Geom {
+    static constexpr uint32_t LPR     = NB / 2;
+    static constexpr uint32_t LANES   = LPR < kMergeBlock ? LPR : kMergeBlock;
+    static constexpr uint32_t ROWS    = kMergeBlock / LANES;
+    static constexpr uint32_t CPL     = LPR / LANES;
+    static constexpr uint32_t IT_ROWS = ROWS >= 64 ? 1U : 64U / ROWS;
+    static constexpr uint32_t IT_REGS = 8U / CPL;
+    static constexpr int ITERS = static_cast<int>(IT_ROWS < IT_REGS ? IT_ROWS : IT_REGS);
+    static constexpr uint32_t TILE = ROWS * ITERS;
+};
+
 template <int NB, bool POLY, int ITERS, typename TI, typename TO>
 __device__ __forceinline__ void merge_v4_block(uint32_t bid, const TI* __restrict__ in,
                                                TO* __restrict__ out,
@@ -205,7 +220,8 @@ __device__ __forceinline__ void merge_v4_block(uint32_t bid, const TI* __restric
                                                uint64_t ncur, uint64_t nprev,
                                                const TileSchedule& ts, float scale,
                                                const uint16_t* __restrict__ ein = nullptr,
-                                               uint8_t* __restrict__ eout = nullptr) {
+                                               uint8_t* __restrict__ eout = nullptr,
+                                               bool remap = true) {
     constexpr uint32_t kBlock = kMergeBlock;
     using CI = chunk4_t<TI>;
     using CO = chunk4_t<TO>;
@@ -216,7 +232,7 @@ __device__ __forceinline__ void merge_v4_block(uint32_t bid, const TI* __restric
     constexpr uint32_t TILE = ROWS * ITERS;
     static_assert(!(kBlockFP<TI> || kBlockFP<TO>) || (C4 <= 64 && LPR <= kBlock),
                   "block floating point: a component row must fit one wave");
-    const Tile tl = tile_of(ts, xcd_remap(bid, ts.nblocks), TILE);
+    const Tile tl = tile_of(ts, remap ? xcd_remap(bid, ts.nblocks) : bid, TILE);
     if (tl.count == 0) {
         return;
     }

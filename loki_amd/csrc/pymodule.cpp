@@ -498,8 +498,10 @@ public:
         check(loki_batch_execute(m_h.get(), reinterpret_cast<void*>(stream)));
     }
     void set_graph(bool on) { check(loki_batch_set_graph(m_h.get(), on ? 1 : 0)); }
+    /// Keys: merge_group, merge_run, final_group, final_run, persistent and
+    /// score_lanes_<nbins> (nbins 8..128); keys left out keep their defaults.
     void set_schedule(const std::map<std::string, uint32_t>& sc) {
-        loki_batch_schedule b{16, 16, 16, 16};
+        loki_batch_schedule b{16, 16, 16, 16, 0, {0, 0, 0, 0, 0}};
         for (const auto& [k, v] : sc) {
             if (k == "merge_group") {
                 b.merge_group = v;
@@ -509,19 +511,44 @@ public:
                 b.final_group = v;
             } else if (k == "final_run") {
                 b.final_run = v;
+            } else if (k == "persistent") {
+                b.persistent = v;
+            } else if (k.rfind("score_lanes_", 0) == 0) {
+                const int nb = std::stoi(k.substr(12));
+                int idx      = -1;
+                for (int i = 0; i < 5; ++i) {
+                    idx = (nb == (8 << i)) ? i : idx;
+                }
+                if (idx < 0) {
+                    throw std::runtime_error("unknown schedule key " + k);
+                }
+                b.score_lanes[idx] = v;
             } else {
                 throw std::runtime_error("unknown schedule key " + k);
             }
         }
         check(loki_batch_set_schedule(m_h.get(), &b));
     }
+    static std::map<std::string, uint32_t> as_map(const loki_batch_schedule& b) {
+        std::map<std::string, uint32_t> m{{"merge_group", b.merge_group},
+                                          {"merge_run", b.merge_run},
+                                          {"final_group", b.final_group},
+                                          {"final_run", b.final_run},
+                                          {"persistent", b.persistent}};
+        for (int i = 0; i < 5; ++i) {
+            m["score_lanes_" + std::to_string(8 << i)] = b.score_lanes[i];
+        }
+        return m;
+    }
     std::map<std::string, uint32_t> autotune(int repeats, uintptr_t stream) {
         loki_batch_schedule b{};
         check(loki_batch_autotune(m_h.get(), reinterpret_cast<void*>(stream), repeats, &b));
-        return {{"merge_group", b.merge_group},
-                {"merge_run", b.merge_run},
-                {"final_group", b.final_group},
-                {"final_run", b.final_run}};
+        return as_map(b);
+    }
+    uint32_t errors() {
+        uint32_t f = 0;
+        check(loki_batch_errors(m_h.get(), &f));
+        return f;
     }
     py::dict info() const {
         size_t n = 0, trials = 0, bytes = 0, launches = 0;
@@ -1124,6 +1151,7 @@ PYBIND11_MODULE(_loki_amd, m) {
         .def("set_graph", &Batch::set_graph, py::arg("enable"))
         .def("set_schedule", &Batch::set_schedule, py::arg("schedule"))
         .def("autotune", &Batch::autotune, py::arg("repeats") = 3, py::arg("stream") = 0)
+        .def("errors", &Batch::errors)
         .def("info", &Batch::info);
     m_ffa.def(
         "compute_ffa_fourier_cuda",
