@@ -822,10 +822,25 @@ def main() -> int:
     torch.cuda.synchronize()
     ffa.set_timing(False)
 
+    def merge_levels_ms():
+        """hipEvent time of the stored levels' merges (the memory-bound part that the fold
+        storage changes), min over 3 steps."""
+        ffa.set_timing(True)
+        best = None
+        with torch.cuda.stream(stream):
+            for _ in range(3):
+                step()
+                t = sum(ms for kind, _level, ms in ffa.timings() if str(kind) == "merge")
+                best = t if best is None else min(best, t)
+        torch.cuda.synchronize()
+        ffa.set_timing(False)
+        return best
+
     # The same step with the other fold storage precisions (same series, same stream).
     storage_cmp = None
     if not fourier and not args.no_storage_compare:
         storage_cmp = {args.storage: ms_per_step}
+        merge_cmp = {args.storage: merge_levels_ms()}
         for st in ("f32", "f16", "bf16", "bfp16"):
             if st == args.storage:
                 continue
@@ -839,6 +854,7 @@ def main() -> int:
                     step()
             torch.cuda.synchronize()
             storage_cmp[st] = (time.perf_counter() - ta) * 1e3 / args.steps
+            merge_cmp[st] = merge_levels_ms()
         ffa.set_fold_storage(args.storage)
         with torch.cuda.stream(stream):
             step()  # scores of the main run's storage for the recovery check
@@ -847,8 +863,13 @@ def main() -> int:
             "ms_per_step": {k: round(v, 4) for k, v in storage_cmp.items()},
             "speedup_vs_f32": {k: round(storage_cmp["f32"] / v, 4)
                                for k, v in storage_cmp.items()},
-            "note": "levels 0..L-2 stored at that precision (bfp16: block floating point, "
-                    "levels 1..L-2, level 0 in FP32), every add in FP32; S/N error vs FP64 in "
+            "merge_levels_ms": {k: round(v, 4) for k, v in merge_cmp.items()},
+            "merge_levels_speedup_vs_f32": {k: round(merge_cmp["f32"] / v, 4)
+                                            for k, v in merge_cmp.items() if v > 0},
+            "note": "stored levels at that precision (bfp16: block floating point, level 0 in "
+                    "FP32), every add in FP32; merge_levels_ms: the stored levels' merges alone "
+                    "(hipEvent, the memory-bound part), the fused last level reads them once "
+                    "and computes in FP32 for every storage; S/N error vs FP64 in "
                     "snr_error_vs_fp64 (jerk workload) and "
                     "tests/test_gpu_parity.py::test_storage_snr_error_vs_fp64",
         }

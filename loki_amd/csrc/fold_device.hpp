@@ -216,6 +216,7 @@ __device__ __forceinline__ void bf_units_block(uint32_t bid, const float* __rest
     const float* __restrict__ ge = ts_e + seg * seglen;
     const float* __restrict__ gv = ts_v + seg * seglen;
     const int64_t nb = nbins;
+    const bool pow2  = (nbins & (nbins - 1U)) == 0;
     const uint32_t tid = threadIdx.x;
     for (uint32_t i = tid; i < nf * nbins; i += kFoldBlock) {
         acc[i] = make_float2(0.0F, 0.0F);
@@ -251,8 +252,9 @@ __device__ __forceinline__ void bf_units_block(uint32_t bid, const float* __rest
         }
         __syncthreads();
         const uint32_t total = s_pref[nf];
+        uint32_t fi = 0;  // units are ordered by frequency and t only grows
         for (uint32_t t = tid; t < total; t += kFoldBlock) {
-            uint32_t lo = 0, hi = nf;  // largest fi with s_pref[fi] <= t
+            uint32_t lo = fi, hi = nf;  // largest fi with s_pref[fi] <= t
             while (hi - lo > 1) {
                 const uint32_t mid = (lo + hi) / 2;
                 if (s_pref[mid] <= t) {
@@ -261,11 +263,14 @@ __device__ __forceinline__ void bf_units_block(uint32_t bid, const float* __rest
                     hi = mid;
                 }
             }
-            const uint32_t fi  = lo;
+            fi                 = lo;
             const int64_t u0   = s_ulo[fi] + static_cast<int64_t>(t - s_pref[fi]);
             const int64_t uhi  = s_uhi[fi];
             const RunGeom g    = s_geom[fi];
-            const uint32_t b   = static_cast<uint32_t>(((u0 % nb) + nb) % nb);
+            // bin of the unit: u0 mod nbins (a mask for power-of-two nbins, which holds
+            // for negative u0 too in two's complement)
+            const uint32_t b   = pow2 ? static_cast<uint32_t>(u0) & (nbins - 1U)
+                                      : static_cast<uint32_t>(((u0 % nb) + nb) % nb);
             float2* slot       = acc + fi * nbins + b;
             float ae = slot->x, av = slot->y;
             for (int64_t u = u0; u <= uhi; u += nb) {
@@ -280,14 +285,32 @@ __device__ __forceinline__ void bf_units_block(uint32_t bid, const float* __rest
                     av = av + x.y;
                     ++k;
                 }
-                // 16-byte reads of two (e, v) pairs: fewer LDS instructions, and 8 lanes per
-                // LDS pass instead of 16 (fewer bank collisions between the lanes' runs)
-                const auto* s4 = reinterpret_cast<const float4*>(s_ev);
-                for (; k + 8 <= z; k += 8) {
+                // 16-byte reads of two (e, v) pairs (fewer LDS instructions, and 8 lanes
+                // per LDS pass instead of 16), 16 samples per step, then 8, then one by
+                // one: every sample is still added in ascending order
+                const auto* __restrict__ p = reinterpret_cast<const float4*>(s_ev) +
+                                              (k < z ? (k - c0) / 2 : 0);
+                const uint32_t npair = k < z ? (z - k) / 2 : 0;
+                const float4* const pend = p + npair;
+                for (; p + 8 <= pend; p += 8) {
+                    float4 y[8];
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        y[i] = p[i];
+                    }
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        ae = ae + y[i].x;
+                        av = av + y[i].y;
+                        ae = ae + y[i].z;
+                        av = av + y[i].w;
+                    }
+                }
+                if (p + 4 <= pend) {
                     float4 y[4];
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
-                        y[i] = s4[(k - c0) / 2 + i];
+                        y[i] = p[i];
                     }
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
@@ -296,8 +319,17 @@ __device__ __forceinline__ void bf_units_block(uint32_t bid, const float* __rest
                         ae = ae + y[i].z;
                         av = av + y[i].w;
                     }
+                    p += 4;
                 }
-                for (; k < z; ++k) {
+                for (; p < pend; ++p) {
+                    const float4 y = *p;
+                    ae = ae + y.x;
+                    av = av + y.y;
+                    ae = ae + y.z;
+                    av = av + y.w;
+                }
+                k += 2 * npair;
+                if (k < z) {  // an odd last sample
                     const float2 x = s_ev[k - c0];
                     ae = ae + x.x;
                     av = av + x.y;
