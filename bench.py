@@ -715,6 +715,35 @@ def fused_roofline(plan, nbins, nw, args, wl, avg_ms, dom_name, variant, traffic
             "hbm": hbm}
 
 
+def practical_hbm(nbytes: int = 2 << 30) -> dict:
+    """What a plain streaming kernel reaches on this GPU: torch's device-to-device copy of
+    nbytes (read + write), best of 3 by hipEvent, beside the 8 TB/s peak."""
+    import torch
+
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    out = {}
+    for name, fn, moved in (("copy", lambda: b.copy_(a), 2 * nbytes),):
+        fn()
+        best = None
+        for _ in range(3):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            e1.synchronize()
+            ms = e0.elapsed_time(e1)
+            best = ms if best is None else min(best, ms)
+        out[f"{name}_GBps"] = moved / (best * 1e-3) / 1e9
+    del a, b
+    torch.cuda.empty_cache()
+    out["peak_GBps"] = HBM_PEAK_GBPS
+    out["note"] = ("a plain streaming copy (torch copy_ of 2 GiB, bytes read + written) in the "
+                   "same run: the bandwidth a merge level can expect in practice, beside the "
+                   "nominal peak")
+    return out
+
+
 def step_floor(plan, nbins, nw, nsamps, args, ms_per_step, fourier):
     """HBM floor of one step: the input series read once, every level that goes through HBM
     written once and read once, the scores written -- against the measured step."""
@@ -992,6 +1021,7 @@ def main() -> int:
                             "fused_irfft_score" if fourier else "fused"),
         },
         "roofline": roof,
+        "hbm_practical": practical_hbm() if not fourier else None,
         "step_floor": step_floor(plan, nbins, nw, wl["nsamps"], args, ms_per_step, fourier),
         "unfused_alg_bytes": total_alg,
         "kernel_ms": kern_ms,
