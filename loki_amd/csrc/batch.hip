@@ -133,13 +133,23 @@ void FFABatch::set_schedule(const BatchSchedule& sc) {
                  "FFABatch: the persistent launch is disabled (its cross-workgroup completion "
                  "counters stalled on MI355X); use the grouped launches (persistent = 0)");
     for (uint32_t k = 0; k < 5; ++k) {
-        LOKI_REQUIRE(sc.score_lanes[k] == 0 || kern::final_wide_supported(8U << k, sc.score_lanes[k]),
+        const uint32_t lanes = sc.score_lanes[k];
+        LOKI_REQUIRE(lanes == 0 || kern::final_wide_supported(8U << k, lanes),
                      "FFABatch: score_lanes must be 0, 1, 2 or 4, with nbins >= 4 * lanes and "
                      "nbins <= 32 * lanes above 32 bins");
     }
-    m_sched = sc;
     DeviceGuard g(m_device);
-    build_tasks();
+    const BatchSchedule prev = m_sched;
+    m_sched                  = sc;
+    try {
+        build_tasks();
+    } catch (...) {
+        // a schedule the batch cannot run (e.g. a last level over the LDS budget): keep the
+        // previous one in force
+        m_sched = prev;
+        build_tasks();
+        throw;
+    }
 }
 
 void FFABatch::set_graph(bool on) {

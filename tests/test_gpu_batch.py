@@ -191,6 +191,43 @@ def test_batch_width_schedules_equal_one_lane_per_trial():
                                               err_msg=f"{sc} replay {rep} series {i}")
 
 
+def test_batch_width_schedules_small_nbins_and_wide_windows(oracle):
+    """Width schedules at the edges: nbins 8 and 16 (two lanes per trial at most for 8) and
+    nbins 64 with windows up to 63 bins, whose prefix rows need four lanes per trial to fit the
+    64 KiB workgroup budget (two lanes are rejected); every score equals the oracle's."""
+    from loki_amd import libculoki, libloki
+    from loki_amd.synth import make_batch_series, make_batch_specs
+    from oracle.plan import SearchConfig
+
+    specs = make_batch_specs(6, 4100, log2_len=(15, 17), nbins_choices=(8, 16, 64),
+                             period_centre=(0.05, 0.3))
+    assert {sp["nbins"] for sp in specs} == {8, 16, 64}
+    widths = np.arange(1, 64, dtype=np.uint64)
+    cfgs, series = [], []
+    for sp in specs:
+        cfgs.append(libloki.configs.PulsarSearchConfig(
+            nsamps=sp["nsamps"], tsamp=TSAMP, nbins=sp["nbins"], eta=1.0,
+            param_limits=np.array([[sp["f_lo"], sp["f_hi"]]]), use_fourier=False))
+        series.append(make_batch_series(sp, TSAMP))
+    b = libculoki.ffa.FFABatchCUDA(cfgs, widths=widths)
+    for i, (e, v) in enumerate(series):
+        b.set_input(i, e, v)
+    with pytest.raises(RuntimeError, match="64 KiB"):
+        b.set_schedule({"score_lanes_64": 2})
+    b.set_graph(False)
+    for sc in ({"score_lanes_8": 1, "score_lanes_16": 4, "score_lanes_64": 4},
+               {"score_lanes_8": 2, "score_lanes_16": 2, "score_lanes_64": 4}):
+        print("schedule", sc, flush=True)
+        b.set_schedule(sc)
+        b.execute()
+        for i, (sp, (e, v)) in enumerate(zip(specs, series)):
+            w = widths[widths < sp["nbins"]]
+            ocfg = SearchConfig(sp["nsamps"], TSAMP, sp["nbins"], 1.0, [(sp["f_lo"], sp["f_hi"])],
+                                use_fourier=False)
+            want, _ = oracle.compute_ffa_scores(e, v, ocfg, widths=w)
+            np.testing.assert_array_equal(b.scores(i), want, err_msg=f"{sc} series {i}")
+
+
 def test_batch_schedule_errors():
     from loki_amd import libculoki
 
