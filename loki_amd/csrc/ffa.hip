@@ -511,6 +511,12 @@ FFACoord resolve_rc(const kern::ResolveLevel& lv, const double* dv, uint32_t row
     }
 }
 
+kern::RowTerms row_terms_of(const kern::ResolveLevel& lv, const double* dv, uint32_t row,
+                            int side) {
+    return lv.np == 2 ? kern::row_terms<2>(lv, dv, row, side)
+                      : kern::row_terms<3>(lv, dv, row, side);
+}
+
 }  // namespace
 
 void FFAExecutor::prepare_final2(hipStream_t s) {
@@ -566,6 +572,22 @@ void FFAExecutor::prepare_final2(hipStream_t s) {
                              m_f2_chunks_host.size() * sizeof(kern::RowChunk),
                              hipMemcpyHostToDevice, s),
               "copy final-level row chunks");
+    // the frequency-independent resolver terms of every grid row, formed once
+    // here with the kernel's own (host and device) code
+    m_f2_rows_host.clear();
+    if (lv.np >= 2) {
+        m_f2_rows_host.resize(2 * n_rows);
+        for (SizeType r = 0; r < n_rows; ++r) {
+            for (int side = 0; side < 2; ++side) {
+                m_f2_rows_host[2 * r + side] = row_terms_of(lv, dv, static_cast<uint32_t>(r), side);
+            }
+        }
+        m_f2_rows.reserve(m_f2_rows_host.size() * sizeof(kern::RowTerms));
+        hip_check(hipMemcpyAsync(m_f2_rows.get(), m_f2_rows_host.data(),
+                                 m_f2_rows_host.size() * sizeof(kern::RowTerms),
+                                 hipMemcpyHostToDevice, s),
+                  "copy final-level row terms");
+    }
     // parent rows staged per side: the exact widest range of any block, from
     // the device resolver itself (one pass per plan; synchronises s)
     kern::Final2Args a{};
@@ -659,6 +681,7 @@ void FFAExecutor::scores_levels(const float* ts_e, const float* ts_v,
         a.fo             = fo != nullptr ? *fo : kern::FilterOut{};
         a.chunks         = m_f2_chunks.as<kern::RowChunk>();
         a.n_chunks       = static_cast<uint32_t>(m_f2_chunks_host.size());
+        a.rows           = m_f2_rows_host.empty() ? nullptr : m_f2_rows.as<kern::RowTerms>();
         a.rows_per_block = m_f2_rows_per_block;
         a.cap            = m_f2_cap;
         a.grand          = grand ? 1U : 0U;

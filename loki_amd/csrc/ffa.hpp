@@ -184,6 +184,8 @@ private:
     bool m_f2_ok{false};  // parent ranges fit the LDS budget
     std::vector<kern::RowChunk> m_f2_chunks_host;
     DeviceBuffer m_f2_chunks;
+    std::vector<kern::RowTerms> m_f2_rows_host;  // (tail, head) per grid row, np >= 2
+    DeviceBuffer m_f2_rows;
     uint32_t m_f2_rows_per_block{1};
     uint32_t m_f2_cap{0};
     uint32_t m_f2_nblocks{0};

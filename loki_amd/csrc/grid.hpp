@@ -20,17 +20,24 @@ inline constexpr double kCval     = 299792458.0;
 inline constexpr double kInvCval  = 3.3356409519815204e-09;
 inline constexpr double kFloatEps = 1e-6;
 
-/// psr_utils.hpp:32-49: frac((t - delay) * f) * nbins, rounded to float;
-/// a value equal to nbins wraps to 0.
-LOKI_HD inline float phase_idx(double proper_time, double freq, uint32_t nbins,
-                               double delay) {
-    const double total = (proper_time - delay) * freq;
+/// psr_utils.hpp:32-49 (phase_idx below) with its first factor
+/// td = proper_time - delay formed by the caller: a grid row's resolver terms
+/// hold it once per row.
+LOKI_HD inline float phase_idx_td(double td, double freq, uint32_t nbins) {
+    const double total = td * freq;
     const double norm  = total - floor(total);
     double iphase      = norm * static_cast<double>(nbins);
     if (iphase >= static_cast<double>(nbins)) {
         iphase = 0.0;
     }
     return static_cast<float>(iphase);
+}
+
+/// psr_utils.hpp:32-49: frac((t - delay) * f) * nbins, rounded to float;
+/// a value equal to nbins wraps to 0.
+LOKI_HD inline float phase_idx(double proper_time, double freq, uint32_t nbins,
+                               double delay) {
+    return phase_idx_td(proper_time - delay, freq, nbins);
 }
 
 /// psr_utils.hpp:53-63: integer phase bin u32(phase + 0.5f), nbins -> 0.

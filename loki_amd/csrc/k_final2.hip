@@ -7,7 +7,8 @@
 // consecutive frequencies; one wave covers 64 frequencies of one row and each
 // lane one trial:
 //   0. the lane resolves its trial in FP64 (taylor_ffa.cpp:19-271 order, the
-//      same bits as the table kernels): parent indices and shifts;
+//      same bits as the table kernels): parent indices and shifts, from the
+//      grid row's frequency-independent terms formed once per row on the host;
 //   1. the block's parent rows -- a contiguous range per side, since the
 //      chunk's rows share parent rows and the parent frequency index is
 //      monotone in the trial frequency -- are staged in LDS as (e, v) pairs
@@ -43,7 +44,8 @@ constexpr uint32_t kWaves = 4;
 constexpr uint32_t kBlock = kWaves * kLanes;
 /// Grid rows per block (rows sharing their parent rows): wave w takes rows
 /// w, w + 4, w + 8 and w + 12 of its chunk, so the staged parent rows serve
-/// up to 16 rows of trials.
+/// up to 16 rows of trials.  Chunks of at most 4 rows run the one-row-per-wave
+/// instance (RPW = 1).
 constexpr uint32_t kRowsPerWave = 4;
 constexpr uint32_t kMaxRows     = kWaves * kRowsPerWave;
 
@@ -63,21 +65,6 @@ constexpr uint32_t kPad = 3;
 template <int NB>
 constexpr uint32_t kStride = ((NB + kPad + 10) / 16) * 16 + 5;
 static_assert(kStride<32> == 37 && kStride<16> == 21 && kStride<8> == 21);
-
-__device__ __forceinline__ uint32_t wave_min(uint32_t x) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) {
-        x = min(x, static_cast<uint32_t>(__shfl_xor(static_cast<int>(x), o)));
-    }
-    return x;
-}
-__device__ __forceinline__ uint32_t wave_max(uint32_t x) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) {
-        x = max(x, static_cast<uint32_t>(__shfl_xor(static_cast<int>(x), o)));
-    }
-    return x;
-}
 
 /// Reduced-precision rounding of a stored level: FP32 -> TI -> FP32 (exact
 /// round trip of the rounded value); identity for FP32.
@@ -160,7 +147,7 @@ __device__ __forceinline__ float4 normalise4v(const float4 e, const float4 v) {
     }
 }
 
-template <int NB, int NP, bool FINDMAX, bool EXACT, typename TI>
+template <int NB, int NP, int RPW, bool FINDMAX, bool EXACT, typename TI>
 __global__ void __launch_bounds__(kBlock)
 k_final2(Final2Args a) {
     constexpr uint32_t C4     = NB / 4;           // 4-bin chunks per row
@@ -168,7 +155,8 @@ k_final2(Final2Args a) {
     extern __shared__ float2 s_rows[];            // [2 * cap][STRIDE]
     __shared__ uint32_t s_rng[4];                 // lo_t, hi_t, lo_h, hi_h
 
-    const uint32_t wave = threadIdx.x / kLanes;  // rows wave, wave + 4, ... of the chunk
+    // rows wave, wave + 4, ... of the chunk (wave-uniform: the row terms are scalar loads)
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
     const uint32_t lane = threadIdx.x % kLanes;  // frequency of the tile
     const uint32_t nthr = blockDim.x;
     const uint32_t id   = xcd_remap(blockIdx.x, gridDim.x);
@@ -186,13 +174,19 @@ k_final2(Final2Args a) {
     auto has_row = [&](uint32_t k) { return in_tile && wave + k * kWaves < nrows; };
 
     // 0. this lane's trials (one per row of the wave)
-    FFACoord cds[kRowsPerWave];
+    FFACoord cds[RPW];
     uint32_t lt = 0xFFFFFFFFU, ht = 0, lh = 0xFFFFFFFFU, hh = 0;
 #pragma unroll
-    for (uint32_t k = 0; k < kRowsPerWave; ++k) {
+    for (uint32_t k = 0; k < RPW; ++k) {
         cds[k] = FFACoord{0, 0.0F, 0, 0.0F};
         if (has_row(k)) {
-            cds[k] = resolve_pair_rc<NP>(a.lv, a.fg.dv, row0 + wave + k * kWaves, jf);
+            if constexpr (NP == 1) {
+                cds[k] = resolve_pair_rc<1>(a.lv, a.fg.dv, row0 + wave + k * kWaves, jf);
+            } else {
+                // the row's terms were formed once per grid row (same operations)
+                const RowTerms* rt = a.rows + 2 * static_cast<size_t>(row0 + wave + k * kWaves);
+                cds[k]             = resolve_pair_rt(a.lv, a.fg.dv, rt[0], rt[1], jf);
+            }
             lt     = min(lt, cds[k].i_tail);
             ht     = max(ht, cds[k].i_tail);
             lh     = min(lh, cds[k].i_head);
@@ -204,10 +198,17 @@ k_final2(Final2Args a) {
     }
     __syncthreads();
     {
-        lt = wave_min(lt);
-        ht = wave_max(ht);
-        lh = wave_min(lh);
-        hh = wave_max(hh);
+        // the parent indices are non-decreasing in the frequency along a grid
+        // row (monotone roundings of f * (1 - v/c)), so the wave's minima sit
+        // in lane 0 and its maxima in the last lane inside the grid
+        const uint64_t f0   = static_cast<uint64_t>(tile) * kLanes;
+        const uint32_t last = f0 < n_f ? static_cast<uint32_t>(
+                                             min<uint64_t>(kLanes - 1, n_f - 1 - f0))
+                                       : 0U;
+        lt = __builtin_amdgcn_readfirstlane(lt);
+        lh = __builtin_amdgcn_readfirstlane(lh);
+        ht = __builtin_amdgcn_readlane(ht, last);
+        hh = __builtin_amdgcn_readlane(hh, last);
         if (lane == 0 && lt != 0xFFFFFFFFU) {
             atomicMin(&s_rng[0], lt);
             atomicMax(&s_rng[1], ht);
@@ -223,10 +224,24 @@ k_final2(Final2Args a) {
     }
     const uint32_t n_t = s_rng[1] - lo_t + 1;
     const uint32_t n_h = s_rng[3] - lo_h + 1;
-    if (n_t > a.cap || n_h > a.cap) {
+    // each trial's parent rows as staged slots and integer shifts, one word
+    // per row (slots < 256: the LDS budget bounds cap; shifts < NB)
+    uint32_t trial[RPW];
+    bool outside = n_t > a.cap || n_h > a.cap;
+#pragma unroll
+    for (uint32_t k = 0; k < RPW; ++k) {
+        const uint32_t St = NP == 1 ? 0U : shift_int(cds[k].shift_tail, NB);
+        const uint32_t Sh = shift_int(cds[k].shift_head, NB);
+        const uint32_t dt = cds[k].i_tail - lo_t;
+        const uint32_t dh = cds[k].i_head - lo_h;
+        outside           = outside || (has_row(k) && (dt >= n_t || dh >= n_h));
+        trial[k]          = (dt & 0xFFU) | (St << 8) | ((dh & 0xFFU) << 16) | (Sh << 24);
+    }
+    if (__syncthreads_or(outside ? 1 : 0) != 0) {
         // cannot happen: cap is the exact maximum over all blocks
-        // (final2_ranges); never index past the staged rows
-        for (uint32_t k = 0; k < kRowsPerWave; ++k) {
+        // (final2_ranges) and the range holds every trial of the block; never
+        // index past the staged rows
+        for (uint32_t k = 0; k < RPW; ++k) {
             if (has_row(k) && a.fo.count == nullptr) {
                 const uint64_t c_idx =
                     static_cast<uint64_t>(row0 + wave + k * kWaves) * n_f + jf;
@@ -237,16 +252,6 @@ k_final2(Final2Args a) {
             }
         }
         return;
-    }
-    // each trial's parent rows as staged slots and integer shifts, one word
-    // per row (slots < 256: the LDS budget bounds cap; shifts < NB)
-    uint32_t trial[kRowsPerWave];
-#pragma unroll
-    for (uint32_t k = 0; k < kRowsPerWave; ++k) {
-        const uint32_t St = NP == 1 ? 0U : shift_int(cds[k].shift_tail, NB);
-        const uint32_t Sh = shift_int(cds[k].shift_head, NB);
-        trial[k] = ((cds[k].i_tail - lo_t) & 0xFFU) | (St << 8) |
-                   (((cds[k].i_head - lo_h) & 0xFFU) << 16) | (Sh << 24);
     }
 
     // 1. stage the parent rows: task (row slot, 4-bin chunk)
@@ -352,7 +357,7 @@ k_final2(Final2Args a) {
     __syncthreads();
 
 #pragma unroll 1
-    for (uint32_t k = 0; k < kRowsPerWave; ++k) {
+    for (uint32_t k = 0; k < RPW; ++k) {
         if (!has_row(k)) {
             continue;
         }
@@ -474,18 +479,27 @@ k_final2_ranges(Final2Args a, uint32_t nblocks, uint32_t* out) {
     atomicMax(out, m);
 }
 
+template <int NB, int NP, int RPW, typename TI>
+void launch4(const Final2Args& a, uint32_t nblocks, size_t lds, bool find_max, bool exact,
+             hipStream_t s) {
+    if (find_max && exact) {
+        k_final2<NB, NP, RPW, true, true, TI><<<nblocks, kBlock, lds, s>>>(a);
+    } else if (find_max) {
+        k_final2<NB, NP, RPW, true, false, TI><<<nblocks, kBlock, lds, s>>>(a);
+    } else if (exact) {
+        k_final2<NB, NP, RPW, false, true, TI><<<nblocks, kBlock, lds, s>>>(a);
+    } else {
+        k_final2<NB, NP, RPW, false, false, TI><<<nblocks, kBlock, lds, s>>>(a);
+    }
+}
+
 template <int NB, int NP, typename TI>
 void launch3(const Final2Args& a, uint32_t nblocks, bool find_max, bool exact, hipStream_t s) {
-    const size_t lds     = static_cast<size_t>(2) * a.cap * kStride<NB> * sizeof(float2);
-    const uint32_t block = kBlock;
-    if (find_max && exact) {
-        k_final2<NB, NP, true, true, TI><<<nblocks, block, lds, s>>>(a);
-    } else if (find_max) {
-        k_final2<NB, NP, true, false, TI><<<nblocks, block, lds, s>>>(a);
-    } else if (exact) {
-        k_final2<NB, NP, false, true, TI><<<nblocks, block, lds, s>>>(a);
+    const size_t lds = static_cast<size_t>(2) * a.cap * kStride<NB> * sizeof(float2);
+    if (a.rows_per_block > kWaves) {
+        launch4<NB, NP, kRowsPerWave, TI>(a, nblocks, lds, find_max, exact, s);
     } else {
-        k_final2<NB, NP, false, false, TI><<<nblocks, block, lds, s>>>(a);
+        launch4<NB, NP, 1, TI>(a, nblocks, lds, find_max, exact, s);
     }
 }
 
@@ -541,6 +555,7 @@ void final2_merge_score(const Final2Args& a, FoldStorage si, uint32_t nblocks, b
     LOKI_REQUIRE(a.cap <= 256, "final-level kernel: parent ranges above 256 rows");
     LOKI_REQUIRE(a.n_chunks > 0 && a.chunks != nullptr && a.cap > 0,
                  "final-level kernel: no row chunks");
+    LOKI_REQUIRE(a.lv.np == 1 || a.rows != nullptr, "final-level kernel: row terms missing");
     if (a.grand) {
         LOKI_REQUIRE(a.lv.np == 1 ? a.coords1f != nullptr : a.coords1 != nullptr,
                      "final-level kernel: parent shift table missing");

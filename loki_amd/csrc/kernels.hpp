@@ -27,6 +27,16 @@ struct ResolveLevel {
     uint32_t np;       ///< 1 (freq), 2 (accel, freq) or 3 (jerk, accel, freq)
 };
 
+/// Row part of the last level's resolver for one side of one grid row
+/// (flattened (jerk, accel) cell): f_new = f_cur * fac, the phase total
+/// td * f_cur (td = delta_t - delay_rel) and the parent index base + idx_f.
+struct RowTerms {
+    double fac;
+    double td;
+    uint32_t base;
+    uint32_t pad;
+};
+
 /// Extra per-level constants of the fused last-level kernel.
 struct FinalGrid {
     double dv[3];     ///< (hi - lo) / n_cur per parameter (psr_utils.hpp:290-291)
@@ -220,6 +230,7 @@ struct Final2Args {
     float* scores;
     FilterOut fo;
     const RowChunk* chunks;  ///< device
+    const RowTerms* rows;    ///< device, np >= 2: (tail, head) terms of every grid row
     uint32_t n_chunks;
     uint32_t rows_per_block;  ///< most rows in a chunk (<= 16; 4 waves, 4 rows each)
     uint32_t cap;             ///< staged parent rows per side
