@@ -254,19 +254,24 @@ k_final2(Final2Args a) {
         return;
     }
 
-    // 1. stage the parent rows: task (row slot, 4-bin chunk)
+    // 1. stage the parent rows: task (row slot, chunks qp and qp + TPS of 4
+    //    bins); `unsafe` marks a staged variance outside [2^-96, 2^126), the
+    //    range whose pairwise sums the short exact normalisation takes
+    constexpr uint32_t CH  = 2;         // 4-bin chunks per task
+    constexpr uint32_t TPS = C4 / CH;   // tasks per staged row
+    bool unsafe = kScaledHalf<TI>;      // scaled storage: sums are rescaled first
     {
-        const uint32_t ntask = (n_t + n_h) * C4;
+        const uint32_t ntask = (n_t + n_h) * TPS;
         for (uint32_t t = threadIdx.x; t < ntask; t += nthr) {
-            const uint32_t slot = t / C4;
-            const uint32_t q4   = t % C4;
+            const uint32_t slot = t / TPS;
+            const uint32_t qp   = t % TPS;
             const uint32_t side = slot < n_t ? 0U : 1U;
             const uint32_t k    = side == 0 ? lo_t + slot : lo_h + (slot - n_t);
             float2* dst         = s_rows + (side * a.cap + (side == 0 ? slot : slot - n_t)) * STRIDE;
             using CI            = chunk4_t<TI>;
             const TI* in        = static_cast<const TI*>(a.in);
-            float4 e, v;
-            uint32_t b0 = 4 * q4;
+            float4 e[CH], v[CH];
+            uint32_t b0 = 0;  // sums of chunk q4 go to bins 4*q4 + i + b0
             if (!a.grand) {
                 const uint64_t r = (side * a.n_p1) + k;
                 const CI* src    = reinterpret_cast<const CI*>(in + r * (2 * NB));
@@ -275,8 +280,11 @@ k_final2(Final2Args a) {
                     se = bfp_row_scale(a.ein, r, 0);
                     sv = bfp_row_scale(a.ein, r, 1);
                 }
-                e = chunk4_st<TI>(src, q4, se);
-                v = chunk4_st<TI>(src + C4, q4, sv);
+#pragma unroll
+                for (uint32_t h = 0; h < CH; ++h) {
+                    e[h] = chunk4_st<TI>(src, qp + h * TPS, se);
+                    v[h] = chunk4_st<TI>(src + C4, qp + h * TPS, sv);
+                }
             } else {
                 uint32_t it, st, ih, sh;
                 parent_record<NP>(a, k, it, st, ih, sh, NB);
@@ -294,67 +302,101 @@ k_final2(Final2Args a) {
                 // parent[j] = tail[j - St] + head[j - Sh]: the tail is read
                 // unrotated and the sums placed at j = 4*q4 + i + St, so only
                 // the head is rotated (by Sh - St)
-                const uint32_t d  = wrap<NB>(sh + NB - st);
-                const float4 te   = chunk4_st<TI>(trow, q4, ste);
-                const float4 tv   = chunk4_st<TI>(trow + C4, q4, stv);
-                const float4 he   = rotated4_st<NB, TI>(hrow, q4, d, she);
-                const float4 hv   = rotated4_st<NB, TI>(hrow + C4, q4, d, shv);
-                e.x = te.x + he.x;
-                e.y = te.y + he.y;
-                e.z = te.z + he.z;
-                e.w = te.w + he.w;
-                v.x = tv.x + hv.x;
-                v.y = tv.y + hv.y;
-                v.z = tv.z + hv.z;
-                v.w = tv.w + hv.w;
+                const uint32_t d = wrap<NB>(sh + NB - st);
+#pragma unroll
+                for (uint32_t h = 0; h < CH; ++h) {
+                    const uint32_t q4 = qp + h * TPS;
+                    const float4 te   = chunk4_st<TI>(trow, q4, ste);
+                    const float4 tv   = chunk4_st<TI>(trow + C4, q4, stv);
+                    const float4 he   = rotated4_st<NB, TI>(hrow, q4, d, she);
+                    const float4 hv   = rotated4_st<NB, TI>(hrow + C4, q4, d, shv);
+                    e[h].x = te.x + he.x;
+                    e[h].y = te.y + he.y;
+                    e[h].z = te.z + he.z;
+                    e[h].w = te.w + he.w;
+                    v[h].x = tv.x + hv.x;
+                    v[h].y = tv.y + hv.y;
+                    v[h].z = tv.z + hv.z;
+                    v[h].w = tv.w + hv.w;
+                    // one chunk's loads in flight at a time (60 VGPRs, not 76)
+                    __builtin_amdgcn_sched_barrier(0);
+                }
                 bool bfp_parent = kBlockFP<TI>;
                 if constexpr (std::is_same_v<TI, float>) {
                     bfp_parent = a.pbfp != 0;  // FP32 brute fold under a BFP16 parent
                 }
                 if (bfp_parent) {
                     // the parent level as stored: row exponents from the row
-                    // maxima over the C4 consecutive tasks (lanes) of this row
-                    const int ee = bfp_exponent(group_absmax<C4>(__builtin_fmaxf(
-                        __builtin_fmaxf(__builtin_fabsf(e.x), __builtin_fabsf(e.y)),
-                        __builtin_fmaxf(__builtin_fabsf(e.z), __builtin_fabsf(e.w)))));
-                    const int ev = bfp_exponent(group_absmax<C4>(__builtin_fmaxf(
-                        __builtin_fmaxf(__builtin_fabsf(v.x), __builtin_fabsf(v.y)),
-                        __builtin_fmaxf(__builtin_fabsf(v.z), __builtin_fabsf(v.w)))));
+                    // maxima over the TPS consecutive tasks (lanes) of this row
+                    float me = 0.0F, mv = 0.0F;
+#pragma unroll
+                    for (uint32_t h = 0; h < CH; ++h) {
+                        me = __builtin_fmaxf(
+                            me, __builtin_fmaxf(
+                                    __builtin_fmaxf(__builtin_fabsf(e[h].x), __builtin_fabsf(e[h].y)),
+                                    __builtin_fmaxf(__builtin_fabsf(e[h].z), __builtin_fabsf(e[h].w))));
+                        mv = __builtin_fmaxf(
+                            mv, __builtin_fmaxf(
+                                    __builtin_fmaxf(__builtin_fabsf(v[h].x), __builtin_fabsf(v[h].y)),
+                                    __builtin_fmaxf(__builtin_fabsf(v[h].z), __builtin_fabsf(v[h].w))));
+                    }
+                    const int ee = bfp_exponent(group_absmax<TPS>(me));
+                    const int ev = bfp_exponent(group_absmax<TPS>(mv));
                     const float ie = pow2f(-ee), se = pow2f(ee);
                     const float iv = pow2f(-ev), sv = pow2f(ev);
-                    e.x = round_bfp(e.x, ie, se);
-                    e.y = round_bfp(e.y, ie, se);
-                    e.z = round_bfp(e.z, ie, se);
-                    e.w = round_bfp(e.w, ie, se);
-                    v.x = round_bfp(v.x, iv, sv);
-                    v.y = round_bfp(v.y, iv, sv);
-                    v.z = round_bfp(v.z, iv, sv);
-                    v.w = round_bfp(v.w, iv, sv);
-                } else if constexpr (kScaledHalf<TI>) {
-                    e.x = round_stored<TI>(e.x * a.scale1);
-                    e.y = round_stored<TI>(e.y * a.scale1);
-                    e.z = round_stored<TI>(e.z * a.scale1);
-                    e.w = round_stored<TI>(e.w * a.scale1);
-                    v.x = round_stored<TI>(v.x * a.scale1);
-                    v.y = round_stored<TI>(v.y * a.scale1);
-                    v.z = round_stored<TI>(v.z * a.scale1);
-                    v.w = round_stored<TI>(v.w * a.scale1);
-                }
-                b0 += st;
-            }
-            const float ev[4][2] = {{e.x, v.x}, {e.y, v.y}, {e.z, v.z}, {e.w, v.w}};
 #pragma unroll
-            for (uint32_t i = 0; i < 4; ++i) {
-                const uint32_t b = wrap<NB>(b0 + i);
-                const float2 pr  = make_float2(ev[i][0], ev[i][1]);
-                dst[b]           = pr;
-                if (b < kPad) {
-                    dst[NB + b] = pr;
+                    for (uint32_t h = 0; h < CH; ++h) {
+                        e[h].x = round_bfp(e[h].x, ie, se);
+                        e[h].y = round_bfp(e[h].y, ie, se);
+                        e[h].z = round_bfp(e[h].z, ie, se);
+                        e[h].w = round_bfp(e[h].w, ie, se);
+                        v[h].x = round_bfp(v[h].x, iv, sv);
+                        v[h].y = round_bfp(v[h].y, iv, sv);
+                        v[h].z = round_bfp(v[h].z, iv, sv);
+                        v[h].w = round_bfp(v[h].w, iv, sv);
+                    }
+                } else if constexpr (kScaledHalf<TI>) {
+#pragma unroll
+                    for (uint32_t h = 0; h < CH; ++h) {
+                        e[h].x = round_stored<TI>(e[h].x * a.scale1);
+                        e[h].y = round_stored<TI>(e[h].y * a.scale1);
+                        e[h].z = round_stored<TI>(e[h].z * a.scale1);
+                        e[h].w = round_stored<TI>(e[h].w * a.scale1);
+                        v[h].x = round_stored<TI>(v[h].x * a.scale1);
+                        v[h].y = round_stored<TI>(v[h].y * a.scale1);
+                        v[h].z = round_stored<TI>(v[h].z * a.scale1);
+                        v[h].w = round_stored<TI>(v[h].w * a.scale1);
+                    }
                 }
+                b0 = st;
+            }
+#pragma unroll
+            for (uint32_t h = 0; h < CH; ++h) {
+                const float ev[4][2] = {{e[h].x, v[h].x}, {e[h].y, v[h].y}, {e[h].z, v[h].z},
+                                        {e[h].w, v[h].w}};
+                const uint32_t bq    = 4 * (qp + h * TPS) + b0;
+#pragma unroll
+                for (uint32_t i = 0; i < 4; ++i) {
+                    const uint32_t b = wrap<NB>(bq + i);
+                    const float2 pr  = make_float2(ev[i][0], ev[i][1]);
+                    dst[b]           = pr;
+                    if (b < kPad) {
+                        dst[NB + b] = pr;
+                    }
+                }
+                const uint32_t lo = 0x0F800000U;  // 2^-96
+                const uint32_t hi = 0x7E800000U;  // 2^126
+                const uint32_t m  = max(max(__float_as_uint(v[h].x) - lo, __float_as_uint(v[h].y) - lo),
+                                        max(__float_as_uint(v[h].z) - lo, __float_as_uint(v[h].w) - lo));
+                unsafe            = unsafe || m >= hi - lo;
             }
         }
     }
-    __syncthreads();
+    // every trial of the block sums two staged variances: with all of them in
+    // [2^-96, 2^126) each sum is in [2^-96, 2^127), inside the short sequences'
+    // range, and the per-chunk range checks are skipped (block-uniform)
+    const bool fast_norm =
+        __builtin_amdgcn_readfirstlane(__syncthreads_or(unsafe ? 1 : 0) == 0 ? 1 : 0) != 0;
 
 #pragma unroll 1
     for (uint32_t k = 0; k < RPW; ++k) {
@@ -371,12 +413,18 @@ k_final2(Final2Args a) {
         const float2* trow = s_rows + (tw & 0xFFU) * STRIDE;
         const float2* hrow = s_rows + (a.cap + ((tw >> 16) & 0xFFU)) * STRIDE;
         float ps[NB];
-        {
+        // FAST: every variance sum of the block is inside the short exact
+        // sequences' range (no per-chunk range check)
+        auto prefix = [&]<bool FAST>() {
             float acc = 0.0F;
 #pragma unroll
             for (uint32_t c = 0; c < C4; ++c) {
-                const float2* tp = trow + wrap<NB>(4 * c + NB - St);
-                const float2* hp = hrow + wrap<NB>(4 * c + NB - Sh);
+                // this chunk's read offsets formed here, in each of the two
+                // instances (not hoisted ahead of the branch for all chunks)
+                uint32_t st = St, sh = Sh;
+                __asm__ volatile("" : "+v"(st), "+v"(sh));
+                const float2* tp = trow + wrap<NB>(4 * c + NB - st);
+                const float2* hp = hrow + wrap<NB>(4 * c + NB - sh);
                 const float2 t0 = tp[0], t1 = tp[1], t2 = tp[2], t3 = tp[3];
                 const float2 h0 = hp[0], h1 = hp[1], h2 = hp[2], h3 = hp[3];
                 float4 e, v;
@@ -398,21 +446,34 @@ k_final2(Final2Args a) {
                     v.z *= a.unscale;
                     v.w *= a.unscale;
                 }
-                const float4 p = normalise4v<EXACT>(e, v);
-                acc            = (c == 0) ? p.x : acc + p.x;
-                ps[4 * c]      = acc;
-                acc            = acc + p.y;
-                ps[4 * c + 1]  = acc;
-                acc            = acc + p.z;
-                ps[4 * c + 2]  = acc;
-                acc            = acc + p.w;
-                ps[4 * c + 3]  = acc;
+                float4 p;
+                if constexpr (FAST) {
+                    p.x = e.x * rcp_rn_normal(sqrt_rn_normal(v.x));
+                    p.y = e.y * rcp_rn_normal(sqrt_rn_normal(v.y));
+                    p.z = e.z * rcp_rn_normal(sqrt_rn_normal(v.z));
+                    p.w = e.w * rcp_rn_normal(sqrt_rn_normal(v.w));
+                } else {
+                    p = normalise4v<EXACT>(e, v);
+                }
+                acc           = (c == 0) ? p.x : acc + p.x;
+                ps[4 * c]     = acc;
+                acc           = acc + p.y;
+                ps[4 * c + 1] = acc;
+                acc           = acc + p.z;
+                ps[4 * c + 2] = acc;
+                acc           = acc + p.w;
+                ps[4 * c + 3] = acc;
                 // materialise this chunk's prefix values before the next chunk's
                 // reads: left pending (the compiler sinks the products and adds),
                 // every chunk's operands stay live and the kernel needs 256 VGPRs
                 __asm__ volatile("" : "+v"(ps[4 * c]), "+v"(ps[4 * c + 1]), "+v"(ps[4 * c + 2]),
                                  "+v"(ps[4 * c + 3]), "+v"(acc));
             }
+        };
+        if (EXACT && fast_norm) {
+            prefix.template operator()<true>();
+        } else {
+            prefix.template operator()<false>();
         }
 
         // 3. boxcar S/N over the widths (score.cpp:95-172), windows in registers
