@@ -254,12 +254,12 @@ k_final2(Final2Args a) {
         return;
     }
 
-    // 1. stage the parent rows: task (row slot, chunks qp and qp + TPS of 4
-    //    bins); `unsafe` marks a staged variance outside [2^-96, 2^126), the
+    // 1. stage the parent rows: task (row slot, CH chunks qp, qp + TPS, ... of
+    //    4 bins); `unsafe` marks a staged variance outside [2^-96, 2^126), the
     //    range whose pairwise sums the short exact normalisation takes
-    constexpr uint32_t CH  = 2;         // 4-bin chunks per task
+    constexpr uint32_t CH  = 1;         // 4-bin chunks per task (2: no faster)
     constexpr uint32_t TPS = C4 / CH;   // tasks per staged row
-    bool unsafe = kScaledHalf<TI>;      // scaled storage: sums are rescaled first
+    bool unsafe = !EXACT || kScaledHalf<TI>;  // scaled storage: sums are rescaled first
     {
         const uint32_t ntask = (n_t + n_h) * TPS;
         for (uint32_t t = threadIdx.x; t < ntask; t += nthr) {
@@ -318,7 +318,8 @@ k_final2(Final2Args a) {
                     v[h].y = tv.y + hv.y;
                     v[h].z = tv.z + hv.z;
                     v[h].w = tv.w + hv.w;
-                    // one chunk's loads in flight at a time (60 VGPRs, not 76)
+                    // one chunk's loads in flight at a time (with CH = 2: 60
+                    // VGPRs, not 76)
                     __builtin_amdgcn_sched_barrier(0);
                 }
                 bool bfp_parent = kBlockFP<TI>;
@@ -384,6 +385,9 @@ k_final2(Final2Args a) {
                         dst[NB + b] = pr;
                     }
                 }
+                if constexpr (!EXACT) {
+                    continue;
+                }
                 const uint32_t lo = 0x0F800000U;  // 2^-96
                 const uint32_t hi = 0x7E800000U;  // 2^126
                 const uint32_t m  = max(max(__float_as_uint(v[h].x) - lo, __float_as_uint(v[h].y) - lo),
@@ -395,8 +399,13 @@ k_final2(Final2Args a) {
     // every trial of the block sums two staged variances: with all of them in
     // [2^-96, 2^126) each sum is in [2^-96, 2^127), inside the short sequences'
     // range, and the per-chunk range checks are skipped (block-uniform)
-    const bool fast_norm =
-        __builtin_amdgcn_readfirstlane(__syncthreads_or(unsafe ? 1 : 0) == 0 ? 1 : 0) != 0;
+    bool fast_norm = false;
+    if constexpr (EXACT) {
+        fast_norm =
+            __builtin_amdgcn_readfirstlane(__syncthreads_or(unsafe ? 1 : 0) == 0 ? 1 : 0) != 0;
+    } else {
+        __syncthreads();
+    }
 
 #pragma unroll 1
     for (uint32_t k = 0; k < RPW; ++k) {
