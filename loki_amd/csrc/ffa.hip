@@ -178,6 +178,28 @@ namespace {
 
 void check_launch(const char* what) { hip_check(hipGetLastError(), what); }
 
+/// Resolver terms of grid row `row` (np 2 or 3), one side; dv as in
+/// FinalGrid.
+kern::RowTerms row_terms_of(const kern::ResolveLevel& lv, const double* dv, uint32_t row,
+                            int side) {
+    return lv.np == 2 ? kern::row_terms<2>(lv, dv, row, side)
+                      : kern::row_terms<3>(lv, dv, row, side);
+}
+
+/// Every grid row's (tail, head) terms of one level, np 2 or 3.
+void level_row_terms(const kern::ResolveLevel& lv, std::vector<kern::RowTerms>& out) {
+    double dv[3];
+    for (int k = 0; k < 3; ++k) {
+        dv[k] = (lv.hi[k] - lv.lo[k]) / static_cast<double>(lv.n_cur[k]);
+    }
+    const uint64_t nrows = static_cast<uint64_t>(lv.n_cur[0]) * lv.n_cur[1];
+    for (uint64_t r = 0; r < nrows; ++r) {
+        for (int side = 0; side < 2; ++side) {
+            out.push_back(row_terms_of(lv, dv, static_cast<uint32_t>(r), side));
+        }
+    }
+}
+
 }  // namespace
 
 FFAExecutor::FFAExecutor(const SearchConfig& cfg, FoldKind kind, int device_id,
@@ -211,6 +233,21 @@ SizeType FFAExecutor::coord_record_bytes() const noexcept {
 void FFAExecutor::resolve_coordinates(void* coords_dev, hipStream_t s, SizeType last_level) {
     auto* base = static_cast<char*>(coords_dev);
     const SizeType top = std::min(last_level, m_plan.n_levels() - 1);
+    if (!m_freq_only && !m_rt_ready) {
+        // the frequency-independent resolver terms of every level's grid rows
+        m_rt_host.clear();
+        m_rt_off.assign(m_plan.n_levels(), 0);
+        for (SizeType l = 1; l < m_plan.n_levels(); ++l) {
+            m_rt_off[l] = m_rt_host.size();
+            level_row_terms(resolve_level(m_plan, l), m_rt_host);
+        }
+        m_rt.reserve(std::max<SizeType>(m_rt_host.size(), 1) * sizeof(kern::RowTerms));
+        hip_check(hipMemcpyAsync(m_rt.get(), m_rt_host.data(),
+                                 m_rt_host.size() * sizeof(kern::RowTerms),
+                                 hipMemcpyHostToDevice, s),
+                  "copy resolver row terms");
+        m_rt_ready = true;
+    }
     for (SizeType l = 1; l <= top; ++l) {
         const auto lv      = resolve_level(m_plan, l);
         const SizeType off = m_plan.ncoords_offsets64()[l];
@@ -219,7 +256,8 @@ void FFAExecutor::resolve_coordinates(void* coords_dev, hipStream_t s, SizeType 
         if (m_freq_only) {
             kern::resolve_freq(lv, reinterpret_cast<FFACoordFreq*>(base) + off, n, s);
         } else {
-            kern::resolve_poly(lv, reinterpret_cast<FFACoord*>(base) + off, n, s);
+            kern::resolve_poly(lv, m_rt.as<kern::RowTerms>() + m_rt_off[l],
+                               reinterpret_cast<FFACoord*>(base) + off, n, s);
         }
         check_launch("resolver launch");
         m_timer.stop(slot, s);
@@ -509,12 +547,6 @@ FFACoord resolve_rc(const kern::ResolveLevel& lv, const double* dv, uint32_t row
     case 2: return kern::resolve_pair_rc<2>(lv, dv, row, jf);
     default: return kern::resolve_pair_rc<3>(lv, dv, row, jf);
     }
-}
-
-kern::RowTerms row_terms_of(const kern::ResolveLevel& lv, const double* dv, uint32_t row,
-                            int side) {
-    return lv.np == 2 ? kern::row_terms<2>(lv, dv, row, side)
-                      : kern::row_terms<3>(lv, dv, row, side);
 }
 
 }  // namespace

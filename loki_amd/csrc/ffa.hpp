@@ -186,6 +186,11 @@ private:
     DeviceBuffer m_f2_chunks;
     std::vector<kern::RowTerms> m_f2_rows_host;  // (tail, head) per grid row, np >= 2
     DeviceBuffer m_f2_rows;
+    // resolver row terms of every level (np >= 2), built once per plan
+    std::vector<kern::RowTerms> m_rt_host;
+    std::vector<SizeType> m_rt_off;  // records into m_rt_host per level
+    DeviceBuffer m_rt;
+    bool m_rt_ready{false};
     uint32_t m_f2_rows_per_block{1};
     uint32_t m_f2_cap{0};
     uint32_t m_f2_nblocks{0};

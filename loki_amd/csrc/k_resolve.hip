@@ -2,7 +2,9 @@
 //
 // Restates taylor_ffa.cpp:249-271 (freq-only) and taylor_ffa.cpp:19-188
 // (accel / jerk, tail and head) with the same operation order; compiled with
-// -ffp-contract=off so the tables are bit-identical to the FP64 oracle.
+// -ffp-contract=off so the tables are bit-identical to the FP64 oracle.  The
+// accel / jerk tables take each grid row's frequency-independent terms from a
+// per-level table formed once on the host by the same host/device code.
 #include "resolve_device.hpp"
 
 namespace loki_amd::kern {
@@ -22,15 +24,21 @@ k_resolve_freq(ResolveLevel lv, FFACoordFreq* __restrict__ out, uint64_t n) {
     out[i]  = c;
 }
 
-template <int NP>
+// One workgroup per (grid row, 256 frequencies): the row's terms are
+// workgroup-uniform scalar loads and each lane resolves one frequency.
 __global__ void __launch_bounds__(256)
-k_resolve_poly(ResolveLevel lv, FFACoord* __restrict__ out, uint64_t n) {
-    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) {
+k_resolve_poly(ResolveLevel lv, FinalGrid fg, const RowTerms* __restrict__ rows,
+               FFACoord* __restrict__ out, uint32_t tiles_per_row) {
+    const uint32_t row  = blockIdx.x / tiles_per_row;
+    const uint32_t tile = blockIdx.x - row * tiles_per_row;
+    const uint32_t jf   = tile * 256 + threadIdx.x;
+    const uint32_t n_f  = lv.n_cur[2];
+    if (jf >= n_f) {
         return;
     }
-    const FFACoord c = resolve_coord<NP>(lv, i);
-    out[i] = c;
+    out[static_cast<uint64_t>(row) * n_f + jf] =
+        resolve_pair_rt(lv, fg.dv, rows[2 * static_cast<size_t>(row)],
+                        rows[2 * static_cast<size_t>(row) + 1], jf);
 }
 
 __device__ __forceinline__ uint32_t find_task_resolve(const ResolveTask* __restrict__ tasks,
@@ -85,18 +93,23 @@ void resolve_freq(const ResolveLevel& lv, FFACoordFreq* out, uint64_t ncoords, h
     k_resolve_freq<<<grid_for(ncoords, 256), 256, 0, s>>>(lv, out, ncoords);
 }
 
-void resolve_poly(const ResolveLevel& lv, FFACoord* out, uint64_t ncoords, hipStream_t s) {
+void resolve_poly(const ResolveLevel& lv, const RowTerms* rows, FFACoord* out, uint64_t ncoords,
+                  hipStream_t s) {
     LOKI_REQUIRE(lv.np == 2 || lv.np == 3, "resolve_poly needs np in {2, 3}");
     LOKI_REQUIRE(ncoords == static_cast<uint64_t>(lv.n_cur[0]) * lv.n_cur[1] * lv.n_cur[2],
                  "resolve_poly: coords size mismatch");
+    LOKI_REQUIRE(rows != nullptr, "resolve_poly: row terms missing");
     if (ncoords == 0) {
         return;
     }
-    if (lv.np == 2) {
-        k_resolve_poly<2><<<grid_for(ncoords, 256), 256, 0, s>>>(lv, out, ncoords);
-    } else {
-        k_resolve_poly<3><<<grid_for(ncoords, 256), 256, 0, s>>>(lv, out, ncoords);
+    FinalGrid fg{};
+    for (int k = 0; k < 3; ++k) {
+        fg.dv[k] = (lv.hi[k] - lv.lo[k]) / static_cast<double>(lv.n_cur[k]);
     }
+    const uint64_t nrows = static_cast<uint64_t>(lv.n_cur[0]) * lv.n_cur[1];
+    const uint32_t tpr   = (lv.n_cur[2] + 255) / 256;
+    const unsigned grid  = grid_for(nrows * tpr * 256, 256);
+    k_resolve_poly<<<grid, 256, 0, s>>>(lv, fg, rows, out, tpr);
 }
 
 void resolve_grouped(const ResolveTask* tasks, uint32_t ntasks, uint32_t nblocks, uint32_t np,

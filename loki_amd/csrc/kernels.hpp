@@ -45,8 +45,10 @@ struct FinalGrid {
 
 /// Fills ncoords FFACoordFreq records (np == 1) of one level.
 void resolve_freq(const ResolveLevel& lv, FFACoordFreq* out, uint64_t ncoords, hipStream_t s);
-/// Fills ncoords FFACoord records (np == 2 or 3) of one level, tail and head.
-void resolve_poly(const ResolveLevel& lv, FFACoord* out, uint64_t ncoords, hipStream_t s);
+/// Fills ncoords FFACoord records (np == 2 or 3) of one level, tail and head,
+/// from the level's row terms `rows` (device; tail and head per grid row).
+void resolve_poly(const ResolveLevel& lv, const RowTerms* rows, FFACoord* out, uint64_t ncoords,
+                  hipStream_t s);
 
 /// Bucket lists of the time-domain brute fold, built once per plan: for
 /// each frequency f and phase bin b, the sample indices of the bin in
