@@ -38,6 +38,9 @@ MFMA_F32_PEAK_TFLOPS = 157.3
 SIMDS = 1024
 CLOCK_HZ = 2.4e9
 VALU_LANE_INST_PEAK_T = SIMDS * 16 * 2 * CLOCK_HZ / 1e12
+# LDS: 128 bytes per clock per CU (the CDNA3 figure; if gfx950 reads faster, the fraction
+# below overstates the LDS utilisation) -> 78.6 TB/s over 256 CUs.
+LDS_PEAK_TBPS = 256 * 128 * CLOCK_HZ / 1e12
 WIDTHS = (1, 2, 3, 4, 6, 8, 12, 16, 24, 32)
 
 WORKLOADS = {
@@ -513,12 +516,17 @@ def run_batch(args, wl) -> int:
     torch.cuda.synchronize()
     ms_per_step = (time.perf_counter() - t0) * 1e3 / args.steps
     trials = int(info["total_trials"])
-    # algorithmic bytes of the whole batch (per-series pipeline bytes, FP32 time folds)
+    # algorithmic bytes of the whole batch (per-series pipeline bytes, FP32 time folds), and
+    # the level-0 brute fold's (sample, frequency) pairs: each reads one staged (e, v)
+    # float pair from LDS
     alg = 0
+    pairs = 0
     for i in range(batch.size):
         plan = batch.plan(i)
         nb = int(cfgs[i].nbins)
         alg += pipeline_bytes(plan, nb, int((widths < nb).sum()), int(cfgs[i].nsamps))
+        pairs += int(cfgs[i].nsamps) * int(plan.ncoords[0])
+    lds_bytes = 8 * pairs
     # recovery: best S/N of the injected series at its true period
     hits = 0
     for i, sp in enumerate(specs):
@@ -546,12 +554,19 @@ def run_batch(args, wl) -> int:
             "schedule": schedule,
         },
         "roofline": {
-            "bound": "hbm", "kernel": "batch_pipeline",
-            "achieved": alg / (ms_per_step * 1e-3) / 1e9, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-            "frac": alg / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBPS, "traffic": None,
-            "alg_bytes_per_launch": alg,
-            "note": "whole batch (grouped launches in one graph) against the sum of the series' "
-                    "algorithmic bytes",
+            "bound": "lds", "kernel": "k_bf_units_grouped (level-0 brute fold)",
+            "achieved": lds_bytes / (ms_per_step * 1e-3) / 1e12, "peak": LDS_PEAK_TBPS,
+            "unit": "TB/s (LDS)",
+            "frac": lds_bytes / (ms_per_step * 1e-3) / 1e12 / LDS_PEAK_TBPS, "traffic": None,
+            "lds_bytes_per_step": lds_bytes, "sample_freq_pairs": pairs,
+            "note": "8 LDS bytes per (sample, frequency) pair of the brute fold, which takes 95 % "
+                    "of the step (profiles/batched_uneven_kernel_trace.md; its SQ counters in "
+                    "profiles/batched_uneven_brute_fold_sq_counters.md), over the whole step's "
+                    "time, so the fold's own fraction is about 5 % higher; LDS peak 128 B per "
+                    "clock per CU",
+            "hbm": {"alg_bytes_per_step": alg,
+                    "achieved_GBps": alg / (ms_per_step * 1e-3) / 1e9,
+                    "frac": alg / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBPS},
         },
         "graph_vs_eager_ms": {"graph": ms_per_step, "eager": eager_ms},
         "recovered": {"signals": nsig, "found_within_2_cells": hits},
