@@ -288,12 +288,18 @@ k_final2(Final2Args a) {
             } else {
                 uint32_t it, st, ih, sh;
                 parent_record<NP>(a, k, it, st, ih, sh, NB);
-                const uint64_t rt = ((2 * side) * a.n_p2) + it;
-                const uint64_t rh = ((2 * side + 1) * a.n_p2) + ih;
-                const CI* trow    = reinterpret_cast<const CI*>(in + rt * (2 * NB));
-                const CI* hrow    = reinterpret_cast<const CI*>(in + rh * (2 * NB));
+                // rows (2 side) and (2 side + 1) of the grandparent level: a
+                // side's tail half selected once, each row one 64-bit multiply-add
+                const uint64_t half = a.n_p2 * (2 * NB);
+                const TI* tbase     = in + (side != 0 ? 2 * half : 0);
+                const CI* trow =
+                    reinterpret_cast<const CI*>(tbase + static_cast<uint64_t>(it) * (2 * NB));
+                const CI* hrow = reinterpret_cast<const CI*>(
+                    tbase + half + static_cast<uint64_t>(ih) * (2 * NB));
                 float ste = 1.0F, stv = 1.0F, she = 1.0F, shv = 1.0F;
                 if constexpr (kBlockFP<TI>) {
+                    const uint64_t rt = ((2 * side) * a.n_p2) + it;
+                    const uint64_t rh = ((2 * side + 1) * a.n_p2) + ih;
                     ste = bfp_row_scale(a.ein, rt, 0);
                     stv = bfp_row_scale(a.ein, rt, 1);
                     she = bfp_row_scale(a.ein, rh, 0);
