@@ -506,7 +506,7 @@ k_final2(Final2Args a) {
             }
             const float dmax =
                 window_max_reg<NB>(std::make_integer_sequence<int, NB - 1>{}, ps, a.wt.w[iw]);
-            const float snr = ((a.wt.h[iw] + a.wt.b[iw]) * dmax) - (a.wt.b[iw] * sum);
+            const float snr = (a.wt.hb[iw] * dmax) - (a.wt.b[iw] * sum);
             if constexpr (FINDMAX) {
                 best = (best < snr) ? snr : best;
             } else if (a.fo.count != nullptr) {

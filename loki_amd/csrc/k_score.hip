@@ -205,6 +205,7 @@ WidthTable make_width_table(const SizeType* widths, SizeType nw, SizeType nbins)
         wt.w[i] = static_cast<uint32_t>(w);
         wt.h[i] = std::sqrt(static_cast<float>(nbins - w) / static_cast<float>(nbins * w));
         wt.b[i] = static_cast<float>(w) * wt.h[i] / static_cast<float>(nbins - w);
+        wt.hb[i] = wt.h[i] + wt.b[i];
         wt.wmax = std::max(wt.wmax, wt.w[i]);
     }
     return wt;

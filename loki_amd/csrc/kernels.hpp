@@ -192,6 +192,7 @@ struct WidthTable {
     uint32_t w[kMax];
     float h[kMax];
     float b[kMax];
+    float hb[kMax];  ///< h + b, the same FP32 sum the kernels would form
 };
 WidthTable make_width_table(const SizeType* widths, SizeType nw, SizeType nbins);
 
